@@ -1,0 +1,250 @@
+/*
+ * msbfs.h -- C ABI of the MI355X multi-source BFS engine (libmsbfs.so).
+ *
+ * This is the drop-in boundary that replaces the reference's wgpu runtime
+ * (crates/gpu_bfs/src/{context,buffer,pipeline}.rs) and WGSL pipelines
+ * (the WGSL files under crates/shaders/src/) underneath the `gpu_bfs` engine API.  All
+ * signatures take plain pointers and sizes; device memory and streams are
+ * passed as opaque `void*` (a hipStream_t may be NULL for the library's own
+ * stream).  Every entry point returns an msbfs_status; the message of the
+ * last failure on the calling thread is available from msbfs_last_error().
+ *
+ * Distances use the reference's layout and sentinels
+ * (crates/core/src/lib.rs:21-39): dist[s*|V| + v], 0 at the source itself,
+ * 0xFFFFFFFF (u32) or 0xFFFF (u16) when unreachable.
+ */
+#ifndef MSBFS_H
+#define MSBFS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  The acceptance rules follow the reference engine:
+ * S == 0 -> OK with empty output (crates/gpu_bfs/src/engine.rs:228-231);
+ * |V| == 0 with S > 0 -> MSBFS_ERR_SOURCE_OOB (engine.rs:233-238);
+ * a source >= |V| -> MSBFS_ERR_SOURCE_OOB (engine.rs:240-249). */
+typedef enum msbfs_status {
+    MSBFS_OK = 0,
+    MSBFS_ERR_INVALID_ARG = 1,
+    MSBFS_ERR_SOURCE_OOB = 2,
+    MSBFS_ERR_EMPTY_GRAPH = 3,
+    MSBFS_ERR_OUT_TOO_SMALL = 4,
+    MSBFS_ERR_HIP = 5,
+    MSBFS_ERR_OOM = 6,
+    MSBFS_ERR_DIAMETER_U16 = 7, /* a distance >= 0xFFFF cannot be stored as u16 */
+    MSBFS_ERR_INVALID_GRAPH = 8,
+    MSBFS_ERR_UNSUPPORTED = 9,
+    MSBFS_ERR_NO_DEVICE = 10,
+    MSBFS_ERR_BUSY = 11 /* serving: admission refused, retry later */
+} msbfs_status;
+
+/* Method selector: same names and order as GpuBfsMethod
+ * (crates/gpu_bfs/src/engine.rs:33-54).  Each method maps onto one of this
+ * library's traversal families; all of them return identical distances. */
+typedef enum msbfs_method {
+    MSBFS_METHOD_NAIVE_SEQUENTIAL = 0,      /* out of scope: MSBFS_ERR_UNSUPPORTED */
+    MSBFS_METHOD_BATCHED_DENSE_SPMM = 1,    /* out of scope: MSBFS_ERR_UNSUPPORTED */
+    MSBFS_METHOD_BIT_PARALLEL_32 = 2,       /* wide family, push only */
+    MSBFS_METHOD_BIT_PARALLEL_64 = 3,       /* wide family, push only */
+    MSBFS_METHOD_BIT_PARALLEL_64_PULL = 4,  /* wide family, pull only */
+    MSBFS_METHOD_BOOLEAN_REACHABILITY = 5,  /* wide family, push only */
+    MSBFS_METHOD_PUSH_PULL_ADAPTIVE = 6,    /* wide family, direction switch */
+    MSBFS_METHOD_SPARSE_QUEUE = 7,          /* wide family, push only */
+    MSBFS_METHOD_SPARSE_BIT_PARALLEL_64 = 8,/* wide family, push only */
+    MSBFS_METHOD_MULTI_WORD_BIT_PARALLEL = 9,/* wide family, pull only */
+    MSBFS_METHOD_INDIRECT_MACRO_BATCH = 10, /* wide family, pull only */
+    MSBFS_METHOD_INDIRECT_SPARSE_QUEUE_PUSH = 11, /* wide family, direction switch */
+    MSBFS_METHOD_MULTI_WORD_BIT_PARALLEL_PUSH = 12, /* wide family, push only */
+    MSBFS_METHOD_WORKGROUP_PERSISTENT_BFS = 13,     /* deep family (LDS-resident) */
+    MSBFS_METHOD_AUTO = 14
+} msbfs_method;
+
+typedef enum msbfs_dist_type {
+    MSBFS_DIST_U32 = 0,
+    MSBFS_DIST_U16 = 1
+} msbfs_dist_type;
+
+/* Traversal families (what a method resolves to). */
+typedef enum msbfs_family {
+    MSBFS_FAMILY_NONE = 0,
+    MSBFS_FAMILY_DEEP = 1, /* per-workgroup instances, visited bits in LDS */
+    MSBFS_FAMILY_WIDE = 2  /* device-wide level-synchronous push/pull */
+} msbfs_family;
+
+typedef enum msbfs_direction_policy {
+    MSBFS_DIR_ADAPTIVE = 0,  /* push/pull switch (alpha, beta) */
+    MSBFS_DIR_PUSH = 1,
+    MSBFS_DIR_PULL = 2
+} msbfs_direction_policy;
+
+typedef struct msbfs_context msbfs_context;
+typedef struct msbfs_graph msbfs_graph;
+typedef struct msbfs_engine msbfs_engine;
+typedef struct msbfs_batch msbfs_batch;
+typedef struct msbfs_server msbfs_server;
+
+typedef struct msbfs_device_info {
+    char name[64];
+    char arch[32];
+    int32_t compute_units;
+    int32_t lds_bytes_per_block;
+    uint64_t total_mem_bytes;
+    int32_t cooperative_launch;
+    int32_t reserved;
+} msbfs_device_info;
+
+typedef struct msbfs_graph_info {
+    uint64_t num_vertices;
+    uint64_t num_edges;
+    uint32_t max_degree;
+    uint32_t symmetric;   /* 1 when CSC aliases CSR (exact check) */
+    uint64_t device_bytes;
+} msbfs_graph_info;
+
+/* Tuning knobs (the autotuner's search space).  Zero means "default". */
+typedef struct msbfs_tuning {
+    uint32_t alpha;          /* push->pull when m_f > |E|/alpha (push_pull.rs:22-36), default 14 */
+    uint32_t beta;           /* pull->push when n_f < |V|/beta, default 24 */
+    uint32_t tile_words;     /* wide family wave tile: 64/tile_words vertices x tile_words words (1,2,4,8) */
+    uint32_t words_per_lane; /* wide family: words each lane walks per parent (1..4) */
+    uint32_t deep_bits;      /* deep family: sources per instance (1,2,4,8,16,32); 0 = auto */
+    uint32_t deep_threads;   /* deep family workgroup size (256, 512, 1024) */
+    uint32_t wide_blocks_per_cu; /* wide family persistent blocks per CU */
+    uint32_t direction_rule; /* 0: reference estimate n_f*|E|/|V|; 1: exact frontier degree sum */
+} msbfs_tuning;
+
+typedef struct msbfs_run_stats {
+    int32_t method;          /* method actually run (after Auto) */
+    int32_t family;          /* msbfs_family */
+    uint32_t levels;         /* BFS levels executed */
+    uint32_t push_levels;
+    uint32_t pull_levels;
+    uint32_t instances;      /* deep family instances */
+    uint32_t bits_per_instance;
+    uint32_t grid_blocks;
+    float kernel_ms;         /* traversal kernel(s) device time, hipEvents */
+    float total_ms;          /* whole call device time, hipEvents */
+    uint64_t workspace_bytes;
+} msbfs_run_stats;
+
+/* ---- errors / version ---- */
+const char *msbfs_last_error(void);
+const char *msbfs_version(void);
+const char *msbfs_status_string(int status);
+
+/* ---- device context: GpuContext::init_blocking
+ * (crates/gpu_bfs/src/context.rs:174-388) ---- */
+int msbfs_context_create(int device, msbfs_context **out);
+int msbfs_context_destroy(msbfs_context *ctx);
+int msbfs_context_info(const msbfs_context *ctx, msbfs_device_info *info);
+int msbfs_context_synchronize(msbfs_context *ctx);
+
+/* ---- graph residency: replaces PushPullGraphBuffers
+ * (crates/gpu_bfs/src/buffer.rs:96-219).  The CSR is validated like
+ * CsrGraph::validate (crates/core/src/graph.rs:226-278, except that neighbour
+ * lists need not be sorted), uploaded once, and its transpose is built on
+ * the host (graph.rs:90-126) unless the graph is exactly symmetric, in which
+ * case the CSC aliases the CSR.  Handles are explicit: nothing is keyed on
+ * (|V|,|E|). ---- */
+int msbfs_graph_create(msbfs_context *ctx, uint64_t num_vertices, uint64_t num_edges,
+                       const uint32_t *row_offsets, const uint32_t *col_indices,
+                       msbfs_graph **out);
+int msbfs_graph_destroy(msbfs_graph *g);
+int msbfs_graph_get_info(const msbfs_graph *g, msbfs_graph_info *info);
+
+/* ---- engine: GpuBfsEngine::new(context, method)
+ * (crates/gpu_bfs/src/engine.rs:68) ---- */
+int msbfs_engine_create(msbfs_context *ctx, int method, msbfs_engine **out);
+int msbfs_engine_destroy(msbfs_engine *e);
+int msbfs_engine_method(const msbfs_engine *e);
+/* PushPullConfig { alpha, beta } (crates/gpu_bfs/src/dispatch/push_pull.rs:22-36) */
+int msbfs_engine_set_push_pull(msbfs_engine *e, uint32_t alpha, uint32_t beta);
+int msbfs_engine_set_tuning(msbfs_engine *e, const msbfs_tuning *t);
+int msbfs_engine_get_tuning(const msbfs_engine *e, msbfs_tuning *t);
+int msbfs_engine_last_stats(msbfs_engine *e, msbfs_run_stats *st);
+
+/* GpuBfsEngine::select_auto_method (crates/gpu_bfs/src/engine.rs:157-198):
+ * the reference's decision table, returned as an msbfs_method. */
+int msbfs_select_auto_method(uint64_t num_vertices, uint64_t num_edges, uint32_t max_degree,
+                             uint64_t num_sources);
+/* GpuBfsEngine::resolve_method (engine.rs:201-206) followed by this
+ * library's family choice; returns an msbfs_family. */
+int msbfs_engine_resolve(const msbfs_engine *e, const msbfs_graph *g, uint64_t num_sources,
+                         int *method_out);
+
+/* ---- traversal ----
+ * BfsEngine::run_ms_bfs / GpuBfsEngine::run_ms_bfs_into
+ * (crates/core/src/lib.rs:21-39, crates/gpu_bfs/src/engine.rs:219-297).
+ * `out` is host memory of out_len elements (u32 or u16 by dist_type);
+ * out_len must be >= S*|V|. */
+int msbfs_run_ms_bfs(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources,
+                     uint64_t num_sources, int dist_type, void *out, uint64_t out_len);
+/* Same, with the output left in device memory `out_device` and work
+ * enqueued on `stream` (NULL = library stream).  Returns after enqueueing
+ * when `sync` is 0; errors found by the kernels (u16 overflow) are then
+ * reported by msbfs_engine_check(). */
+int msbfs_run_ms_bfs_device(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources,
+                            uint64_t num_sources, int dist_type, void *out_device,
+                            uint64_t out_len, void *stream, int sync);
+int msbfs_engine_check(msbfs_engine *e);
+
+/* GpuBfsEngine::run_ms_reachability (crates/gpu_bfs/src/engine.rs:420-479):
+ * vertex-major bitmask words[v*ceil(S/32) + s/32], bit s%32
+ * (ReachabilityBitmask, crates/core/src/graph.rs:446-646). */
+int msbfs_run_ms_reachability(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources,
+                              uint64_t num_sources, uint32_t *words, uint64_t words_len);
+
+/* ---- host-side graph utilities (crates/core) ----
+ * Library-owned host CSR produced by the generators and converters below;
+ * read it with msbfs_host_graph_get and free it with msbfs_host_graph_free. */
+typedef struct msbfs_host_graph msbfs_host_graph;
+int msbfs_host_graph_get(const msbfs_host_graph *h, uint64_t *num_vertices, uint64_t *num_edges,
+                         const uint32_t **row_offsets, const uint32_t **col_indices);
+void msbfs_host_graph_free(msbfs_host_graph *h);
+
+/* CsrGraph::from_edge_list (crates/core/src/graph.rs:51-80). */
+int msbfs_csr_from_edge_list(uint64_t num_vertices, uint64_t num_pairs, const uint32_t *us,
+                             const uint32_t *vs, msbfs_host_graph **out);
+/* csr_from_undirected_pairs (crates/core/src/generators.rs:547-556). */
+int msbfs_csr_from_undirected_pairs(uint64_t num_vertices, uint64_t num_pairs,
+                                    const uint32_t *us, const uint32_t *vs,
+                                    msbfs_host_graph **out);
+/* CsrGraph::to_csc (graph.rs:90-126): output arrays of V+1 and E entries. */
+int msbfs_csr_to_csc(uint64_t num_vertices, uint64_t num_edges, const uint32_t *row_offsets,
+                     const uint32_t *col_indices, uint32_t *col_offsets, uint32_t *row_indices);
+/* CsrGraph::is_symmetric (graph.rs:129-132): 1 / 0, or a negative status. */
+int msbfs_csr_is_symmetric(uint64_t num_vertices, uint64_t num_edges,
+                           const uint32_t *row_offsets, const uint32_t *col_indices);
+/* CsrGraph::validate (graph.rs:226-278): MSBFS_OK or MSBFS_ERR_INVALID_GRAPH. */
+int msbfs_csr_validate(uint64_t num_vertices, uint64_t num_edges, const uint32_t *row_offsets,
+                       const uint32_t *col_indices);
+
+/* Generators.  Grid2D follows crates/core/src/generators.rs:28-80 exactly.
+ * The random generators restate the reference's algorithms
+ * (R-MAT generators.rs:277-342, G(n,p) Batagelj-Brandes generators.rs:139-191)
+ * over this library's own seeded xoshiro256** streams, so their outputs are
+ * reproducible here but are not the reference's rand 0.8 streams. */
+int msbfs_gen_grid2d(uint64_t width, uint64_t height, msbfs_host_graph **out);
+/* Road-like: `width` columns, num_vertices vertices laid out row-major on
+ * the grid (last row partial); a seeded Kruskal spanning tree over the grid
+ * edges plus `extra_edges` distinct non-tree grid edges. */
+int msbfs_gen_road_like(uint64_t num_vertices, uint64_t width, uint64_t extra_edges,
+                        uint64_t seed, msbfs_host_graph **out);
+int msbfs_gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, double c, double d,
+                   uint64_t seed, int directed, msbfs_host_graph **out);
+int msbfs_gen_erdos_renyi_np(uint64_t num_vertices, double p, uint64_t seed,
+                             msbfs_host_graph **out);
+/* Uniform sources without replacement, sorted ascending; when
+ * non_isolated != 0 only vertices of out-degree > 0 are drawn. */
+int msbfs_sample_sources(const uint32_t *row_offsets, uint64_t num_vertices, uint64_t count,
+                         uint64_t seed, int non_isolated, uint32_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MSBFS_H */

@@ -1,0 +1,740 @@
+// Host engine of libmsbfs: device context, resident graphs, the engine
+// object with its reusable workspace, method routing and kernel launches.
+//
+// Mirrors the reference's crates/gpu_bfs/src/engine.rs (GpuBfsEngine): the
+// same method enum, the same Auto decision table, the same argument checks
+// and error behaviour; underneath, the wgpu buffers/pipelines and the
+// host-driven level loops are replaced by device-resident graphs and
+// persistent HIP kernels that iterate every level on the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+using namespace msbfs;
+
+#define MSBFS_VERSION "0.1.0"
+
+namespace {
+
+int hip_fail(hipError_t e, const char *what) {
+    const int code = (e == hipErrorOutOfMemory) ? MSBFS_ERR_OOM : MSBFS_ERR_HIP;
+    return fail(code, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr)                                      \
+    do {                                                   \
+        hipError_t _e = (expr);                            \
+        if (_e != hipSuccess) return hip_fail(_e, #expr);  \
+    } while (0)
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    // Grows the buffer to at least `need` bytes.  Returns 1 when it was
+    // (re)allocated (contents undefined), 0 when kept, <0 on failure.
+    int ensure(size_t need) {
+        if (need == 0) need = 16;
+        if (bytes >= need) return 0;
+        release();
+        size_t want = need + need / 8;
+        hipError_t e = hipMalloc(&p, want);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return -hip_fail(e, "hipMalloc workspace");
+        }
+        bytes = want;
+        return 1;
+    }
+    template <class T>
+    T *as() const { return (T *)p; }
+};
+
+struct PinnedBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    ~PinnedBuf() {
+        if (p) (void)hipHostFree(p);
+    }
+    int ensure(size_t need) {
+        if (need == 0) need = 16;
+        if (bytes >= need) return 0;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        bytes = 0;
+        hipError_t e = hipHostMalloc(&p, need, hipHostMallocDefault);
+        if (e != hipSuccess) return -hip_fail(e, "hipHostMalloc");
+        bytes = need;
+        return 1;
+    }
+};
+
+constexpr uint32_t kDefaultAlpha = 14;  // push_pull.rs:29-35
+constexpr uint32_t kDefaultBeta = 24;
+
+bool is_pull_only(int m) {
+    return m == MSBFS_METHOD_BIT_PARALLEL_64_PULL || m == MSBFS_METHOD_MULTI_WORD_BIT_PARALLEL ||
+           m == MSBFS_METHOD_INDIRECT_MACRO_BATCH;
+}
+bool is_push_only(int m) {
+    return m == MSBFS_METHOD_BIT_PARALLEL_32 || m == MSBFS_METHOD_BIT_PARALLEL_64 ||
+           m == MSBFS_METHOD_BOOLEAN_REACHABILITY || m == MSBFS_METHOD_SPARSE_QUEUE ||
+           m == MSBFS_METHOD_SPARSE_BIT_PARALLEL_64 ||
+           m == MSBFS_METHOD_MULTI_WORD_BIT_PARALLEL_PUSH;
+}
+
+}  // namespace
+
+struct msbfs_context {
+    int device = 0;
+    hipDeviceProp_t prop{};
+    hipStream_t stream = nullptr;
+    int cus = 0;
+    size_t lds_max = 0;
+    int coop = 0;
+};
+
+struct msbfs_graph {
+    msbfs_context *ctx = nullptr;
+    uint64_t V = 0, E = 0;
+    uint32_t max_degree = 0;
+    int symmetric = 0;
+    uint32_t *ro = nullptr, *ci = nullptr, *co = nullptr, *ri = nullptr;
+    uint64_t bytes = 0;
+};
+
+struct msbfs_engine {
+    msbfs_context *ctx = nullptr;
+    int method = MSBFS_METHOD_AUTO;
+    msbfs_tuning tune{};
+    std::mutex mu;
+    // wide workspace (kept all-zero between runs by the kernels)
+    DevBuf seen, frA, frB, nx, qf0, qf1, qc, stamp, ctr;
+    uint64_t wide_words = 0;  // entries allocated per state array
+    uint32_t tick_base = 1;
+    bool dirty = true;
+    // deep workspace
+    DevBuf queues, tasks, graphs, work;
+    // shared
+    DevBuf flags, sources, out_tmp, reach_tmp;
+    PinnedBuf h_sources, h_flags, h_tasks;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    msbfs_run_stats last{};
+    hipStream_t last_stream = nullptr;
+    bool pending = false;
+    int pending_dtype = 0;
+};
+
+extern "C" const char *msbfs_version(void) { return MSBFS_VERSION; }
+
+// ------------------------------------------------------------------ context
+
+extern "C" int msbfs_context_create(int device, msbfs_context **out) {
+    clear_error();
+    if (!out) return fail(MSBFS_ERR_INVALID_ARG, "null out");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0)
+        return fail(MSBFS_ERR_NO_DEVICE, "no HIP device available (%s)",
+                    e == hipSuccess ? "0 devices" : hipGetErrorString(e));
+    if (device < 0 || device >= n)
+        return fail(MSBFS_ERR_INVALID_ARG, "device %d out of range (%d devices)", device, n);
+    auto *c = new msbfs_context;
+    c->device = device;
+    if ((e = hipSetDevice(device)) != hipSuccess ||
+        (e = hipGetDeviceProperties(&c->prop, device)) != hipSuccess ||
+        (e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) {
+        delete c;
+        return hip_fail(e, "context init");
+    }
+    c->cus = c->prop.multiProcessorCount;
+    int lds = 0;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) !=
+            hipSuccess ||
+        lds <= 0)
+        lds = (int)c->prop.sharedMemPerBlock;
+    c->lds_max = (size_t)lds;
+    int coop = 0;
+    (void)hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device);
+    c->coop = coop;
+    *out = c;
+    return MSBFS_OK;
+}
+
+extern "C" int msbfs_context_destroy(msbfs_context *c) {
+    if (!c) return MSBFS_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return MSBFS_OK;
+}
+
+extern "C" int msbfs_context_info(const msbfs_context *c, msbfs_device_info *info) {
+    if (!c || !info) return fail(MSBFS_ERR_INVALID_ARG, "null argument");
+    std::memset(info, 0, sizeof(*info));
+    std::snprintf(info->name, sizeof(info->name), "%s", c->prop.name);
+    std::snprintf(info->arch, sizeof(info->arch), "%s", c->prop.gcnArchName);
+    info->compute_units = c->cus;
+    info->lds_bytes_per_block = (int32_t)c->lds_max;
+    info->total_mem_bytes = c->prop.totalGlobalMem;
+    info->cooperative_launch = c->coop;
+    return MSBFS_OK;
+}
+
+extern "C" int msbfs_context_synchronize(msbfs_context *c) {
+    if (!c) return fail(MSBFS_ERR_INVALID_ARG, "null context");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipDeviceSynchronize());
+    return MSBFS_OK;
+}
+
+// ------------------------------------------------------------------ graph
+
+extern "C" int msbfs_graph_create(msbfs_context *c, uint64_t V, uint64_t E, const uint32_t *ro,
+                                  const uint32_t *ci, msbfs_graph **out) {
+    clear_error();
+    if (!c || !out) return fail(MSBFS_ERR_INVALID_ARG, "null argument");
+    if (V > 0xFFFFFFFEull) return fail(MSBFS_ERR_INVALID_GRAPH, "num_vertices exceeds u32");
+    if (E > 0xFFFFFFFFull) return fail(MSBFS_ERR_INVALID_GRAPH, "num_edges exceeds u32");
+    int rc = msbfs_csr_validate(V, E, ro, ci);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(c->device));
+    auto *g = new msbfs_graph;
+    g->ctx = c;
+    g->V = V;
+    g->E = E;
+    for (uint64_t v = 0; v < V; ++v) g->max_degree = std::max(g->max_degree, ro[v + 1] - ro[v]);
+    g->symmetric = csr_is_symmetric(V, E, ro, ci) ? 1 : 0;
+    std::vector<uint32_t> co, ri;
+    if (!g->symmetric) {
+        co.resize(V + 1);
+        ri.resize(E);
+        csr_transpose(V, ro, ci, co.data(), ri.data());
+    }
+    auto upload = [&](uint32_t **dst, const uint32_t *src, uint64_t n) -> hipError_t {
+        hipError_t e = hipMalloc((void **)dst, std::max<uint64_t>(n, 1) * sizeof(uint32_t));
+        if (e != hipSuccess) return e;
+        g->bytes += n * sizeof(uint32_t);
+        if (n) return hipMemcpy(*dst, src, n * sizeof(uint32_t), hipMemcpyHostToDevice);
+        return hipSuccess;
+    };
+    hipError_t e = upload(&g->ro, ro, V + 1);
+    if (e == hipSuccess) e = upload(&g->ci, ci, E);
+    if (e == hipSuccess) {
+        if (g->symmetric) {
+            g->co = g->ro;
+            g->ri = g->ci;
+        } else {
+            e = upload(&g->co, co.data(), V + 1);
+            if (e == hipSuccess) e = upload(&g->ri, ri.data(), E);
+        }
+    }
+    if (e != hipSuccess) {
+        msbfs_graph_destroy(g);
+        return hip_fail(e, "graph upload");
+    }
+    *out = g;
+    return MSBFS_OK;
+}
+
+extern "C" int msbfs_graph_destroy(msbfs_graph *g) {
+    if (!g) return MSBFS_OK;
+    (void)hipSetDevice(g->ctx->device);
+    if (g->co && g->co != g->ro) (void)hipFree(g->co);
+    if (g->ri && g->ri != g->ci) (void)hipFree(g->ri);
+    if (g->ro) (void)hipFree(g->ro);
+    if (g->ci) (void)hipFree(g->ci);
+    delete g;
+    return MSBFS_OK;
+}
+
+extern "C" int msbfs_graph_get_info(const msbfs_graph *g, msbfs_graph_info *info) {
+    if (!g || !info) return fail(MSBFS_ERR_INVALID_ARG, "null argument");
+    info->num_vertices = g->V;
+    info->num_edges = g->E;
+    info->max_degree = g->max_degree;
+    info->symmetric = (uint32_t)g->symmetric;
+    info->device_bytes = g->bytes;
+    return MSBFS_OK;
+}
+
+// ------------------------------------------------------------------ engine
+
+extern "C" int msbfs_engine_create(msbfs_context *c, int method, msbfs_engine **out) {
+    clear_error();
+    if (!c || !out) return fail(MSBFS_ERR_INVALID_ARG, "null argument");
+    if (method < 0 || method > MSBFS_METHOD_AUTO)
+        return fail(MSBFS_ERR_INVALID_ARG, "unknown method %d", method);
+    HIP_TRY(hipSetDevice(c->device));
+    auto *e = new msbfs_engine;
+    e->ctx = c;
+    e->method = method;
+    for (auto &ev : e->ev) {
+        hipError_t er = hipEventCreate(&ev);
+        if (er != hipSuccess) {
+            msbfs_engine_destroy(e);
+            return hip_fail(er, "hipEventCreate");
+        }
+    }
+    *out = e;
+    return MSBFS_OK;
+}
+
+extern "C" int msbfs_engine_destroy(msbfs_engine *e) {
+    if (!e) return MSBFS_OK;
+    (void)hipSetDevice(e->ctx->device);
+    (void)hipDeviceSynchronize();
+    for (auto &ev : e->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    delete e;
+    return MSBFS_OK;
+}
+
+extern "C" int msbfs_engine_method(const msbfs_engine *e) { return e ? e->method : -1; }
+
+extern "C" int msbfs_engine_set_push_pull(msbfs_engine *e, uint32_t alpha, uint32_t beta) {
+    if (!e) return fail(MSBFS_ERR_INVALID_ARG, "null engine");
+    std::lock_guard<std::mutex> lk(e->mu);
+    e->tune.alpha = alpha;
+    e->tune.beta = beta;
+    return MSBFS_OK;
+}
+
+extern "C" int msbfs_engine_set_tuning(msbfs_engine *e, const msbfs_tuning *t) {
+    if (!e || !t) return fail(MSBFS_ERR_INVALID_ARG, "null argument");
+    if (t->tile_words && t->tile_words != 1 && t->tile_words != 2 && t->tile_words != 4 &&
+        t->tile_words != 8)
+        return fail(MSBFS_ERR_INVALID_ARG, "tile_words must be 1, 2, 4 or 8");
+    if (t->deep_bits && (t->deep_bits > 32 || (t->deep_bits & (t->deep_bits - 1))))
+        return fail(MSBFS_ERR_INVALID_ARG, "deep_bits must be 1, 2, 4, 8, 16 or 32");
+    if (t->deep_threads && (t->deep_threads % 64 || t->deep_threads > 1024))
+        return fail(MSBFS_ERR_INVALID_ARG, "deep_threads must be a multiple of 64 <= 1024");
+    std::lock_guard<std::mutex> lk(e->mu);
+    e->tune = *t;
+    return MSBFS_OK;
+}
+
+extern "C" int msbfs_engine_get_tuning(const msbfs_engine *e, msbfs_tuning *t) {
+    if (!e || !t) return fail(MSBFS_ERR_INVALID_ARG, "null argument");
+    *t = e->tune;
+    return MSBFS_OK;
+}
+
+// GpuBfsEngine::select_auto_method (crates/gpu_bfs/src/engine.rs:157-198).
+extern "C" int msbfs_select_auto_method(uint64_t V, uint64_t E, uint32_t max_degree,
+                                        uint64_t S) {
+    if (V == 0 || S == 0) return MSBFS_METHOD_NAIVE_SEQUENTIAL;
+    const bool road_or_narrow = max_degree <= 8 && E <= 4 * V;
+    if (road_or_narrow) return MSBFS_METHOD_WORKGROUP_PERSISTENT_BFS;
+    if (max_degree >= 128) return MSBFS_METHOD_INDIRECT_SPARSE_QUEUE_PUSH;
+    if (S >= 64) return MSBFS_METHOD_MULTI_WORD_BIT_PARALLEL;
+    if (S >= 32) return MSBFS_METHOD_BIT_PARALLEL_64;
+    if (S > 1) return MSBFS_METHOD_BIT_PARALLEL_32;
+    return MSBFS_METHOD_SPARSE_QUEUE;
+}
+
+namespace {
+
+struct Plan {
+    int method = 0;
+    int family = MSBFS_FAMILY_NONE;
+    int policy = MSBFS_DIR_ADAPTIVE;
+    int bits = 0;       // deep
+    size_t lds = 0;     // deep
+};
+
+size_t deep_lds_bytes(uint64_t V, int bits) { return ((V * bits + 31) / 32) * 4; }
+
+constexpr size_t kDeepStaticLds = 64;
+
+// Deep family: sources per task so that the task count fills the device,
+// with the task's visited bits fitting in one workgroup's LDS.
+int deep_bits_for(const msbfs_context *c, const msbfs_tuning &t, uint64_t V, uint64_t S) {
+    if (V >= (1ull << 24)) return 0;
+    auto fits = [&](int b) { return deep_lds_bytes(V, b) + kDeepStaticLds <= c->lds_max; };
+    if (t.deep_bits) return fits((int)t.deep_bits) ? (int)t.deep_bits : 0;
+    const uint64_t target = (uint64_t)std::max(1, c->cus);
+    int best = 0;
+    for (int b : {1, 2, 4, 8, 16, 32}) {
+        if (!fits(b)) break;
+        if (best == 0 || (S + b - 1) / b >= target) best = b;
+    }
+    return best;
+}
+
+Plan plan_for(const msbfs_engine *e, const msbfs_graph *g, uint64_t S) {
+    Plan p;
+    p.method = e->method;
+    if (p.method == MSBFS_METHOD_AUTO) {
+        const int ref = msbfs_select_auto_method(g->V, g->E, g->max_degree, S);
+        p.method = ref;
+        if (ref == MSBFS_METHOD_WORKGROUP_PERSISTENT_BFS) {
+            p.family = MSBFS_FAMILY_DEEP;
+        } else {
+            p.family = MSBFS_FAMILY_WIDE;
+            p.policy = MSBFS_DIR_ADAPTIVE;
+        }
+    } else if (p.method == MSBFS_METHOD_NAIVE_SEQUENTIAL ||
+               p.method == MSBFS_METHOD_BATCHED_DENSE_SPMM) {
+        p.family = MSBFS_FAMILY_NONE;
+        return p;
+    } else if (p.method == MSBFS_METHOD_WORKGROUP_PERSISTENT_BFS) {
+        p.family = MSBFS_FAMILY_DEEP;
+    } else {
+        p.family = MSBFS_FAMILY_WIDE;
+        p.policy = is_pull_only(p.method)   ? MSBFS_DIR_PULL
+                   : is_push_only(p.method) ? MSBFS_DIR_PUSH
+                                            : MSBFS_DIR_ADAPTIVE;
+    }
+    if (p.family == MSBFS_FAMILY_DEEP) {
+        p.bits = deep_bits_for(e->ctx, e->tune, g->V, S);
+        if (p.bits == 0) {
+            // The task's visited bits do not fit in LDS: the device-wide
+            // family takes over (the reference likewise falls back to its
+            // multi-word pull path, workgroup_persistent.rs:494-512).
+            p.family = MSBFS_FAMILY_WIDE;
+            p.policy = MSBFS_DIR_ADAPTIVE;
+        } else {
+            p.lds = deep_lds_bytes(g->V, p.bits);
+        }
+    }
+    return p;
+}
+
+int check_sources(const msbfs_graph *g, const uint32_t *sources, uint64_t S) {
+    // GpuBfsEngine::run_ms_bfs_into argument checks (engine.rs:228-249).
+    if (g->V == 0)
+        return fail(MSBFS_ERR_SOURCE_OOB, "Source vertex %u at index 0 exceeds graph vertex count 0",
+                    sources[0]);
+    for (uint64_t i = 0; i < S; ++i)
+        if ((uint64_t)sources[i] >= g->V)
+            return fail(MSBFS_ERR_SOURCE_OOB,
+                        "Source vertex %u at index %llu exceeds graph vertex count %llu",
+                        sources[i], (unsigned long long)i, (unsigned long long)g->V);
+    return MSBFS_OK;
+}
+
+int ensure_zeroed(DevBuf &b, size_t bytes, bool force, hipStream_t st) {
+    int r = b.ensure(bytes);
+    if (r < 0) return -r;
+    if (r == 1 || force) HIP_TRY(hipMemsetAsync(b.p, 0, b.bytes, st));
+    return MSBFS_OK;
+}
+
+int ensure_plain(DevBuf &b, size_t bytes) {
+    int r = b.ensure(bytes);
+    return r < 0 ? -r : MSBFS_OK;
+}
+
+// Enqueues one traversal.  `out` (device) may be null when only the
+// reachability words are wanted.
+int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, uint64_t S,
+                int dtype, void *out, uint32_t *reach, uint32_t reach_wpv, hipStream_t st,
+                const Plan &plan) {
+    msbfs_context *c = e->ctx;
+    int rc;
+    const uint64_t V = g->V;
+    // sources: host -> pinned -> device
+    if ((rc = e->h_sources.ensure(S * 4)) < 0) return -rc;
+    std::memcpy(e->h_sources.p, sources, S * 4);
+    if ((rc = ensure_plain(e->sources, S * 4))) return rc;
+    if ((rc = ensure_plain(e->flags, FlagCount * 4))) return rc;
+    if ((rc = e->h_flags.ensure(FlagCount * 4)) < 0) return -rc;
+    HIP_TRY(hipEventRecord(e->ev[0], st));
+    HIP_TRY(hipMemcpyAsync(e->sources.p, e->h_sources.p, S * 4, hipMemcpyHostToDevice, st));
+    HIP_TRY(launch_zero_u32(e->flags.as<uint32_t>(), FlagCount, st));
+
+    e->last = msbfs_run_stats{};
+    e->last.method = plan.method;
+    e->last.family = plan.family;
+
+    if (plan.family == MSBFS_FAMILY_WIDE) {
+        if (!c->coop) return fail(MSBFS_ERR_UNSUPPORTED, "device lacks cooperative launch");
+        const uint64_t W = (S + 63) / 64;
+        const uint64_t words = W * V;
+        const bool force = e->dirty;
+        if ((rc = ensure_zeroed(e->seen, words * 8, force, st))) return rc;
+        if ((rc = ensure_zeroed(e->frA, words * 8, force, st))) return rc;
+        if ((rc = ensure_zeroed(e->frB, words * 8, force, st))) return rc;
+        if ((rc = ensure_zeroed(e->nx, words * 8, force, st))) return rc;
+        if ((rc = ensure_plain(e->qf0, V * 4))) return rc;
+        if ((rc = ensure_plain(e->qf1, V * 4))) return rc;
+        if ((rc = ensure_plain(e->qc, V * 4))) return rc;
+        const size_t stamp_before = e->stamp.bytes;
+        if ((rc = ensure_zeroed(e->stamp, V * 4, force, st))) return rc;
+        if (e->stamp.bytes != stamp_before || force) e->tick_base = 1;
+        if ((uint64_t)e->tick_base + V + 4 >= 0xFFFFFFFFull) {
+            HIP_TRY(hipMemsetAsync(e->stamp.p, 0, e->stamp.bytes, st));
+            e->tick_base = 1;
+        }
+        if ((rc = ensure_plain(e->ctr, 4 * sizeof(WideCounters)))) return rc;
+        e->dirty = false;
+
+        WideParams p{};
+        p.V = (uint32_t)V;
+        p.W = (uint32_t)W;
+        p.S = S;
+        p.E = g->E;
+        p.ro = g->ro;
+        p.ci = g->ci;
+        p.co = g->co;
+        p.ri = g->ri;
+        p.seen = e->seen.as<unsigned long long>();
+        p.frA = e->frA.as<unsigned long long>();
+        p.frB = e->frB.as<unsigned long long>();
+        p.nx = e->nx.as<unsigned long long>();
+        p.qf0 = e->qf0.as<uint32_t>();
+        p.qf1 = e->qf1.as<uint32_t>();
+        p.qc = e->qc.as<uint32_t>();
+        p.stamp = e->stamp.as<uint32_t>();
+        p.sources = e->sources.as<uint32_t>();
+        p.ctr = e->ctr.as<WideCounters>();
+        p.flags = e->flags.as<uint32_t>();
+        p.out = out;
+        p.reach = reach;
+        p.reach_wpv = reach_wpv;
+        p.tick_base = e->tick_base;
+        p.out_u16 = dtype == MSBFS_DIST_U16;
+        p.policy = (uint32_t)plan.policy;
+        p.alpha = e->tune.alpha ? e->tune.alpha : kDefaultAlpha;
+        p.beta = e->tune.beta ? e->tune.beta : kDefaultBeta;
+        p.rule = e->tune.direction_rule;
+        e->tick_base += (uint32_t)(V + 2);
+
+        const int tw = e->tune.tile_words ? (int)e->tune.tile_words : 1;
+        int occ = 0;
+        HIP_TRY(wide_max_blocks_per_cu(tw, kWideThreads, &occ));
+        if (occ < 1) return fail(MSBFS_ERR_HIP, "wide kernel cannot be resident");
+        int per_cu = e->tune.wide_blocks_per_cu ? (int)e->tune.wide_blocks_per_cu : 2;
+        per_cu = std::min(per_cu, occ);
+        const int blocks = c->cus * per_cu;
+
+        HIP_TRY(launch_zero_u32((uint32_t *)e->ctr.p, 4 * sizeof(WideCounters) / 4, st));
+        HIP_TRY(launch_wide_seed(p, st));
+        HIP_TRY(hipEventRecord(e->ev[1], st));
+        HIP_TRY(launch_wide(tw, p, blocks, kWideThreads, st));
+        HIP_TRY(hipEventRecord(e->ev[2], st));
+        e->last.grid_blocks = (uint32_t)blocks;
+        e->last.workspace_bytes = e->seen.bytes * 4 + e->stamp.bytes * 4;
+    } else {
+        const int B = plan.bits;
+        const uint64_t ntasks = (S + B - 1) / B;
+        const int threads = e->tune.deep_threads ? (int)e->tune.deep_threads : 512;
+        HIP_TRY(deep_prepare(B, plan.lds));
+        int occ = 0;
+        HIP_TRY(deep_max_blocks_per_cu(B, threads, plan.lds, &occ));
+        if (occ < 1) return fail(MSBFS_ERR_HIP, "deep kernel cannot be resident (lds %zu)", plan.lds);
+        const uint64_t resident = (uint64_t)c->cus * occ;
+        const int blocks = (int)std::min<uint64_t>(ntasks, resident);
+        const uint64_t cap = std::max<uint64_t>((uint64_t)B * V, (uint64_t)B);
+        const size_t esz = B <= 8 ? 4 : 8;
+        if ((rc = ensure_plain(e->queues, (size_t)blocks * 2 * cap * esz))) return rc;
+        if ((rc = ensure_plain(e->tasks, ntasks * sizeof(DeepTask)))) return rc;
+        if ((rc = ensure_plain(e->graphs, sizeof(DeepGraph)))) return rc;
+        if ((rc = ensure_plain(e->work, 16))) return rc;
+        if ((rc = e->h_tasks.ensure(ntasks * sizeof(DeepTask) + sizeof(DeepGraph))) < 0) return -rc;
+        // The pinned staging buffer may still be read by a previous enqueue.
+        HIP_TRY(hipStreamSynchronize(st));
+        auto *ht = (DeepTask *)e->h_tasks.p;
+        for (uint64_t t = 0; t < ntasks; ++t) {
+            ht[t].graph = 0;
+            ht[t].src_off = t * B;
+            ht[t].nsrc = (uint32_t)std::min<uint64_t>(B, S - t * B);
+            ht[t].out_off = t * B * V;
+            ht[t].reach_bit = t * B;
+        }
+        auto *hg = (DeepGraph *)((char *)e->h_tasks.p + ntasks * sizeof(DeepTask));
+        hg->ro = g->ro;
+        hg->ci = g->ci;
+        hg->V = (uint32_t)V;
+        hg->pad = 0;
+        HIP_TRY(hipMemcpyAsync(e->tasks.p, ht, ntasks * sizeof(DeepTask), hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(e->graphs.p, hg, sizeof(DeepGraph), hipMemcpyHostToDevice, st));
+        HIP_TRY(launch_zero_u32(e->work.as<uint32_t>(), 4, st));
+
+        DeepParams p{};
+        p.tasks = e->tasks.as<DeepTask>();
+        p.graphs = e->graphs.as<DeepGraph>();
+        p.sources = e->sources.as<uint32_t>();
+        p.out = out;
+        p.queues = e->queues.p;
+        p.cap = cap;
+        p.work = e->work.as<uint32_t>();
+        p.flags = e->flags.as<uint32_t>();
+        p.reach = reach;
+        p.reach_wpv = reach_wpv;
+        p.ntasks = (uint32_t)ntasks;
+        p.out_u16 = dtype == MSBFS_DIST_U16;
+        HIP_TRY(hipEventRecord(e->ev[1], st));
+        HIP_TRY(launch_deep(B, p, blocks, threads, plan.lds, st));
+        HIP_TRY(hipEventRecord(e->ev[2], st));
+        e->last.instances = (uint32_t)ntasks;
+        e->last.bits_per_instance = (uint32_t)B;
+        e->last.grid_blocks = (uint32_t)blocks;
+        e->last.workspace_bytes = e->queues.bytes;
+    }
+    HIP_TRY(hipMemcpyAsync(e->h_flags.p, e->flags.p, FlagCount * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipEventRecord(e->ev[3], st));
+    e->pending = true;
+    e->last_stream = st;
+    e->pending_dtype = dtype;
+    return MSBFS_OK;
+}
+
+// Waits for the pending run and turns its device flags into a status.
+int finish_run(msbfs_engine *e) {
+    if (!e->pending) return MSBFS_OK;
+    e->pending = false;
+    hipError_t er = hipStreamSynchronize(e->last_stream);
+    if (er != hipSuccess) {
+        e->dirty = true;
+        return hip_fail(er, "traversal");
+    }
+    const uint32_t *f = (const uint32_t *)e->h_flags.p;
+    e->last.levels = f[FlagLevels];
+    e->last.push_levels = f[FlagPushLevels];
+    e->last.pull_levels = f[FlagPullLevels];
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, e->ev[1], e->ev[2]) == hipSuccess) e->last.kernel_ms = ms;
+    if (hipEventElapsedTime(&ms, e->ev[0], e->ev[3]) == hipSuccess) e->last.total_ms = ms;
+    if (f[FlagQueueOverflow]) {
+        e->dirty = true;
+        return fail(MSBFS_ERR_HIP, "internal frontier queue overflow");
+    }
+    if (f[FlagU16Overflow] && e->pending_dtype == MSBFS_DIST_U16)
+        return fail(MSBFS_ERR_DIAMETER_U16,
+                    "a distance >= 65535 cannot be stored as u16 (0xFFFF is the unreachable "
+                    "sentinel); use u32 output");
+    return MSBFS_OK;
+}
+
+int run_common(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, uint64_t S,
+               int dtype, void *out_dev, uint32_t *reach_dev, uint32_t reach_wpv, hipStream_t st,
+               bool sync) {
+    const Plan plan = plan_for(e, g, S);
+    if (plan.family == MSBFS_FAMILY_NONE)
+        return fail(MSBFS_ERR_UNSUPPORTED,
+                    "method %d (NaiveSequential / BatchedDenseSpMM baselines) is not provided by "
+                    "this engine",
+                    e->method);
+    int rc = finish_run(e);
+    if (rc) return rc;
+    rc = enqueue_run(e, g, sources, S, dtype, out_dev, reach_dev, reach_wpv, st, plan);
+    if (rc) {
+        e->dirty = true;
+        e->pending = false;
+        return rc;
+    }
+    return sync ? finish_run(e) : MSBFS_OK;
+}
+
+}  // namespace
+
+extern "C" int msbfs_engine_resolve(const msbfs_engine *e, const msbfs_graph *g, uint64_t S,
+                                    int *method_out) {
+    if (!e || !g) return fail(MSBFS_ERR_INVALID_ARG, "null argument");
+    const Plan p = plan_for(e, g, S);
+    if (method_out) *method_out = p.method;
+    return p.family;
+}
+
+extern "C" int msbfs_run_ms_bfs_device(msbfs_engine *e, const msbfs_graph *g,
+                                       const uint32_t *sources, uint64_t S, int dtype,
+                                       void *out_device, uint64_t out_len, void *stream,
+                                       int sync) {
+    clear_error();
+    if (!e || !g) return fail(MSBFS_ERR_INVALID_ARG, "null engine or graph");
+    if (dtype != MSBFS_DIST_U32 && dtype != MSBFS_DIST_U16)
+        return fail(MSBFS_ERR_INVALID_ARG, "unknown dist type %d", dtype);
+    if (S == 0) return MSBFS_OK;
+    if (!sources) return fail(MSBFS_ERR_INVALID_ARG, "null sources");
+    std::lock_guard<std::mutex> lk(e->mu);
+    int rc = check_sources(g, sources, S);
+    if (rc) return rc;
+    if (out_len < S * g->V)
+        return fail(MSBFS_ERR_OUT_TOO_SMALL, "output holds %llu elements, %llu needed",
+                    (unsigned long long)out_len, (unsigned long long)(S * g->V));
+    if (!out_device) return fail(MSBFS_ERR_INVALID_ARG, "null output");
+    HIP_TRY(hipSetDevice(e->ctx->device));
+    hipStream_t st = stream ? (hipStream_t)stream : e->ctx->stream;
+    return run_common(e, g, sources, S, dtype, out_device, nullptr, 0, st, sync != 0);
+}
+
+extern "C" int msbfs_engine_check(msbfs_engine *e) {
+    if (!e) return fail(MSBFS_ERR_INVALID_ARG, "null engine");
+    std::lock_guard<std::mutex> lk(e->mu);
+    return finish_run(e);
+}
+
+extern "C" int msbfs_engine_last_stats(msbfs_engine *e, msbfs_run_stats *st) {
+    if (!e || !st) return fail(MSBFS_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(e->mu);
+    int rc = finish_run(e);
+    *st = e->last;
+    return rc;
+}
+
+extern "C" int msbfs_run_ms_bfs(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources,
+                                uint64_t S, int dtype, void *out, uint64_t out_len) {
+    clear_error();
+    if (!e || !g) return fail(MSBFS_ERR_INVALID_ARG, "null engine or graph");
+    if (dtype != MSBFS_DIST_U32 && dtype != MSBFS_DIST_U16)
+        return fail(MSBFS_ERR_INVALID_ARG, "unknown dist type %d", dtype);
+    if (S == 0) return MSBFS_OK;
+    if (!sources) return fail(MSBFS_ERR_INVALID_ARG, "null sources");
+    std::lock_guard<std::mutex> lk(e->mu);
+    int rc = check_sources(g, sources, S);
+    if (rc) return rc;
+    const uint64_t n = S * g->V;
+    if (out_len < n)
+        return fail(MSBFS_ERR_OUT_TOO_SMALL, "output holds %llu elements, %llu needed",
+                    (unsigned long long)out_len, (unsigned long long)n);
+    if (!out) return fail(MSBFS_ERR_INVALID_ARG, "null output");
+    HIP_TRY(hipSetDevice(e->ctx->device));
+    const size_t esz = dtype == MSBFS_DIST_U16 ? 2 : 4;
+    if ((rc = ensure_plain(e->out_tmp, n * esz))) return rc;
+    hipStream_t st = e->ctx->stream;
+    rc = run_common(e, g, sources, S, dtype, e->out_tmp.p, nullptr, 0, st, true);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(out, e->out_tmp.p, n * esz, hipMemcpyDeviceToHost));
+    return MSBFS_OK;
+}
+
+extern "C" int msbfs_run_ms_reachability(msbfs_engine *e, const msbfs_graph *g,
+                                         const uint32_t *sources, uint64_t S, uint32_t *words,
+                                         uint64_t words_len) {
+    clear_error();
+    if (!e || !g) return fail(MSBFS_ERR_INVALID_ARG, "null engine or graph");
+    // run_ms_reachability: S == 0 or |V| == 0 -> all-zero bitmask (engine.rs:428-430).
+    if (S == 0 || g->V == 0) return MSBFS_OK;
+    if (!sources) return fail(MSBFS_ERR_INVALID_ARG, "null sources");
+    std::lock_guard<std::mutex> lk(e->mu);
+    int rc = check_sources(g, sources, S);
+    if (rc) return rc;
+    const uint64_t wpv = (S + 31) / 32;
+    const uint64_t n = g->V * wpv;
+    if (words_len < n)
+        return fail(MSBFS_ERR_OUT_TOO_SMALL, "bitmask holds %llu words, %llu needed",
+                    (unsigned long long)words_len, (unsigned long long)n);
+    if (!words) return fail(MSBFS_ERR_INVALID_ARG, "null output");
+    HIP_TRY(hipSetDevice(e->ctx->device));
+    if ((rc = ensure_plain(e->reach_tmp, n * 4))) return rc;
+    hipStream_t st = e->ctx->stream;
+    HIP_TRY(launch_zero_u32(e->reach_tmp.as<uint32_t>(), n, st));
+    rc = run_common(e, g, sources, S, MSBFS_DIST_U32, nullptr, e->reach_tmp.as<uint32_t>(),
+                    (uint32_t)wpv, st, true);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(words, e->reach_tmp.p, n * 4, hipMemcpyDeviceToHost));
+    return MSBFS_OK;
+}

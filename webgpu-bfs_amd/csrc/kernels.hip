@@ -1,0 +1,665 @@
+// Hand-written gfx950 (CDNA4, wave64) kernels of the multi-source BFS.
+//
+// All arithmetic is integer: source sets are bit masks, a BFS level ORs the
+// frontier masks of a vertex's parents into its children and keeps the bits
+// not seen before.  No MFMA: the boolean semiring product is a gather/scatter
+// over the graph, bounded by memory traffic, not a dense contraction.
+//
+// Two families:
+//  * deep  -- one workgroup per group of B <= 32 sources.  The group's
+//             visited bits (B per vertex) live in LDS; the frontier is a
+//             queue of (vertex, new-bits) entries in global memory, filled
+//             with wave-aggregated appends (ballot + popcount prefix, one LDS
+//             atomic per wave).  Every level of the group runs inside the
+//             workgroup with s_barrier only; workgroups pull groups from a
+//             global counter (persistent, no grid barrier).  Suited to
+//             high-diameter graphs, where a device-wide barrier per level
+//             would dominate.
+//  * wide  -- one cooperative persistent kernel for all S sources at once,
+//             64 sources per 64-bit word, state in HBM word-major ([w][v]).
+//             Per level it either pushes (CSR, atomicOr of 64-bit words,
+//             candidate queue deduplicated by a level stamp, then a filter
+//             pass) or pulls (CSC, per (vertex, word) gather of parent words
+//             with early exit once every still-needed bit is found), chosen
+//             on the device from the previous level's counters.  Levels are
+//             separated by grid barriers; the host is not involved.
+//
+// The reference computes the same distances with WGSL kernels driven level
+// by level from the host (crates/shaders/src/bit_parallel_64_push.wgsl:15-48,
+// bit_parallel_64.wgsl:18-78, bit_parallel_64_pull.wgsl:27-111,
+// sparse_bp_push.wgsl:23-70, workgroup_persistent_bfs.wgsl:39-196).
+#include <hip/hip_cooperative_groups.h>
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace cg = cooperative_groups;
+
+namespace msbfs {
+
+typedef unsigned long long u64;
+
+// ------------------------------------------------------------------ helpers
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// Reserves one slot per lane with `pred` set in the queue counted by *ctr,
+// using one atomic per wave.  Every active lane of the wave must call it.
+template <typename Ctr>
+__device__ __forceinline__ uint32_t wave_reserve(Ctr *ctr, bool pred) {
+    const u64 bal = __ballot(pred);
+    uint32_t base = 0;
+    if (bal) {
+        const int leader = __ffsll((long long)bal) - 1;
+        if ((int)lane_id() == leader) base = atomicAdd(ctr, (uint32_t)__popcll(bal));
+        base = __shfl(base, leader);
+    }
+    const u64 below = lane_id() ? (bal & ((~0ull) >> (64 - lane_id()))) : 0ull;
+    return base + (uint32_t)__popcll(below);
+}
+
+__device__ __forceinline__ u64 wave_or64(u64 x) {
+    uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        lo |= (uint32_t)__shfl_xor((int)lo, off);
+        hi |= (uint32_t)__shfl_xor((int)hi, off);
+    }
+    return ((u64)hi << 32) | lo;
+}
+
+__device__ __forceinline__ u64 wave_sum64(u64 x) {
+    uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        uint32_t olo = (uint32_t)__shfl_xor((int)lo, off);
+        uint32_t ohi = (uint32_t)__shfl_xor((int)hi, off);
+        u64 s = ((u64)hi << 32 | lo) + ((u64)ohi << 32 | olo);
+        lo = (uint32_t)s;
+        hi = (uint32_t)(s >> 32);
+    }
+    return ((u64)hi << 32) | lo;
+}
+
+__device__ __forceinline__ void store_dist(void *out, bool u16, uint64_t idx, uint32_t d,
+                                           uint32_t *flags) {
+    if (u16) {
+        if (d >= 0xFFFFu) {
+            atomicOr(&flags[FlagU16Overflow], 1u);
+            d = 0xFFFEu;
+        }
+        ((uint16_t *)out)[idx] = (uint16_t)d;
+    } else {
+        ((uint32_t *)out)[idx] = d;
+    }
+}
+
+__device__ __forceinline__ void store_sentinel(void *out, bool u16, uint64_t idx) {
+    if (u16)
+        ((uint16_t *)out)[idx] = 0xFFFFu;
+    else
+        ((uint32_t *)out)[idx] = 0xFFFFFFFFu;
+}
+
+__global__ void zero_u32_kernel(uint32_t *p, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+hipError_t launch_zero_u32(uint32_t *ptr, uint64_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    uint64_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(zero_u32_kernel, dim3((unsigned)blocks), dim3(256), 0, st, ptr, n);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- deep family
+
+template <int B>
+struct DeepEntry;
+template <>
+struct DeepEntry<1> { typedef uint32_t T; };
+template <>
+struct DeepEntry<2> { typedef uint32_t T; };
+template <>
+struct DeepEntry<4> { typedef uint32_t T; };
+template <>
+struct DeepEntry<8> { typedef uint32_t T; };
+template <>
+struct DeepEntry<16> { typedef u64 T; };
+template <>
+struct DeepEntry<32> { typedef u64 T; };
+
+// Entries pack (vertex, bits).  B <= 8: v << 8 | bits (needs V < 2^24);
+// B >= 16: bits << 32 | v.
+__device__ __forceinline__ uint32_t pack_entry(uint32_t v, uint32_t m, uint32_t *) {
+    return (v << 8) | m;
+}
+__device__ __forceinline__ u64 pack_entry(uint32_t v, uint32_t m, u64 *) {
+    return ((u64)m << 32) | v;
+}
+__device__ __forceinline__ void unpack_entry(uint32_t x, uint32_t &v, uint32_t &m) {
+    v = x >> 8;
+    m = x & 0xFFu;
+}
+__device__ __forceinline__ void unpack_entry(u64 x, uint32_t &v, uint32_t &m) {
+    v = (uint32_t)x;
+    m = (uint32_t)(x >> 32);
+}
+
+template <int B>
+__global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
+    typedef typename DeepEntry<B>::T Entry;
+    constexpr uint32_t VPW = 32 / B;  // vertices per LDS word
+    constexpr uint32_t FULL = (B == 32) ? 0xFFFFFFFFu : ((1u << B) - 1u);
+    extern __shared__ uint32_t vis[];
+    __shared__ uint32_t s_task;
+    __shared__ uint32_t s_cnt[2];
+
+    const uint32_t tid = threadIdx.x, nth = blockDim.x;
+    Entry *const qbase = (Entry *)p.queues + (uint64_t)blockIdx.x * 2 * p.cap;
+    const bool u16 = p.out_u16 != 0;
+
+    for (;;) {
+        if (tid == 0) s_task = atomicAdd(p.work, 1u);
+        __syncthreads();
+        const uint32_t t = s_task;
+        if (t >= p.ntasks) break;
+        const DeepTask task = p.tasks[t];
+        const DeepGraph g = p.graphs[task.graph];
+        const uint32_t V = g.V;
+        const uint32_t nwords = (V + VPW - 1) / VPW;
+        for (uint32_t i = tid; i < nwords; i += nth) vis[i] = 0;
+        if (tid == 0) {
+            s_cnt[0] = task.nsrc;
+            s_cnt[1] = 0;
+        }
+        __syncthreads();
+
+        Entry *qa = qbase, *qb = qbase + p.cap;
+        if (tid < task.nsrc) {
+            const uint32_t s = p.sources[task.src_off + tid];
+            const uint32_t bit = 1u << tid;
+            atomicOr(&vis[s / VPW], bit << ((s % VPW) * B));
+            qa[tid] = pack_entry(s, bit, (Entry *)nullptr);
+            if (p.out) store_dist(p.out, u16, task.out_off + (uint64_t)tid * V + s, 0, p.flags);
+        }
+        __syncthreads();
+
+        uint32_t level = 0;
+        for (;;) {
+            const uint32_t n = s_cnt[level & 1];
+            if (n == 0) break;
+            uint32_t *const cnt_next = &s_cnt[(level + 1) & 1];
+            const uint32_t nd = level + 1;
+            for (uint32_t base = 0; base < n; base += nth) {
+                const uint32_t i = base + tid;
+                uint32_t u = 0, f = 0, e = 0, eend = 0;
+                if (i < n) {
+                    unpack_entry(qa[i], u, f);
+                    e = g.ro[u];
+                    eend = g.ro[u + 1];
+                }
+                // Wave-uniform edge loop so that every lane takes part in the
+                // queue reservation.
+                while (__any(e < eend)) {
+                    uint32_t v = 0, newly = 0;
+                    if (e < eend) {
+                        v = g.ci[e];
+                        const uint32_t wi = v / VPW, sh = (v % VPW) * B;
+                        const uint32_t m = f & ~((vis[wi] >> sh) & FULL);
+                        if (m) {
+                            const uint32_t old = atomicOr(&vis[wi], m << sh);
+                            newly = m & ~((old >> sh) & FULL);
+                        }
+                        ++e;
+                    }
+                    const uint32_t pos = wave_reserve(cnt_next, newly != 0);
+                    if (newly) {
+                        if (pos < p.cap)
+                            qb[pos] = pack_entry(v, newly, (Entry *)nullptr);
+                        else
+                            atomicOr(&p.flags[FlagQueueOverflow], 1u);
+                        if (p.out) {
+                            uint32_t bits = newly;
+                            while (bits) {
+                                const uint32_t b = __builtin_ctz(bits);
+                                store_dist(p.out, u16, task.out_off + (uint64_t)b * V + v, nd,
+                                           p.flags);
+                                bits &= bits - 1;
+                            }
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+            if (tid == 0) s_cnt[level & 1] = 0;
+            Entry *tq = qa;
+            qa = qb;
+            qb = tq;
+            ++level;
+            __syncthreads();
+        }
+        if (tid == 0) atomicMax(&p.flags[FlagLevels], level);
+
+        // Unreached (source, vertex) pairs get the sentinel; rows are written
+        // by consecutive threads, so the stores coalesce.
+        if (p.out) {
+            for (uint32_t b = 0; b < task.nsrc; ++b) {
+                const uint64_t row = task.out_off + (uint64_t)b * V;
+                for (uint32_t v = tid; v < V; v += nth)
+                    if (!((vis[v / VPW] >> ((v % VPW) * B + b)) & 1u))
+                        store_sentinel(p.out, u16, row + v);
+            }
+        }
+        if (p.reach) {
+            // Sources of a task occupy B consecutive bits that never straddle
+            // a 32-bit word (tasks start at multiples of B).
+            const uint32_t word = (uint32_t)(task.reach_bit / 32), shift = task.reach_bit % 32;
+            const uint32_t valid = (task.nsrc == 32) ? 0xFFFFFFFFu : ((1u << task.nsrc) - 1u);
+            for (uint32_t v = tid; v < V; v += nth) {
+                const uint32_t bits = (vis[v / VPW] >> ((v % VPW) * B)) & valid;
+                if (bits) atomicOr(&p.reach[(uint64_t)v * p.reach_wpv + word], bits << shift);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <int B>
+static hipError_t deep_launch_t(const DeepParams &p, int blocks, int threads, size_t lds,
+                                hipStream_t st) {
+    hipLaunchKernelGGL(deep_bfs_kernel<B>, dim3(blocks), dim3(threads), lds, st, p);
+    return hipGetLastError();
+}
+
+template <int B>
+static hipError_t deep_prepare_t(size_t lds) {
+    return hipFuncSetAttribute((const void *)deep_bfs_kernel<B>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+}
+
+template <int B>
+static hipError_t deep_occ_t(int threads, size_t lds, int *out) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, deep_bfs_kernel<B>, threads, lds);
+}
+
+hipError_t launch_deep(int bits, const DeepParams &p, int blocks, int threads, size_t lds,
+                       hipStream_t st) {
+#define CALL(B) deep_launch_t<B>(p, blocks, threads, lds, st)
+    switch (bits) {
+    case 1: return CALL(1);
+    case 2: return CALL(2);
+    case 4: return CALL(4);
+    case 8: return CALL(8);
+    case 16: return CALL(16);
+    case 32: return CALL(32);
+    default: return hipErrorInvalidValue;
+    }
+#undef CALL
+}
+
+hipError_t deep_prepare(int bits, size_t lds) {
+    switch (bits) {
+    case 1: return deep_prepare_t<1>(lds);
+    case 2: return deep_prepare_t<2>(lds);
+    case 4: return deep_prepare_t<4>(lds);
+    case 8: return deep_prepare_t<8>(lds);
+    case 16: return deep_prepare_t<16>(lds);
+    case 32: return deep_prepare_t<32>(lds);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t deep_max_blocks_per_cu(int bits, int threads, size_t lds, int *out) {
+    switch (bits) {
+    case 1: return deep_occ_t<1>(threads, lds, out);
+    case 2: return deep_occ_t<2>(threads, lds, out);
+    case 4: return deep_occ_t<4>(threads, lds, out);
+    case 8: return deep_occ_t<8>(threads, lds, out);
+    case 16: return deep_occ_t<16>(threads, lds, out);
+    case 32: return deep_occ_t<32>(threads, lds, out);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// ---------------------------------------------------------------- wide family
+
+__device__ __forceinline__ u64 word_valid(uint64_t S, uint32_t w) {
+    const uint64_t lo = (uint64_t)w * 64;
+    if (lo + 64 <= S) return ~0ull;
+    if (lo >= S) return 0ull;
+    return (1ull << (S - lo)) - 1ull;
+}
+
+__device__ __forceinline__ uint32_t ld_dev_u32(const uint32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ u64 ld_dev_u64(const u64 *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Writes distance `d` for every bit of `n` (sources 64*w + b) at vertex v.
+// The loop runs over the union of the wave's bits so that, for one source,
+// the lanes holding consecutive vertices store together.
+__device__ __forceinline__ void wide_write_bits(const WideParams &p, bool ok, uint32_t w,
+                                                uint32_t v, u64 n, uint32_t d) {
+    u64 un = wave_or64(ok ? n : 0ull);
+    const bool u16 = p.out_u16 != 0;
+    while (un) {
+        const uint32_t b = (uint32_t)__builtin_ctzll(un);
+        un &= un - 1;
+        if (ok && ((n >> b) & 1ull))
+            store_dist(p.out, u16, ((uint64_t)w * 64 + b) * p.V + v, d, p.flags);
+    }
+}
+
+__global__ void wide_seed_kernel(WideParams p) {
+    const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= p.S) return;
+    const uint32_t s = p.sources[i];
+    const uint32_t w = (uint32_t)(i / 64);
+    const u64 bit = 1ull << (i % 64);
+    const uint64_t idx = (uint64_t)w * p.V + s;
+    atomicOr(&p.seen[idx], bit);
+    atomicOr(&p.frA[idx], bit);
+    if (p.out) store_dist(p.out, p.out_u16 != 0, i * p.V + s, 0, p.flags);
+    if (atomicExch(&p.stamp[s], p.tick_base) != p.tick_base) {
+        const uint32_t pos = atomicAdd(&p.ctr[3].next_count, 1u);
+        p.qf0[pos] = s;
+    }
+}
+
+hipError_t launch_wide_seed(const WideParams &p, hipStream_t st) {
+    const uint64_t blocks = (p.S + 255) / 256;
+    hipLaunchKernelGGL(wide_seed_kernel, dim3((unsigned)blocks), dim3(256), 0, st, p);
+    return hipGetLastError();
+}
+
+enum { DirPush = 0, DirPull = 1 };
+
+template <int TW>
+__global__ void __launch_bounds__(kWideThreads) wide_bfs_kernel(WideParams p) {
+    cg::grid_group grid = cg::this_grid();
+    constexpr uint32_t TV = 64 / TW;
+    const uint32_t lane = lane_id();
+    const uint32_t vi = lane % TV, wi = lane / TV;
+    const uint64_t gwave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / 64;
+    const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / 64;
+    const uint64_t gthread = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    const uint64_t nthreads = (uint64_t)gridDim.x * blockDim.x;
+    const uint32_t V = p.V, W = p.W;
+    const uint32_t WG = (W + TW - 1) / TW;
+    const bool first = blockIdx.x == 0 && threadIdx.x == 0;
+
+    u64 *cur = p.frA, *nxt = p.frB;
+    uint32_t *qcur = p.qf0, *qnext = p.qf1;
+    uint32_t n_front = ld_dev_u32(&p.ctr[3].next_count);
+    u64 prev_bits = p.S, prev_deg = 0;
+    uint32_t dir = (p.policy == 2) ? DirPull : DirPush;
+    uint32_t push_levels = 0, pull_levels = 0;
+    uint32_t L = 0;
+
+    for (;; ++L) {
+        if (n_front == 0 || L > V) break;
+        WideCounters *c = &p.ctr[L & 3];
+        if (p.policy == 0 && L > 0) {
+            // Direction controller (crates/gpu_bfs/src/dispatch/push_pull.rs:483-498):
+            // Push->Pull when m_f > |E|/alpha, Pull->Push when n_f < |V|/beta.
+            u64 n_f, m_f;
+            if (p.rule == 0) {
+                n_f = (prev_bits + p.S - 1) / p.S;
+                m_f = V ? (n_f * p.E) / V : 0;
+            } else {
+                n_f = n_front;
+                m_f = prev_deg;
+            }
+            if (dir == DirPush) {
+                if (m_f > p.E / (p.alpha ? p.alpha : 1)) dir = DirPull;
+            } else {
+                if (n_f < (u64)V / (p.beta ? p.beta : 1)) dir = DirPush;
+            }
+        }
+        const uint32_t nd = L + 1;
+        const uint32_t tick = p.tick_base + L + 1;
+        uint64_t my_bits = 0, my_deg = 0;
+
+        if (dir == DirPush) {
+            ++push_levels;
+            // Expand: each wave takes TV frontier vertices x TW words.
+            const uint64_t ntiles = (uint64_t)((n_front + TV - 1) / TV) * WG;
+            for (uint64_t t = gwave; t < ntiles; t += nwaves) {
+                const uint32_t i = (uint32_t)(t / WG) * TV + vi;
+                const uint32_t w = (uint32_t)(t % WG) * TW + wi;
+                const bool ok = i < n_front && w < W;
+                uint32_t u = 0, e = 0, eend = 0;
+                u64 f = 0;
+                if (ok) {
+                    u = qcur[i];
+                    const uint64_t idx = (uint64_t)w * V + u;
+                    f = cur[idx];
+                    if (f) {
+                        cur[idx] = 0;
+                        e = p.ro[u];
+                        eend = p.ro[u + 1];
+                    }
+                }
+                while (__any(e < eend)) {
+                    bool app = false;
+                    uint32_t v = 0;
+                    if (e < eend) {
+                        v = p.ci[e];
+                        const uint64_t idx = (uint64_t)w * V + v;
+                        const u64 m = f & ~p.seen[idx];
+                        if (m) {
+                            atomicOr(&p.nx[idx], m);
+                            if (atomicExch(&p.stamp[v], tick) != tick) app = true;
+                        }
+                        ++e;
+                    }
+                    const uint32_t pos = wave_reserve(&c->cand_count, app);
+                    if (app) p.qc[pos] = v;
+                }
+            }
+            grid.sync();
+            // Filter: new bits of each candidate, next frontier, distances.
+            const uint32_t ncand = ld_dev_u32(&c->cand_count);
+            const uint64_t ftiles = (ncand + TV - 1) / TV;
+            for (uint64_t t = gwave; t < ftiles; t += nwaves) {
+                const uint32_t i = (uint32_t)t * TV + vi;
+                const bool okv = i < ncand;
+                const uint32_t v = okv ? p.qc[i] : 0;
+                bool any = false;
+                for (uint32_t g = 0; g < WG; ++g) {
+                    const uint32_t w = g * TW + wi;
+                    const bool ok = okv && w < W;
+                    u64 n = 0;
+                    if (ok) {
+                        const uint64_t idx = (uint64_t)w * V + v;
+                        const u64 x = p.nx[idx];
+                        if (x) {
+                            p.nx[idx] = 0;
+                            const u64 s = p.seen[idx];
+                            n = x & ~s;
+                            if (n) {
+                                p.seen[idx] = s | n;
+                                nxt[idx] = n;
+                            }
+                        }
+                    }
+                    if (p.out) wide_write_bits(p, ok, w, v, n, nd);
+                    any |= n != 0;
+                    my_bits += __popcll(n);
+                }
+                const u64 bal = __ballot(any);
+                bool vany = false;
+#pragma unroll
+                for (uint32_t k = 0; k < TW; ++k) vany |= ((bal >> (k * TV + vi)) & 1ull) != 0;
+                const bool app = wi == 0 && okv && vany;
+                const uint32_t pos = wave_reserve(&c->next_count, app);
+                if (app) {
+                    qnext[pos] = v;
+                    my_deg += p.ro[v + 1] - p.ro[v];
+                }
+            }
+        } else {
+            ++pull_levels;
+            // Pull: each wave takes TV consecutive vertices x TW words and
+            // gathers parent frontier words until every needed bit is found.
+            const uint64_t ntiles = (V + TV - 1) / TV;
+            for (uint64_t t = gwave; t < ntiles; t += nwaves) {
+                const uint32_t v = (uint32_t)t * TV + vi;
+                const bool okv = v < V;
+                uint32_t e0 = 0, e1 = 0;
+                if (okv) {
+                    e0 = p.co[v];
+                    e1 = p.co[v + 1];
+                }
+                bool any = false;
+                for (uint32_t g = 0; g < WG; ++g) {
+                    const uint32_t w = g * TW + wi;
+                    const bool ok = okv && w < W;
+                    u64 n = 0;
+                    if (ok) {
+                        const uint64_t idx = (uint64_t)w * V + v;
+                        const u64 s = p.seen[idx];
+                        const u64 need = ~s & word_valid(p.S, w);
+                        if (need) {
+                            u64 acc = 0;
+                            const u64 *cw = cur + (uint64_t)w * V;
+                            for (uint32_t e = e0; e < e1; ++e) {
+                                acc |= cw[p.ri[e]];
+                                if ((acc & need) == need) break;
+                            }
+                            n = acc & need;
+                            if (n) {
+                                p.seen[idx] = s | n;
+                                nxt[idx] = n;
+                            }
+                        }
+                    }
+                    if (p.out) wide_write_bits(p, ok, w, v, n, nd);
+                    any |= n != 0;
+                    my_bits += __popcll(n);
+                }
+                const u64 bal = __ballot(any);
+                bool vany = false;
+#pragma unroll
+                for (uint32_t k = 0; k < TW; ++k) vany |= ((bal >> (k * TV + vi)) & 1ull) != 0;
+                const bool app = wi == 0 && okv && vany;
+                const uint32_t pos = wave_reserve(&c->next_count, app);
+                if (app) {
+                    qnext[pos] = v;
+                    my_deg += p.ro[v + 1] - p.ro[v];
+                }
+            }
+            grid.sync();
+            // The consumed frontier words are cleared before this array is
+            // written again as the frontier after next.
+            const uint64_t nclear = (uint64_t)n_front * W;
+            for (uint64_t k = gthread; k < nclear; k += nthreads) {
+                const uint32_t w = (uint32_t)(k / n_front);
+                const uint32_t i = (uint32_t)(k % n_front);
+                cur[(uint64_t)w * V + qcur[i]] = 0;
+            }
+        }
+        // Level counters for the direction controller: one atomic per wave.
+        {
+            const u64 sb = wave_sum64(my_bits), sd = wave_sum64(my_deg);
+            if (lane == 0) {
+                if (sb) atomicAdd(&c->active_bits, sb);
+                if (sd) atomicAdd(&c->deg_sum, sd);
+            }
+        }
+        if (first) {
+            WideCounters *z = &p.ctr[(L + 2) & 3];
+            z->next_count = 0;
+            z->cand_count = 0;
+            z->active_bits = 0;
+            z->deg_sum = 0;
+        }
+        grid.sync();
+        n_front = ld_dev_u32(&c->next_count);
+        prev_bits = ld_dev_u64(&c->active_bits);
+        prev_deg = ld_dev_u64(&c->deg_sum);
+        u64 *tf = cur;
+        cur = nxt;
+        nxt = tf;
+        uint32_t *tq = qcur;
+        qcur = qnext;
+        qnext = tq;
+    }
+
+    if (first) {
+        p.flags[FlagLevels] = L;
+        p.flags[FlagPushLevels] = push_levels;
+        p.flags[FlagPullLevels] = pull_levels;
+    }
+
+    // Final sweep: sentinels for unreached pairs, reachability words, and
+    // the visited words cleared so the workspace starts clean next time.
+    const uint64_t ntiles = (uint64_t)((V + TV - 1) / TV) * WG;
+    for (uint64_t t = gwave; t < ntiles; t += nwaves) {
+        const uint32_t v = (uint32_t)(t / WG) * TV + vi;
+        const uint32_t w = (uint32_t)(t % WG) * TW + wi;
+        const bool ok = v < V && w < W;
+        u64 s = 0, miss = 0;
+        if (ok) {
+            const uint64_t idx = (uint64_t)w * V + v;
+            s = p.seen[idx];
+            if (s) p.seen[idx] = 0;
+            miss = ~s & word_valid(p.S, w);
+        }
+        if (p.out) {
+            u64 un = wave_or64(miss);
+            const bool u16 = p.out_u16 != 0;
+            while (un) {
+                const uint32_t b = (uint32_t)__builtin_ctzll(un);
+                un &= un - 1;
+                if ((miss >> b) & 1ull) store_sentinel(p.out, u16, ((uint64_t)w * 64 + b) * V + v);
+            }
+        }
+        if (p.reach && ok) {
+            const u64 r = s & word_valid(p.S, w);
+            const uint64_t base = (uint64_t)v * p.reach_wpv + 2 * w;
+            p.reach[base] = (uint32_t)r;
+            if (2 * w + 1 < p.reach_wpv) p.reach[base + 1] = (uint32_t)(r >> 32);
+        }
+    }
+}
+
+template <int TW>
+static hipError_t wide_launch_t(const WideParams &p, int blocks, int threads, hipStream_t st) {
+    void *args[] = {(void *)&p};
+    return hipLaunchCooperativeKernel((const void *)wide_bfs_kernel<TW>, dim3(blocks),
+                                      dim3(threads), args, 0, st);
+}
+
+template <int TW>
+static hipError_t wide_occ_t(int threads, int *out) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, wide_bfs_kernel<TW>, threads, 0);
+}
+
+hipError_t launch_wide(int tw, const WideParams &p, int blocks, int threads, hipStream_t st) {
+    switch (tw) {
+    case 1: return wide_launch_t<1>(p, blocks, threads, st);
+    case 2: return wide_launch_t<2>(p, blocks, threads, st);
+    case 4: return wide_launch_t<4>(p, blocks, threads, st);
+    case 8: return wide_launch_t<8>(p, blocks, threads, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t wide_max_blocks_per_cu(int tw, int threads, int *out) {
+    switch (tw) {
+    case 1: return wide_occ_t<1>(threads, out);
+    case 2: return wide_occ_t<2>(threads, out);
+    case 4: return wide_occ_t<4>(threads, out);
+    case 8: return wide_occ_t<8>(threads, out);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace msbfs

@@ -1,0 +1,99 @@
+// Device-side data structures and launch wrappers shared by the kernels
+// (kernels.hip) and the host engine (engine.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace msbfs {
+
+// ---------------------------------------------------------------- deep family
+// One task = one group of up to B sources of one graph.  A workgroup keeps
+// the task's visited bits (B bits per vertex) in LDS and runs every BFS
+// level of the task with workgroup barriers only.
+struct DeepTask {
+    uint32_t graph;    // index into the graph table
+    uint32_t nsrc;     // sources in this task (<= B)
+    uint64_t src_off;  // index of the task's first source in `sources`
+    uint64_t out_off;  // element offset of the task's first output row
+    uint64_t reach_bit;  // global source index of bit 0 (reachability output)
+};
+
+struct DeepGraph {
+    const uint32_t *ro;
+    const uint32_t *ci;
+    uint32_t V;
+    uint32_t pad;
+};
+
+struct DeepParams {
+    const DeepTask *tasks;
+    const DeepGraph *graphs;
+    const uint32_t *sources;
+    void *out;             // u32 or u16 distances, or null
+    void *queues;          // per block: 2 * cap entries
+    uint64_t cap;          // entries per queue
+    uint32_t *work;        // [0]: next task; zeroed before launch
+    uint32_t *flags;       // see Flag* below
+    uint32_t *reach;       // reachability words or null
+    uint32_t reach_wpv;
+    uint32_t ntasks;
+    uint32_t out_u16;
+    uint32_t pad;
+};
+
+// ---------------------------------------------------------------- wide family
+struct WideCounters {
+    uint32_t next_count;   // vertices appended to the next frontier queue
+    uint32_t cand_count;   // push: candidate vertices touched this level
+    uint32_t pad0, pad1;
+    unsigned long long active_bits;  // newly set (source, vertex) bits
+    unsigned long long deg_sum;      // out-degree sum of the next frontier
+};
+
+struct WideParams {
+    uint32_t V, W;          // vertices, 64-bit source words
+    uint64_t S, E;
+    const uint32_t *ro, *ci;  // CSR (push)
+    const uint32_t *co, *ri;  // CSC (pull); alias CSR when symmetric
+    unsigned long long *seen, *frA, *frB, *nx;  // word-major [W][V]
+    uint32_t *qf0, *qf1, *qc, *stamp;           // [V]
+    const uint32_t *sources;
+    WideCounters *ctr;      // ring of 4
+    uint32_t *flags;
+    void *out;
+    uint32_t *reach;
+    uint32_t reach_wpv;
+    uint32_t tick_base;
+    uint32_t out_u16;
+    uint32_t policy;        // msbfs_direction_policy
+    uint32_t alpha, beta;
+    uint32_t rule;          // 0 reference estimate, 1 exact degree sum
+    uint32_t pad;
+};
+
+// flags[] layout (device, u32)
+enum : int {
+    FlagU16Overflow = 0,
+    FlagQueueOverflow = 1,
+    FlagLevels = 2,
+    FlagPushLevels = 3,
+    FlagPullLevels = 4,
+    FlagCount = 8
+};
+
+hipError_t launch_deep(int bits, const DeepParams &p, int blocks, int threads, size_t lds_bytes,
+                       hipStream_t st);
+hipError_t deep_max_blocks_per_cu(int bits, int threads, size_t lds_bytes, int *out);
+hipError_t deep_prepare(int bits, size_t lds_bytes);
+
+hipError_t launch_wide(int tile_words, const WideParams &p, int blocks, int threads,
+                       hipStream_t st);
+hipError_t wide_max_blocks_per_cu(int tile_words, int threads, int *out);
+hipError_t launch_wide_seed(const WideParams &p, hipStream_t st);
+
+hipError_t launch_zero_u32(uint32_t *ptr, uint64_t n, hipStream_t st);
+
+constexpr int kWideThreads = 512;
+
+}  // namespace msbfs

@@ -1,0 +1,157 @@
+"""Loads libmsbfs.so (the HIP engine and its C ABI, include/msbfs.h) with ctypes.
+
+The library is built in-tree by `make -C webgpu-bfs_amd/csrc` (or
+`__graft_entry__.build()`).  There is no fallback: if the library is missing
+every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "libmsbfs.so"))
+
+OK = 0
+ERR_INVALID_ARG = 1
+ERR_SOURCE_OOB = 2
+ERR_EMPTY_GRAPH = 3
+ERR_OUT_TOO_SMALL = 4
+ERR_HIP = 5
+ERR_OOM = 6
+ERR_DIAMETER_U16 = 7
+ERR_INVALID_GRAPH = 8
+ERR_UNSUPPORTED = 9
+ERR_NO_DEVICE = 10
+ERR_BUSY = 11
+
+DIST_U32 = 0
+DIST_U16 = 1
+
+FAMILY_NONE, FAMILY_DEEP, FAMILY_WIDE = 0, 1, 2
+
+
+class MsbfsError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(message)
+        self.code = code
+
+
+class SourceOutOfBoundsError(MsbfsError, ValueError):
+    pass
+
+
+class NoDeviceError(MsbfsError):
+    pass
+
+
+class DeviceInfo(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 64), ("arch", ctypes.c_char * 32),
+                ("compute_units", ctypes.c_int32), ("lds_bytes_per_block", ctypes.c_int32),
+                ("total_mem_bytes", ctypes.c_uint64), ("cooperative_launch", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
+class GraphInfo(ctypes.Structure):
+    _fields_ = [("num_vertices", ctypes.c_uint64), ("num_edges", ctypes.c_uint64),
+                ("max_degree", ctypes.c_uint32), ("symmetric", ctypes.c_uint32),
+                ("device_bytes", ctypes.c_uint64)]
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint32) for n in (
+        "alpha", "beta", "tile_words", "words_per_lane", "deep_bits", "deep_threads",
+        "wide_blocks_per_cu", "direction_rule")]
+
+
+class RunStats(ctypes.Structure):
+    _fields_ = [("method", ctypes.c_int32), ("family", ctypes.c_int32),
+                ("levels", ctypes.c_uint32), ("push_levels", ctypes.c_uint32),
+                ("pull_levels", ctypes.c_uint32), ("instances", ctypes.c_uint32),
+                ("bits_per_instance", ctypes.c_uint32), ("grid_blocks", ctypes.c_uint32),
+                ("kernel_ms", ctypes.c_float), ("total_ms", ctypes.c_float),
+                ("workspace_bytes", ctypes.c_uint64)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+P = ctypes.c_void_p
+U32 = ctypes.c_uint32
+U64 = ctypes.c_uint64
+I32 = ctypes.c_int
+PP = ctypes.POINTER(ctypes.c_void_p)
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "msbfs_last_error": (ctypes.c_char_p, []),
+    "msbfs_version": (ctypes.c_char_p, []),
+    "msbfs_status_string": (ctypes.c_char_p, [I32]),
+    "msbfs_context_create": (I32, [I32, PP]),
+    "msbfs_context_destroy": (I32, [P]),
+    "msbfs_context_info": (I32, [P, ctypes.POINTER(DeviceInfo)]),
+    "msbfs_context_synchronize": (I32, [P]),
+    "msbfs_graph_create": (I32, [P, U64, U64, P, P, PP]),
+    "msbfs_graph_destroy": (I32, [P]),
+    "msbfs_graph_get_info": (I32, [P, ctypes.POINTER(GraphInfo)]),
+    "msbfs_engine_create": (I32, [P, I32, PP]),
+    "msbfs_engine_destroy": (I32, [P]),
+    "msbfs_engine_method": (I32, [P]),
+    "msbfs_engine_set_push_pull": (I32, [P, U32, U32]),
+    "msbfs_engine_set_tuning": (I32, [P, ctypes.POINTER(Tuning)]),
+    "msbfs_engine_get_tuning": (I32, [P, ctypes.POINTER(Tuning)]),
+    "msbfs_engine_last_stats": (I32, [P, ctypes.POINTER(RunStats)]),
+    "msbfs_select_auto_method": (I32, [U64, U64, U32, U64]),
+    "msbfs_engine_resolve": (I32, [P, P, U64, ctypes.POINTER(I32)]),
+    "msbfs_run_ms_bfs": (I32, [P, P, P, U64, I32, P, U64]),
+    "msbfs_run_ms_bfs_device": (I32, [P, P, P, U64, I32, P, U64, P, I32]),
+    "msbfs_engine_check": (I32, [P]),
+    "msbfs_run_ms_reachability": (I32, [P, P, P, U64, P, U64]),
+    "msbfs_host_graph_get": (I32, [P, ctypes.POINTER(U64), ctypes.POINTER(U64), PP, PP]),
+    "msbfs_host_graph_free": (None, [P]),
+    "msbfs_csr_from_edge_list": (I32, [U64, U64, P, P, PP]),
+    "msbfs_csr_from_undirected_pairs": (I32, [U64, U64, P, P, PP]),
+    "msbfs_csr_to_csc": (I32, [U64, U64, P, P, P, P]),
+    "msbfs_csr_is_symmetric": (I32, [U64, U64, P, P]),
+    "msbfs_csr_validate": (I32, [U64, U64, P, P]),
+    "msbfs_gen_grid2d": (I32, [U64, U64, PP]),
+    "msbfs_gen_road_like": (I32, [U64, U64, U64, U64, PP]),
+    "msbfs_gen_rmat": (I32, [U32, U32, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                             ctypes.c_double, U64, I32, PP]),
+    "msbfs_gen_erdos_renyi_np": (I32, [U64, ctypes.c_double, U64, PP]),
+    "msbfs_sample_sources": (I32, [P, U64, U64, U64, I32, P]),
+}
+
+_lib = None
+
+
+def lib():
+    """Returns the loaded libmsbfs, raising if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"libmsbfs.so not found at {LIB_PATH}; build it with "
+                "`make -C webgpu-bfs_amd/csrc` (hipcc, gfx950)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    return lib().msbfs_last_error().decode(errors="replace")
+
+
+def check(rc: int) -> None:
+    if rc == OK:
+        return
+    msg = last_error() or lib().msbfs_status_string(rc).decode()
+    if rc == ERR_SOURCE_OOB:
+        raise SourceOutOfBoundsError(rc, msg)
+    if rc == ERR_NO_DEVICE:
+        raise NoDeviceError(rc, msg)
+    raise MsbfsError(rc, msg)
