@@ -1,0 +1,261 @@
+#!/usr/bin/env python3
+"""Multi-source BFS benchmark on one MI355X.
+
+Default workload: the road-like high-diameter graph (114,599 V, 239,332
+directed entries) with S=2048 uniform sources, u32 distances, graph and
+output resident in HBM.  One step = one full MS-BFS of all S sources.
+
+Prints one JSON line: GTEPS (traversed edges per second, counted as the
+reference's CLI does: for every source, the out-degrees of all reached
+vertices), sources/s, the roofline of the traversal kernel and the CPU
+baseline (the oracle's parallel-over-sources restatement of the reference's
+Rayon path, timed on this host's cores).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (os.path.join(ROOT, "webgpu-bfs_amd"), ROOT):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+HBM_PEAK_GBS = 8000.0
+
+CONFIGS = {
+    # name: (graph builder, S, dist dtype, non-isolated sources)
+    "road-s2048": ("road", 2048, "u32", False),
+    "road-s256": ("road", 256, "u32", False),
+    "road-s1": ("road", 1, "u32", False),
+    "grid-s64": ("grid256", 64, "u32", False),
+    "rmat-s64": ("rmat", 64, "u16", True),
+    "rmat-s2048": ("rmat", 2048, "u16", True),
+}
+
+
+def parse_args():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="road-s2048", choices=sorted(CONFIGS))
+    ap.add_argument("--method", default="Auto")
+    ap.add_argument("--tune", action="append", default=[], help="tuning knob key=value")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline threads (0: min(16, cores))")
+    ap.add_argument("--cpu-budget-s", type=float, default=20.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-parity", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a plain device copy of 256 MiB (counter calibration)")
+    return ap.parse_args()
+
+
+def level_bytes(dist, S, V, E, deg, dist_bytes):
+    """Algorithmic bytes of one MS-BFS, from its distance matrix.
+
+    Per level L: A_L vertices with some source at distance L, EA_L their
+    out-degree sum, R_L vertices with some source farther than L (or
+    unreached), N_{L+1} (source, vertex) pairs at distance L+1.  With
+    b = 8*ceil(S/64) bytes of source bits per vertex:
+      push(L) = 8 A_L + 4 EA_L + b (A_L + 2 EA_L) + b_d N_{L+1}
+      pull(L) = 4 (V+1) + 4 E + b V + 2 b R_L + b_d N_{L+1}
+      total   = b_d S V + sum_L min(push(L), pull(L)).
+    `dist` is a torch tensor [S, V] on the device (u32 as int32 or u16 as
+    int16)."""
+    import torch
+
+    inf = 0xFFFFFFFF if dist_bytes == 4 else 0xFFFF
+    mask = 0xFFFFFFFF if dist_bytes == 4 else 0xFFFF
+    dev = dist.device
+    maxl = 0
+    chunk = max(1, (1 << 27) // max(V, 1))
+    for s0 in range(0, S, chunk):
+        d = dist[s0:s0 + chunk].to(torch.int64) & mask
+        fin = d[d != inf]
+        if fin.numel():
+            maxl = max(maxl, int(fin.max()))
+    nl = maxl + 2
+    present = torch.zeros((nl, V), dtype=torch.bool, device=dev)
+    hist = torch.zeros(nl + 1, dtype=torch.int64, device=dev)
+    dmax = torch.zeros(V, dtype=torch.int64, device=dev)
+    vcol = torch.arange(V, device=dev)
+    te = 0
+    degd = torch.as_tensor(deg, dtype=torch.int64, device=dev)
+    for s0 in range(0, S, chunk):
+        d = dist[s0:s0 + chunk].to(torch.int64) & mask
+        reached = d != inf
+        te += int((reached.to(torch.int64) * degd[None, :]).sum())
+        dd = torch.where(reached, d, torch.full_like(d, nl))
+        dmax = torch.maximum(dmax, dd.max(dim=0).values)
+        hist += torch.bincount(dd.flatten(), minlength=nl + 1)[: nl + 1]
+        r = reached.nonzero()
+        present[d[r[:, 0], r[:, 1]], vcol[r[:, 1]]] = True
+    A = present.sum(dim=1).to(torch.int64)
+    EA = (present.to(torch.float64) @ degd.to(torch.float64)).round().to(torch.int64)
+    cnt_le = torch.cumsum(torch.bincount(torch.clamp(dmax, max=nl), minlength=nl + 1), 0)
+    W = (S + 63) // 64
+    b = 8 * W
+    total = dist_bytes * S * V
+    A, EA, cnt_le, hist = A.tolist(), EA.tolist(), cnt_le.tolist(), hist.tolist()
+    for L in range(maxl + 1):
+        R = V - cnt_le[L]
+        N = hist[L + 1] if L + 1 <= maxl else 0
+        push = 8 * A[L] + 4 * EA[L] + b * (A[L] + 2 * EA[L]) + dist_bytes * N
+        pull = 4 * (V + 1) + 4 * E + b * V + 2 * b * R + dist_bytes * N
+        total += min(push, pull)
+    return total, te, maxl
+
+
+def calibrate_copy():
+    import torch
+
+    n = 256 << 20
+    x = torch.ones(n // 4, dtype=torch.int32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "device copy", "bytes_read": n, "bytes_written": n,
+                      "launches": 3}))
+
+
+def main():
+    args = parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
+                          "gpus_requested": args.gpus}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import gpu_bfs as gb
+    from gpu_bfs import workloads as wl
+
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
+
+    gname, S, dname, non_iso = CONFIGS[args.config]
+    t0 = time.time()
+    graph = getattr(wl, gname)()
+    src = wl.sources(graph, S, non_isolated=non_iso)
+    V, E = graph.num_vertices, graph.num_edges
+    gen_s = time.time() - t0
+    print(f"[bench] workload {args.config}: V={V} E={E} S={S} dist={dname} (built in {gen_s:.1f}s)",
+          file=sys.stderr, flush=True)
+
+    ctx = gb.GpuContext(0)
+    eng = gb.GpuBfsEngine(ctx, gb.GpuBfsMethod[args.method])
+    if args.tune:
+        eng.set_tuning(**{k: int(v) for k, v in (t.split("=") for t in args.tune)})
+    dg = gb.DeviceGraph(ctx, graph)
+    n = S * V
+    tdt = torch.int32 if dname == "u32" else torch.int16
+    npdt = np.uint32 if dname == "u32" else np.uint16
+    out = torch.empty(n, dtype=tdt, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def step(sync):
+        eng.run_ms_bfs_device(dg, src, out.data_ptr(), n, dist_dtype=npdt, stream=stream,
+                              sync=sync)
+
+    for _ in range(args.warmup):
+        step(True)
+    torch.cuda.synchronize()
+    print("[bench] warmup done", file=sys.stderr, flush=True)
+
+    kernel_ms = []
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        step(False)
+        kernel_ms.append(eng.last_stats()["kernel_ms"])
+    torch.cuda.synchronize()
+    t_end = time.perf_counter()
+    stats = eng.last_stats()
+    ms_per_step = (t_end - t_start) * 1e3 / args.steps
+    print(f"[bench] timed {args.steps} steps: {ms_per_step:.3f} ms/step", file=sys.stderr, flush=True)
+
+    deg = graph.degrees()
+    dist2d = out.view(S, V)
+    alg_bytes, te, maxl = level_bytes(dist2d, S, V, E, deg, 4 if dname == "u32" else 2)
+    kmean = float(np.mean(kernel_ms))
+    gteps = te / (ms_per_step / 1e3) / 1e9
+    achieved = alg_bytes / (kmean / 1e3) / 1e9
+
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{args.config}.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    result = {
+        "metric": "MS-BFS GTEPS & sources/s at S=2048 on 1 GPU; served p50/p99 latency vs plain",
+        "value": round(gteps, 3),
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64 source words / u32 distances" if dname == "u32" else "u64 source words / u16 distances",
+        "data": "synthetic (seeded generators; graph and distances resident in HBM)",
+        "config": {"workload": args.config, "graph": gname, "num_vertices": V,
+                   "num_edges": E, "num_sources": S, "dist": dname, "method": args.method,
+                   "family": ["none", "deep", "wide"][stats["family"]],
+                   "levels": int(stats["levels"]), "max_distance": maxl,
+                   "bits_per_instance": int(stats["bits_per_instance"]),
+                   "grid_blocks": int(stats["grid_blocks"]), "parallelism": "single GPU"},
+        "sources_per_s": round(S / (ms_per_step / 1e3), 1),
+        "traversed_edges": te,
+        "kernel_ms": round(kmean, 4),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": traffic, "algorithmic_bytes": alg_bytes,
+                     "kernel": "deep_bfs_kernel" if stats["family"] == 1 else "wide_bfs_kernel"},
+    }
+
+    if not args.no_cpu_baseline:
+        from oracle import oracle as orc
+
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        # Bounded sample: as many sources as fit the time budget, estimated
+        # from a 16-source probe.
+        t = time.perf_counter()
+        orc.ms_bfs_parallel(V, graph.row_offsets, graph.col_indices, src[:16], threads)
+        per_src = (time.perf_counter() - t) / 16
+        ns = int(min(S, max(16, args.cpu_budget_s / 4 / max(per_src, 1e-9))))
+        sample = src[:ns]
+        times = []
+        cpu_out = None
+        deadline = time.perf_counter() + args.cpu_budget_s
+        while len(times) < 3 and (not times or time.perf_counter() < deadline):
+            t = time.perf_counter()
+            cpu_out = orc.ms_bfs_parallel(V, graph.row_offsets, graph.col_indices, sample, threads)
+            times.append(time.perf_counter() - t)
+        cpu_s = float(np.median(times))
+        cpu_te = orc.traversed_edges(V, graph.row_offsets, ns, cpu_out)
+        result["cpu_baseline"] = {
+            "value": round(cpu_te / cpu_s / 1e9, 4), "unit": "GTEPS", "cores": threads,
+            "kind": "port",
+            "sample": f"{ns} of {S} sources, {len(times)} timed runs (median {cpu_s*1e3:.1f} ms), "
+                      "parallel-over-sources FIFO BFS incl. output allocation and fill",
+            "sources_per_s": round(ns / cpu_s, 1)}
+        if not args.no_parity:
+            gpu = dist2d[:ns].cpu().numpy().view(npdt).reshape(-1)
+            ref = cpu_out.astype(np.uint64)
+            if npdt == np.uint16:
+                ref = np.where(cpu_out == 0xFFFFFFFF, 0xFFFF, cpu_out).astype(np.uint16)
+            result["parity"] = bool(np.array_equal(gpu, ref.astype(npdt)))
+            result["parity_sources_checked"] = ns
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

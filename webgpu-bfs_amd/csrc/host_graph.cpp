@@ -356,8 +356,9 @@ int msbfs_gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, dou
     return MSBFS_OK;
 }
 
-// G(n, p) by Batagelj-Brandes geometric skipping
-// (crates/core/src/generators.rs:139-191), undirected.
+// G(n, p) by Batagelj-Brandes geometric skipping (Batagelj & Brandes,
+// "Efficient generation of large random networks", 2005), undirected; the
+// reference uses the same method (crates/core/src/generators.rs:139-191).
 int msbfs_gen_erdos_renyi_np(uint64_t V, double p, uint64_t seed, msbfs_host_graph **out) {
     if (!out) return fail(MSBFS_ERR_INVALID_ARG, "null out");
     if (V == 0) return fail(MSBFS_ERR_INVALID_ARG, "G(n,p) needs n >= 1");
@@ -382,10 +383,14 @@ int msbfs_gen_erdos_renyi_np(uint64_t V, double p, uint64_t seed, msbfs_host_gra
                 double kf = std::floor(std::log(r) / lq);
                 uint64_t k = kf > 1e18 ? (uint64_t)1e18 : (uint64_t)kf;
                 v += k;
+                // Carry the overflow into the next row, whose first
+                // candidate is u + 2.  (The reference writes `v += u` here,
+                // generators.rs:170-175, which skips u extra pairs per row
+                // and halves the expected edge count.)
                 while (v >= V && u < V - 1) {
                     v -= V - 1 - u;
                     u += 1;
-                    v += u;
+                    v += 1;
                 }
                 if (u < V - 1) {
                     pairs.push_back((u << 32) | v);

@@ -149,17 +149,64 @@ __device__ __forceinline__ void unpack_entry(u64 x, uint32_t &v, uint32_t &m) {
     m = (uint32_t)(x >> 32);
 }
 
+// A frontier queue: the first `lcap` entries live in LDS, the rest spill to
+// this block's region of global memory.
+template <typename Entry>
+struct DeepQueue {
+    Entry *lds;
+    Entry *glb;
+    uint32_t lcap;
+    uint64_t gcap;
+    __device__ __forceinline__ Entry get(uint32_t i) const {
+        return i < lcap ? lds[i] : glb[i - lcap];
+    }
+    __device__ __forceinline__ bool put(uint32_t i, Entry x) const {
+        if (i < lcap) {
+            lds[i] = x;
+            return true;
+        }
+        if ((uint64_t)(i - lcap) < gcap) {
+            glb[i - lcap] = x;
+            return true;
+        }
+        return false;
+    }
+};
+
 template <int B>
+__device__ __forceinline__ void deep_write_bits(const DeepParams &p, uint64_t out_off, uint32_t V,
+                                                uint32_t v, uint32_t bits, uint32_t d) {
+    const bool u16 = p.out_u16 != 0;
+    while (bits) {
+        const uint32_t b = __builtin_ctz(bits);
+        store_dist(p.out, u16, out_off + (uint64_t)b * V + v, d, p.flags);
+        bits &= bits - 1;
+    }
+}
+
+// One workgroup per task (a group of <= B sources of one graph).  Visited
+// bits in LDS; frontier queues in LDS with global spill.  With an ELL
+// neighbour table (graphs of degree <= ELL) a queue entry costs one vector
+// load of its neighbour slots, its claims are independent LDS atomics, and
+// the wave appends all its new entries with one LDS atomic.
+template <int B, int ELL>
 __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
     typedef typename DeepEntry<B>::T Entry;
     constexpr uint32_t VPW = 32 / B;  // vertices per LDS word
     constexpr uint32_t FULL = (B == 32) ? 0xFFFFFFFFu : ((1u << B) - 1u);
-    extern __shared__ uint32_t vis[];
+    extern __shared__ uint32_t lds[];
     __shared__ uint32_t s_task;
     __shared__ uint32_t s_cnt[2];
 
+    uint32_t *const vis = lds;
     const uint32_t tid = threadIdx.x, nth = blockDim.x;
-    Entry *const qbase = (Entry *)p.queues + (uint64_t)blockIdx.x * 2 * p.cap;
+    const uint32_t lane = lane_id();
+    const u64 lt_mask = lane ? ((~0ull) >> (64 - lane)) : 0ull;
+    Entry *const lq0 = (Entry *)(lds + p.lds_vis_words);
+    Entry *const lq1 = lq0 + p.lds_qcap;
+    const uint64_t gcap = p.cap - p.lds_qcap;  // spill entries per queue
+    Entry *const gq0 = (Entry *)p.queues + (uint64_t)blockIdx.x * 2 * gcap;
+    Entry *const gq1 = gq0 + gcap;
     const bool u16 = p.out_u16 != 0;
 
     for (;;) {
@@ -178,12 +225,13 @@ __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
         }
         __syncthreads();
 
-        Entry *qa = qbase, *qb = qbase + p.cap;
+        DeepQueue<Entry> qa{lq0, gq0, p.lds_qcap, gcap};
+        DeepQueue<Entry> qb{lq1, gq1, p.lds_qcap, gcap};
         if (tid < task.nsrc) {
             const uint32_t s = p.sources[task.src_off + tid];
             const uint32_t bit = 1u << tid;
             atomicOr(&vis[s / VPW], bit << ((s % VPW) * B));
-            qa[tid] = pack_entry(s, bit, (Entry *)nullptr);
+            qa.put(tid, pack_entry(s, bit, (Entry *)nullptr));
             if (p.out) store_dist(p.out, u16, task.out_off + (uint64_t)tid * V + s, 0, p.flags);
         }
         __syncthreads();
@@ -196,47 +244,97 @@ __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
             const uint32_t nd = level + 1;
             for (uint32_t base = 0; base < n; base += nth) {
                 const uint32_t i = base + tid;
-                uint32_t u = 0, f = 0, e = 0, eend = 0;
-                if (i < n) {
-                    unpack_entry(qa[i], u, f);
-                    e = g.ro[u];
-                    eend = g.ro[u + 1];
-                }
-                // Wave-uniform edge loop so that every lane takes part in the
-                // queue reservation.
-                while (__any(e < eend)) {
-                    uint32_t v = 0, newly = 0;
-                    if (e < eend) {
-                        v = g.ci[e];
-                        const uint32_t wi = v / VPW, sh = (v % VPW) * B;
-                        const uint32_t m = f & ~((vis[wi] >> sh) & FULL);
-                        if (m) {
-                            const uint32_t old = atomicOr(&vis[wi], m << sh);
-                            newly = m & ~((old >> sh) & FULL);
+                uint32_t u = 0, f = 0;
+                if (i < n) unpack_entry(qa.get(i), u, f);
+                if constexpr (ELL > 0) {
+                    // All neighbour slots in one or two 16-byte loads.
+                    uint32_t nb[ELL];
+                    if (i < n) {
+                        const uint4 *row = (const uint4 *)(g.ell + (uint64_t)u * ELL);
+#pragma unroll
+                        for (int k = 0; k < ELL / 4; ++k) {
+                            const uint4 x = row[k];
+                            nb[4 * k] = x.x;
+                            nb[4 * k + 1] = x.y;
+                            nb[4 * k + 2] = x.z;
+                            nb[4 * k + 3] = x.w;
                         }
-                        ++e;
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < ELL; ++k) nb[k] = 0xFFFFFFFFu;
                     }
-                    const uint32_t pos = wave_reserve(cnt_next, newly != 0);
-                    if (newly) {
-                        if (pos < p.cap)
-                            qb[pos] = pack_entry(v, newly, (Entry *)nullptr);
-                        else
-                            atomicOr(&p.flags[FlagQueueOverflow], 1u);
-                        if (p.out) {
-                            uint32_t bits = newly;
-                            while (bits) {
-                                const uint32_t b = __builtin_ctz(bits);
-                                store_dist(p.out, u16, task.out_off + (uint64_t)b * V + v, nd,
-                                           p.flags);
-                                bits &= bits - 1;
+                    uint32_t newly[ELL];
+                    uint32_t c = 0;
+#pragma unroll
+                    for (int k = 0; k < ELL; ++k) {
+                        newly[k] = 0;
+                        const uint32_t v = nb[k];
+                        if (v != 0xFFFFFFFFu) {
+                            const uint32_t wi = v / VPW, sh = (v % VPW) * B;
+                            const uint32_t m = f & ~((vis[wi] >> sh) & FULL);
+                            if (m) {
+                                const uint32_t old = atomicOr(&vis[wi], m << sh);
+                                newly[k] = m & ~((old >> sh) & FULL);
                             }
+                        }
+                        c += newly[k] != 0;
+                    }
+                    // Exclusive prefix of c over the wave from ELL ballots:
+                    // lanes below with c > j, summed over j.
+                    uint32_t pre = 0, tot = 0;
+#pragma unroll
+                    for (int j = 0; j < ELL; ++j) {
+                        const u64 bj = __ballot(c > (uint32_t)j);
+                        pre += (uint32_t)__popcll(bj & lt_mask);
+                        tot += (uint32_t)__popcll(bj);
+                    }
+                    uint32_t wbase = 0;
+                    if (tot) {
+                        if (lane == 0) wbase = atomicAdd(cnt_next, tot);
+                        wbase = __shfl(wbase, 0);
+                    }
+                    uint32_t pos = wbase + pre;
+#pragma unroll
+                    for (int k = 0; k < ELL; ++k) {
+                        if (newly[k]) {
+                            if (!qb.put(pos, pack_entry(nb[k], newly[k], (Entry *)nullptr)))
+                                atomicOr(&p.flags[FlagQueueOverflow], 1u);
+                            ++pos;
+                            if (p.out) deep_write_bits<B>(p, task.out_off, V, nb[k], newly[k], nd);
+                        }
+                    }
+                } else {
+                    uint32_t e = 0, eend = 0;
+                    if (i < n) {
+                        e = g.ro[u];
+                        eend = g.ro[u + 1];
+                    }
+                    // Wave-uniform edge loop so that every lane takes part
+                    // in the queue reservation.
+                    while (__any(e < eend)) {
+                        uint32_t v = 0, newly = 0;
+                        if (e < eend) {
+                            v = g.ci[e];
+                            const uint32_t wi = v / VPW, sh = (v % VPW) * B;
+                            const uint32_t m = f & ~((vis[wi] >> sh) & FULL);
+                            if (m) {
+                                const uint32_t old = atomicOr(&vis[wi], m << sh);
+                                newly = m & ~((old >> sh) & FULL);
+                            }
+                            ++e;
+                        }
+                        const uint32_t pos = wave_reserve(cnt_next, newly != 0);
+                        if (newly) {
+                            if (!qb.put(pos, pack_entry(v, newly, (Entry *)nullptr)))
+                                atomicOr(&p.flags[FlagQueueOverflow], 1u);
+                            if (p.out) deep_write_bits<B>(p, task.out_off, V, v, newly, nd);
                         }
                     }
                 }
             }
             __syncthreads();
             if (tid == 0) s_cnt[level & 1] = 0;
-            Entry *tq = qa;
+            DeepQueue<Entry> tq = qa;
             qa = qb;
             qb = tq;
             ++level;
@@ -268,61 +366,57 @@ __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
     }
 }
 
-template <int B>
+template <int B, int ELL>
 static hipError_t deep_launch_t(const DeepParams &p, int blocks, int threads, size_t lds,
                                 hipStream_t st) {
-    hipLaunchKernelGGL(deep_bfs_kernel<B>, dim3(blocks), dim3(threads), lds, st, p);
+    hipLaunchKernelGGL((deep_bfs_kernel<B, ELL>), dim3(blocks), dim3(threads), lds, st, p);
     return hipGetLastError();
 }
 
-template <int B>
+template <int B, int ELL>
 static hipError_t deep_prepare_t(size_t lds) {
-    return hipFuncSetAttribute((const void *)deep_bfs_kernel<B>,
+    return hipFuncSetAttribute((const void *)deep_bfs_kernel<B, ELL>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 }
 
-template <int B>
+template <int B, int ELL>
 static hipError_t deep_occ_t(int threads, size_t lds, int *out) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, deep_bfs_kernel<B>, threads, lds);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, deep_bfs_kernel<B, ELL>, threads, lds);
 }
 
-hipError_t launch_deep(int bits, const DeepParams &p, int blocks, int threads, size_t lds,
-                       hipStream_t st) {
-#define CALL(B) deep_launch_t<B>(p, blocks, threads, lds, st)
-    switch (bits) {
-    case 1: return CALL(1);
-    case 2: return CALL(2);
-    case 4: return CALL(4);
-    case 8: return CALL(8);
-    case 16: return CALL(16);
-    case 32: return CALL(32);
-    default: return hipErrorInvalidValue;
+// Dispatches a templated call over bits in {1,2,4,8,16,32} x ell in {0,4,8}.
+#define DEEP_SWITCH(FN, ...)                                                 \
+    switch (bits * 16 + ell) {                                               \
+    case 1 * 16 + 0: return FN<1, 0>(__VA_ARGS__);                           \
+    case 1 * 16 + 4: return FN<1, 4>(__VA_ARGS__);                           \
+    case 1 * 16 + 8: return FN<1, 8>(__VA_ARGS__);                           \
+    case 2 * 16 + 0: return FN<2, 0>(__VA_ARGS__);                           \
+    case 2 * 16 + 4: return FN<2, 4>(__VA_ARGS__);                           \
+    case 2 * 16 + 8: return FN<2, 8>(__VA_ARGS__);                           \
+    case 4 * 16 + 0: return FN<4, 0>(__VA_ARGS__);                           \
+    case 4 * 16 + 4: return FN<4, 4>(__VA_ARGS__);                           \
+    case 4 * 16 + 8: return FN<4, 8>(__VA_ARGS__);                           \
+    case 8 * 16 + 0: return FN<8, 0>(__VA_ARGS__);                           \
+    case 8 * 16 + 4: return FN<8, 4>(__VA_ARGS__);                           \
+    case 8 * 16 + 8: return FN<8, 8>(__VA_ARGS__);                           \
+    case 16 * 16 + 0: return FN<16, 0>(__VA_ARGS__);                         \
+    case 16 * 16 + 4: return FN<16, 4>(__VA_ARGS__);                         \
+    case 16 * 16 + 8: return FN<16, 8>(__VA_ARGS__);                         \
+    case 32 * 16 + 0: return FN<32, 0>(__VA_ARGS__);                         \
+    case 32 * 16 + 4: return FN<32, 4>(__VA_ARGS__);                         \
+    case 32 * 16 + 8: return FN<32, 8>(__VA_ARGS__);                         \
+    default: return hipErrorInvalidValue;                                    \
     }
-#undef CALL
+
+hipError_t launch_deep(int bits, int ell, const DeepParams &p, int blocks, int threads,
+                       size_t lds, hipStream_t st) {
+    DEEP_SWITCH(deep_launch_t, p, blocks, threads, lds, st)
 }
 
-hipError_t deep_prepare(int bits, size_t lds) {
-    switch (bits) {
-    case 1: return deep_prepare_t<1>(lds);
-    case 2: return deep_prepare_t<2>(lds);
-    case 4: return deep_prepare_t<4>(lds);
-    case 8: return deep_prepare_t<8>(lds);
-    case 16: return deep_prepare_t<16>(lds);
-    case 32: return deep_prepare_t<32>(lds);
-    default: return hipErrorInvalidValue;
-    }
-}
+hipError_t deep_prepare(int bits, int ell, size_t lds) { DEEP_SWITCH(deep_prepare_t, lds) }
 
-hipError_t deep_max_blocks_per_cu(int bits, int threads, size_t lds, int *out) {
-    switch (bits) {
-    case 1: return deep_occ_t<1>(threads, lds, out);
-    case 2: return deep_occ_t<2>(threads, lds, out);
-    case 4: return deep_occ_t<4>(threads, lds, out);
-    case 8: return deep_occ_t<8>(threads, lds, out);
-    case 16: return deep_occ_t<16>(threads, lds, out);
-    case 32: return deep_occ_t<32>(threads, lds, out);
-    default: return hipErrorInvalidValue;
-    }
+hipError_t deep_max_blocks_per_cu(int bits, int ell, int threads, size_t lds, int *out) {
+    DEEP_SWITCH(deep_occ_t, threads, lds, out)
 }
 
 // ---------------------------------------------------------------- wide family

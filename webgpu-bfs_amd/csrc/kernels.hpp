@@ -22,8 +22,9 @@ struct DeepTask {
 struct DeepGraph {
     const uint32_t *ro;
     const uint32_t *ci;
+    const uint32_t *ell;   // V * ell_width neighbour slots (0xFFFFFFFF = empty) or null
     uint32_t V;
-    uint32_t pad;
+    uint32_t ell_width;    // 0 (use CSR), 4 or 8
 };
 
 struct DeepParams {
@@ -31,8 +32,10 @@ struct DeepParams {
     const DeepGraph *graphs;
     const uint32_t *sources;
     void *out;             // u32 or u16 distances, or null
-    void *queues;          // per block: 2 * cap entries
-    uint64_t cap;          // entries per queue
+    void *queues;          // per block: 2 * (cap - lds_qcap) spill entries (global)
+    uint64_t cap;          // entries per queue (LDS part + global spill)
+    uint32_t lds_qcap;     // entries per queue held in LDS
+    uint32_t lds_vis_words;  // LDS words reserved for visited bits
     uint32_t *work;        // [0]: next task; zeroed before launch
     uint32_t *flags;       // see Flag* below
     uint32_t *reach;       // reachability words or null
@@ -82,10 +85,10 @@ enum : int {
     FlagCount = 8
 };
 
-hipError_t launch_deep(int bits, const DeepParams &p, int blocks, int threads, size_t lds_bytes,
-                       hipStream_t st);
-hipError_t deep_max_blocks_per_cu(int bits, int threads, size_t lds_bytes, int *out);
-hipError_t deep_prepare(int bits, size_t lds_bytes);
+hipError_t launch_deep(int bits, int ell, const DeepParams &p, int blocks, int threads,
+                       size_t lds_bytes, hipStream_t st);
+hipError_t deep_max_blocks_per_cu(int bits, int ell, int threads, size_t lds_bytes, int *out);
+hipError_t deep_prepare(int bits, int ell, size_t lds_bytes);
 
 hipError_t launch_wide(int tile_words, const WideParams &p, int blocks, int threads,
                        hipStream_t st);
