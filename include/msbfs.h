@@ -114,8 +114,22 @@ typedef struct msbfs_tuning {
     uint32_t deep_bits;      /* deep family: sources per instance (1,2,4,8,16,32); 0 = auto */
     uint32_t deep_threads;   /* deep family workgroup size (256, 512, 1024) */
     uint32_t wide_blocks_per_cu; /* wide family persistent blocks per CU */
-    uint32_t direction_rule; /* 0: reference estimate n_f*|E|/|V|; 1: exact frontier degree sum */
+    uint32_t direction_rule; /* 0: measured frontier degree sum and size (default);
+                                1: the reference's estimate n_f = ceil(bits/S), m_f = n_f*|E|/|V| */
+    uint32_t trace_levels;   /* wide family: record up to this many levels (0 = off) */
 } msbfs_tuning;
+
+/* One BFS level of the last wide-family run (recorded when trace_levels > 0). */
+typedef struct msbfs_level_record {
+    uint32_t level;
+    uint32_t direction;      /* 0 push, 1 pull */
+    uint32_t frontier;       /* vertices in the level's frontier */
+    uint32_t reserved;
+    uint64_t new_bits;       /* (source, vertex) pairs first reached by the level */
+    double start_us;         /* device clock, relative to level 0 */
+    double mid_us;           /* after expand (push) or gather (pull) */
+    double end_us;
+} msbfs_level_record;
 
 typedef struct msbfs_run_stats {
     int32_t method;          /* method actually run (after Auto) */
@@ -166,6 +180,11 @@ int msbfs_engine_set_push_pull(msbfs_engine *e, uint32_t alpha, uint32_t beta);
 int msbfs_engine_set_tuning(msbfs_engine *e, const msbfs_tuning *t);
 int msbfs_engine_get_tuning(const msbfs_engine *e, msbfs_tuning *t);
 int msbfs_engine_last_stats(msbfs_engine *e, msbfs_run_stats *st);
+/* Per-level trace of the last wide-family run: writes up to max_levels
+ * records and the number recorded.  The reference has no such facility (its
+ * timing is wall clock around whole calls, crates/cli/src/bench.rs:338-370). */
+int msbfs_engine_level_trace(msbfs_engine *e, msbfs_level_record *out, uint32_t max_levels,
+                             uint32_t *n_levels);
 
 /* GpuBfsEngine::select_auto_method (crates/gpu_bfs/src/engine.rs:157-198):
  * the reference's decision table, returned as an msbfs_method. */

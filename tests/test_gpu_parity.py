@@ -106,15 +106,24 @@ def test_road_like_full(engines, road, S):
 
 
 def test_road_like_deep_tile_variants(ctx, road):
-    src = wl.sources(road, 300)
-    expect = oracle(road, src)
-    for bits in (1, 2, 4, 8, 16):
-        for threads in (256, 1024):
-            e = gb.GpuBfsEngine(ctx, M.WorkgroupPersistentBfs)
-            e.set_tuning(deep_bits=bits, deep_threads=threads)
-            say("road S=300 deep bits", bits, "threads", threads)
-            assert np.array_equal(e.run_ms_bfs(road, src), expect), (bits, threads)
-            assert e.last_stats()["bits_per_instance"] == bits
+    """Every task width whose visited bits fit in LDS, at two workgroup
+    sizes; widths that do not fit fall back to the wide family."""
+    small = gb.generate_road_like(20_000, 160, 900, seed=3)
+    for g, widths in ((road, (1, 2, 4, 8)), (small, (16, 32))):
+        src = wl.sources(g, 300)
+        expect = oracle(g, src)
+        for bits in widths:
+            for threads in (256, 1024):
+                e = gb.GpuBfsEngine(ctx, M.WorkgroupPersistentBfs)
+                e.set_tuning(deep_bits=bits, deep_threads=threads)
+                say("road V=%d S=300 deep bits" % g.num_vertices, bits, "threads", threads)
+                assert np.array_equal(e.run_ms_bfs(g, src), expect), (bits, threads)
+                assert e.last_stats()["bits_per_instance"] == bits
+    e = gb.GpuBfsEngine(ctx, M.WorkgroupPersistentBfs)
+    e.set_tuning(deep_bits=16)  # 16 bits x 114,599 vertices exceed LDS
+    src = wl.sources(road, 64)
+    assert np.array_equal(e.run_ms_bfs(road, src), oracle(road, src))
+    assert e.last_stats()["family"] == int(gb.Family.WIDE)
 
 
 @pytest.fixture(scope="module")

@@ -113,6 +113,10 @@ struct msbfs_graph {
     uint32_t *ro = nullptr, *ci = nullptr, *co = nullptr, *ri = nullptr;
     uint32_t *ell = nullptr;  // V x ell_width neighbour slots, 0xFFFFFFFF padded
     int ell_width = 0;        // 4 or 8 when max degree allows, else 0
+    uint32_t *heavy_out = nullptr, *heavy_in = nullptr;  // degree > kHeavyDegree
+    uint32_t n_heavy_out = 0, n_heavy_in = 0;
+    uint32_t *hchunk_out = nullptr, *hchunk_in = nullptr;  // uint4 (v, e0, e1, 0) items
+    uint32_t n_hchunk_out = 0, n_hchunk_in = 0;
     uint64_t bytes = 0;
 };
 
@@ -122,14 +126,16 @@ struct msbfs_engine {
     msbfs_tuning tune{};
     std::mutex mu;
     // wide workspace (kept all-zero between runs by the kernels)
-    DevBuf seen, frA, frB, nx, qf0, qf1, qc, stamp, ctr;
+    DevBuf seen, frA, frB, nx, qf0, qf1, qc, stamp, ctr, fbA, fbB;
     uint64_t wide_words = 0;  // entries allocated per state array
     uint32_t tick_base = 1;
     bool dirty = true;
     // deep workspace
     DevBuf queues, tasks, graphs, work;
     // shared
-    DevBuf flags, sources, out_tmp, reach_tmp;
+    DevBuf flags, sources, out_tmp, reach_tmp, trace;
+    uint32_t trace_cap = 0;
+    bool trace_valid = false;
     PinnedBuf h_sources, h_flags, h_tasks;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     msbfs_run_stats last{};
@@ -251,6 +257,45 @@ extern "C" int msbfs_graph_create(msbfs_context *c, uint64_t V, uint64_t E, cons
             for (uint32_t k = 0; k < ro[v + 1] - ro[v]; ++k) ell[v * g->ell_width + k] = ci[ro[v] + k];
         e = upload(&g->ell, ell.data(), ell.size());
     }
+    // Vertices whose edge lists the wide family hands to a whole wave.
+    if (e == hipSuccess) {
+        std::vector<uint32_t> hout, hin;
+        for (uint64_t v = 0; v < V; ++v)
+            if (ro[v + 1] - ro[v] > kHeavyDegree) hout.push_back((uint32_t)v);
+        // Heavy edge lists split into work items of at most kHeavyChunk edges.
+        auto chunks = [](const std::vector<uint32_t> &hv, const uint32_t *off) {
+            std::vector<uint32_t> out;
+            for (uint32_t v : hv)
+                for (uint32_t b = off[v]; b < off[v + 1]; b += kHeavyChunk) {
+                    out.push_back(v);
+                    out.push_back(b);
+                    out.push_back(std::min(off[v + 1], b + kHeavyChunk));
+                    out.push_back(0);
+                }
+            return out;
+        };
+        g->n_heavy_out = (uint32_t)hout.size();
+        e = upload(&g->heavy_out, hout.data(), hout.size());
+        std::vector<uint32_t> cout = chunks(hout, ro);
+        g->n_hchunk_out = (uint32_t)(cout.size() / 4);
+        if (e == hipSuccess) e = upload(&g->hchunk_out, cout.data(), cout.size());
+        if (e == hipSuccess) {
+            if (g->symmetric) {
+                g->heavy_in = g->heavy_out;
+                g->n_heavy_in = g->n_heavy_out;
+                g->hchunk_in = g->hchunk_out;
+                g->n_hchunk_in = g->n_hchunk_out;
+            } else {
+                for (uint64_t v = 0; v < V; ++v)
+                    if (co[v + 1] - co[v] > kHeavyDegree) hin.push_back((uint32_t)v);
+                g->n_heavy_in = (uint32_t)hin.size();
+                e = upload(&g->heavy_in, hin.data(), hin.size());
+                std::vector<uint32_t> cin = chunks(hin, co.data());
+                g->n_hchunk_in = (uint32_t)(cin.size() / 4);
+                if (e == hipSuccess) e = upload(&g->hchunk_in, cin.data(), cin.size());
+            }
+        }
+    }
     if (e != hipSuccess) {
         msbfs_graph_destroy(g);
         return hip_fail(e, "graph upload");
@@ -267,6 +312,10 @@ extern "C" int msbfs_graph_destroy(msbfs_graph *g) {
     if (g->ro) (void)hipFree(g->ro);
     if (g->ci) (void)hipFree(g->ci);
     if (g->ell) (void)hipFree(g->ell);
+    if (g->heavy_in && g->heavy_in != g->heavy_out) (void)hipFree(g->heavy_in);
+    if (g->heavy_out) (void)hipFree(g->heavy_out);
+    if (g->hchunk_in && g->hchunk_in != g->hchunk_out) (void)hipFree(g->hchunk_in);
+    if (g->hchunk_out) (void)hipFree(g->hchunk_out);
     delete g;
     return MSBFS_OK;
 }
@@ -325,9 +374,8 @@ extern "C" int msbfs_engine_set_push_pull(msbfs_engine *e, uint32_t alpha, uint3
 
 extern "C" int msbfs_engine_set_tuning(msbfs_engine *e, const msbfs_tuning *t) {
     if (!e || !t) return fail(MSBFS_ERR_INVALID_ARG, "null argument");
-    if (t->tile_words && t->tile_words != 1 && t->tile_words != 2 && t->tile_words != 4 &&
-        t->tile_words != 8)
-        return fail(MSBFS_ERR_INVALID_ARG, "tile_words must be 1, 2, 4 or 8");
+    if (t->tile_words && (t->tile_words > 64 || (t->tile_words & (t->tile_words - 1))))
+        return fail(MSBFS_ERR_INVALID_ARG, "tile_words must be a power of two <= 64");
     if (t->deep_bits && (t->deep_bits > 32 || (t->deep_bits & (t->deep_bits - 1))))
         return fail(MSBFS_ERR_INVALID_ARG, "deep_bits must be 1, 2, 4, 8, 16 or 32");
     if (t->deep_threads && (t->deep_threads % 64 || t->deep_threads > 1024))
@@ -510,6 +558,8 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
         if ((rc = ensure_zeroed(e->frA, words * 8, force, st))) return rc;
         if ((rc = ensure_zeroed(e->frB, words * 8, force, st))) return rc;
         if ((rc = ensure_zeroed(e->nx, words * 8, force, st))) return rc;
+        if ((rc = ensure_zeroed(e->fbA, (V / 32 + 1) * 4, force, st))) return rc;
+        if ((rc = ensure_zeroed(e->fbB, (V / 32 + 1) * 4, force, st))) return rc;
         if ((rc = ensure_plain(e->qf0, V * 4))) return rc;
         if ((rc = ensure_plain(e->qf1, V * 4))) return rc;
         if ((rc = ensure_plain(e->qc, V * 4))) return rc;
@@ -552,11 +602,40 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
         p.alpha = e->tune.alpha ? e->tune.alpha : kDefaultAlpha;
         p.beta = e->tune.beta ? e->tune.beta : kDefaultBeta;
         p.rule = e->tune.direction_rule;
+        p.heavy_t = kHeavyDegree;
+        p.heavy_out = g->heavy_out;
+        p.heavy_in = g->heavy_in;
+        p.n_heavy_out = g->n_heavy_out;
+        p.n_heavy_in = g->n_heavy_in;
+        p.fbA = e->fbA.as<uint32_t>();
+        p.fbB = e->fbB.as<uint32_t>();
+        p.hchunk_out = (const uint4 *)g->hchunk_out;
+        p.hchunk_in = (const uint4 *)g->hchunk_in;
+        p.n_hchunk_out = g->n_hchunk_out;
+        p.n_hchunk_in = g->n_hchunk_in;
         e->tick_base += (uint32_t)(V + 2);
 
-        const int tw = e->tune.tile_words ? (int)e->tune.tile_words : 1;
+        int tw = 1;
+        while (tw < 64 && (uint64_t)tw < W) tw *= 2;
+        if (e->tune.tile_words) tw = (int)e->tune.tile_words;
+        const uint32_t tv = 64 / tw;
+        uint32_t nb = 64;
+        if (W > 64) {
+            nb = 1;
+            while (nb * 2 <= 64 && (size_t)nb * 2 * (W + 1) * 8 <= 32768) nb *= 2;
+        }
+        if (nb < tv) nb = tv;
+        p.pull_nb = nb;
+        const size_t lds = wide_pull_lds(nb, (uint32_t)W);
+        if (e->tune.trace_levels) {
+            e->trace_cap = e->tune.trace_levels;
+            if ((rc = ensure_zeroed(e->trace, (size_t)(e->trace_cap + 1) * 4 * 8, true, st))) return rc;
+            p.trace = e->trace.as<unsigned long long>();
+            p.trace_cap = e->trace_cap;
+        }
+        e->trace_valid = e->tune.trace_levels != 0;
         int occ = 0;
-        HIP_TRY(wide_max_blocks_per_cu(tw, kWideThreads, &occ));
+        HIP_TRY(wide_max_blocks_per_cu(tw, kWideThreads, lds, &occ));
         if (occ < 1) return fail(MSBFS_ERR_HIP, "wide kernel cannot be resident");
         int per_cu = e->tune.wide_blocks_per_cu ? (int)e->tune.wide_blocks_per_cu : 2;
         per_cu = std::min(per_cu, occ);
@@ -565,7 +644,7 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
         HIP_TRY(launch_zero_u32((uint32_t *)e->ctr.p, 4 * sizeof(WideCounters) / 4, st));
         HIP_TRY(launch_wide_seed(p, st));
         HIP_TRY(hipEventRecord(e->ev[1], st));
-        HIP_TRY(launch_wide(tw, p, blocks, kWideThreads, st));
+        HIP_TRY(launch_wide(tw, p, blocks, kWideThreads, lds, st));
         HIP_TRY(hipEventRecord(e->ev[2], st));
         e->last.grid_blocks = (uint32_t)blocks;
         e->last.workspace_bytes = e->seen.bytes * 4 + e->stamp.bytes * 4;
@@ -721,6 +800,37 @@ extern "C" int msbfs_engine_check(msbfs_engine *e) {
     if (!e) return fail(MSBFS_ERR_INVALID_ARG, "null engine");
     std::lock_guard<std::mutex> lk(e->mu);
     return finish_run(e);
+}
+
+extern "C" int msbfs_engine_level_trace(msbfs_engine *e, msbfs_level_record *out,
+                                        uint32_t max_levels, uint32_t *n_levels) {
+    if (!e || !n_levels) return fail(MSBFS_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(e->mu);
+    int rc = finish_run(e);
+    if (rc) return rc;
+    *n_levels = 0;
+    if (!e->trace_valid || e->last.family != MSBFS_FAMILY_WIDE) return MSBFS_OK;
+    const uint32_t L = std::min(e->last.levels, e->trace_cap);
+    std::vector<unsigned long long> raw((size_t)(e->trace_cap + 1) * 4);
+    HIP_TRY(hipMemcpy(raw.data(), e->trace.p, raw.size() * 8, hipMemcpyDeviceToHost));
+    int khz = 0;
+    HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, e->ctx->device));
+    const double us_per_tick = khz > 0 ? 1000.0 / khz : 0.0;
+    const unsigned long long t0 = raw[0];
+    const uint32_t n = std::min(L, max_levels);
+    for (uint32_t l = 0; l < n && out; ++l) {
+        msbfs_level_record &r = out[l];
+        r.level = l;
+        r.direction = (uint32_t)(raw[4 * l + 1] >> 32);
+        r.frontier = (uint32_t)raw[4 * l + 1];
+        r.reserved = 0;
+        r.new_bits = raw[4 * l + 3];
+        r.start_us = (double)(raw[4 * l] - t0) * us_per_tick;
+        r.mid_us = (double)(raw[4 * l + 2] - t0) * us_per_tick;
+        r.end_us = (double)(raw[4 * (l + 1)] - t0) * us_per_tick;
+    }
+    *n_levels = n;
+    return MSBFS_OK;
 }
 
 extern "C" int msbfs_engine_last_stats(msbfs_engine *e, msbfs_run_stats *st) {

@@ -81,6 +81,9 @@ __device__ __forceinline__ u64 wave_sum64(u64 x) {
     return ((u64)hi << 32) | lo;
 }
 
+// Distance stores are streamed (non-temporal): the output is written once
+// and never read back by the kernels, so it should not evict the graph and
+// frontier state from L2.
 __device__ __forceinline__ void store_dist(void *out, bool u16, uint64_t idx, uint32_t d,
                                            uint32_t *flags) {
     if (u16) {
@@ -88,17 +91,17 @@ __device__ __forceinline__ void store_dist(void *out, bool u16, uint64_t idx, ui
             atomicOr(&flags[FlagU16Overflow], 1u);
             d = 0xFFFEu;
         }
-        ((uint16_t *)out)[idx] = (uint16_t)d;
+        __builtin_nontemporal_store((uint16_t)d, (uint16_t *)out + idx);
     } else {
-        ((uint32_t *)out)[idx] = d;
+        __builtin_nontemporal_store(d, (uint32_t *)out + idx);
     }
 }
 
 __device__ __forceinline__ void store_sentinel(void *out, bool u16, uint64_t idx) {
     if (u16)
-        ((uint16_t *)out)[idx] = 0xFFFFu;
+        __builtin_nontemporal_store((uint16_t)0xFFFFu, (uint16_t *)out + idx);
     else
-        ((uint32_t *)out)[idx] = 0xFFFFFFFFu;
+        __builtin_nontemporal_store(0xFFFFFFFFu, (uint32_t *)out + idx);
 }
 
 __global__ void zero_u32_kernel(uint32_t *p, uint64_t n) {
@@ -420,6 +423,11 @@ hipError_t deep_max_blocks_per_cu(int bits, int ell, int threads, size_t lds, in
 }
 
 // ---------------------------------------------------------------- wide family
+//
+// State is vertex-major: the W source words of vertex v are the contiguous
+// run [v*W, v*W + W), so a parent's whole frontier (W*8 bytes, 256 B at
+// S = 2048) is one coalesced read by the TW lanes that share a vertex.
+// Wave tile: TV = 64/TW vertices x TW words (lane = wi*TV + vi).
 
 __device__ __forceinline__ u64 word_valid(uint64_t S, uint32_t w) {
     const uint64_t lo = (uint64_t)w * 64;
@@ -435,18 +443,27 @@ __device__ __forceinline__ u64 ld_dev_u64(const u64 *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Writes distance `d` for every bit of `n` (sources 64*w + b) at vertex v.
-// The loop runs over the union of the wave's bits so that, for one source,
-// the lanes holding consecutive vertices store together.
-__device__ __forceinline__ void wide_write_bits(const WideParams &p, bool ok, uint32_t w,
-                                                uint32_t v, u64 n, uint32_t d) {
-    u64 un = wave_or64(ok ? n : 0ull);
+// OR over the TV lanes that share a word (lanes wi*TV .. wi*TV + TV-1).
+template <uint32_t TV>
+__device__ __forceinline__ u64 group_or64(u64 x) {
+    uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+#pragma unroll
+    for (uint32_t off = 1; off < TV; off <<= 1) {
+        lo |= (uint32_t)__shfl_xor((int)lo, (int)off);
+        hi |= (uint32_t)__shfl_xor((int)hi, (int)off);
+    }
+    return ((u64)hi << 32) | lo;
+}
+
+// Per-lane distance stores for scattered vertices (push filter, heavy
+// vertices): one store per set bit.
+__device__ __forceinline__ void wide_write_lane(const WideParams &p, uint32_t w, uint32_t v,
+                                                u64 n, uint32_t d) {
     const bool u16 = p.out_u16 != 0;
-    while (un) {
-        const uint32_t b = (uint32_t)__builtin_ctzll(un);
-        un &= un - 1;
-        if (ok && ((n >> b) & 1ull))
-            store_dist(p.out, u16, ((uint64_t)w * 64 + b) * p.V + v, d, p.flags);
+    while (n) {
+        const uint32_t b = (uint32_t)__builtin_ctzll(n);
+        n &= n - 1;
+        store_dist(p.out, u16, ((uint64_t)w * 64 + b) * p.V + v, d, p.flags);
     }
 }
 
@@ -456,9 +473,10 @@ __global__ void wide_seed_kernel(WideParams p) {
     const uint32_t s = p.sources[i];
     const uint32_t w = (uint32_t)(i / 64);
     const u64 bit = 1ull << (i % 64);
-    const uint64_t idx = (uint64_t)w * p.V + s;
+    const uint64_t idx = (uint64_t)s * p.W + w;
     atomicOr(&p.seen[idx], bit);
     atomicOr(&p.frA[idx], bit);
+    atomicOr(&p.fbA[s >> 5], 1u << (s & 31));
     if (p.out) store_dist(p.out, p.out_u16 != 0, i * p.V + s, 0, p.flags);
     if (atomicExch(&p.stamp[s], p.tick_base) != p.tick_base) {
         const uint32_t pos = atomicAdd(&p.ctr[3].next_count, 1u);
@@ -474,21 +492,355 @@ hipError_t launch_wide_seed(const WideParams &p, hipStream_t st) {
 
 enum { DirPush = 0, DirPull = 1 };
 
+// Per-wave indices shared by the phases of the persistent kernel.
+struct WaveCtx {
+    uint32_t lane, vi, wi;
+    uint64_t gwave, nwaves;
+};
+
+__device__ __forceinline__ bool in_frontier(const uint32_t *fb, uint32_t u) {
+    return (fb[u >> 5] >> (u & 31)) & 1u;
+}
+__device__ __forceinline__ void mark_frontier(uint32_t *fb, uint32_t v) {
+    atomicOr(&fb[v >> 5], 1u << (v & 31));
+}
+
+// Push, part 1: OR each frontier vertex's words into its children's
+// candidate words (64-bit atomicOr), skipping children whose bits are all
+// seen; the first touch of a child appends it to the candidate queue.
+// Light vertices: TW lanes per vertex walk its edges together.  Heavy
+// vertices: their out-lists come in chunks of kHeavyChunk edges, one wave
+// per chunk, the vertex's active words one at a time, 64 edges per step.
+template <int TW>
+__device__ __forceinline__ void wide_push_expand(const WideParams &p, const WaveCtx &x, u64 *cur,
+                                                 const uint32_t *qcur, uint32_t n_front,
+                                                 const uint32_t *fb_cur, WideCounters *c,
+                                                 uint32_t tick) {
+    constexpr uint32_t TV = 64 / TW;
+    const uint32_t W = p.W, WG = (W + TW - 1) / TW;
+    const uint64_t ntiles = (uint64_t)((n_front + TV - 1) / TV) * WG;
+    for (uint64_t t = x.gwave; t < ntiles; t += x.nwaves) {
+        const uint32_t i = (uint32_t)(t / WG) * TV + x.vi;
+        const uint32_t w = (uint32_t)(t % WG) * TW + x.wi;
+        uint32_t u = 0, e = 0, eend = 0;
+        u64 f = 0;
+        if (i < n_front && w < W) {
+            u = qcur[i];
+            const uint32_t e0 = p.ro[u], e1 = p.ro[u + 1];
+            if (e1 - e0 <= p.heavy_t) {
+                const uint64_t idx = (uint64_t)u * W + w;
+                f = cur[idx];
+                if (f) {
+                    cur[idx] = 0;
+                    e = e0;
+                    eend = e1;
+                }
+            }
+        }
+        while (__any(e < eend)) {
+            bool app = false;
+            uint32_t v = 0;
+            if (e < eend) {
+                v = p.ci[e];
+                const uint64_t idx = (uint64_t)v * W + w;
+                const u64 m = f & ~p.seen[idx];
+                // Only the first toucher of (v, w) can be the first toucher
+                // of v, so only it consults the level stamp.
+                if (m && atomicOr(&p.nx[idx], m) == 0 &&
+                    atomicExch(&p.stamp[v], tick) != tick)
+                    app = true;
+                ++e;
+            }
+            const uint32_t pos = wave_reserve(&c->cand_count, app);
+            if (app) p.qc[pos] = v;
+        }
+    }
+    for (uint64_t t = x.gwave; t < p.n_hchunk_out; t += x.nwaves) {
+        const uint4 ch = p.hchunk_out[t];
+        const uint32_t h = ch.x, e0 = ch.y, e1 = ch.z;
+        if (!in_frontier(fb_cur, h)) continue;  // wave-uniform
+        for (uint32_t w0 = 0; w0 < W; w0 += 64) {
+            const uint32_t wl = w0 + x.lane;
+            const u64 fl = wl < W ? cur[(uint64_t)h * W + wl] : 0ull;
+            u64 act = __ballot(fl != 0);
+            while (act) {
+                const uint32_t k = (uint32_t)__builtin_ctzll(act);
+                act &= act - 1;
+                const uint32_t w = w0 + k;
+                const u64 f = __shfl(fl, (int)k);
+                for (uint32_t base = e0; base < e1; base += 64) {
+                    const uint32_t e = base + x.lane;
+                    bool app = false;
+                    uint32_t v = 0;
+                    if (e < e1) {
+                        v = p.ci[e];
+                        const uint64_t idx = (uint64_t)v * W + w;
+                        const u64 m = f & ~p.seen[idx];
+                        if (m && atomicOr(&p.nx[idx], m) == 0 &&
+                            atomicExch(&p.stamp[v], tick) != tick)
+                            app = true;
+                    }
+                    const uint32_t pos = wave_reserve(&c->cand_count, app);
+                    if (app) p.qc[pos] = v;
+                }
+            }
+        }
+    }
+}
+
+// Push, part 2: new bits of each candidate become its next-frontier words
+// and distances; the candidate joins the next frontier queue.  The consumed
+// frontier (heavy words left by part 1, bitmap words) is cleared here.
+template <int TW>
+__device__ __forceinline__ void wide_push_filter(const WideParams &p, const WaveCtx &x, u64 *cur,
+                                                 const uint32_t *qcur, uint32_t n_front,
+                                                 uint32_t *fb_cur, u64 *nxt, uint32_t *qnext,
+                                                 uint32_t *fb_next, WideCounters *c, uint32_t nd,
+                                                 uint64_t gthread, uint64_t nthreads,
+                                                 u64 &my_bits, u64 &my_deg) {
+    constexpr uint32_t TV = 64 / TW;
+    const uint32_t W = p.W, WG = (W + TW - 1) / TW;
+    const uint32_t ncand = ld_dev_u32(&c->cand_count);
+    const uint64_t ftiles = (ncand + TV - 1) / TV;
+    for (uint64_t t = x.gwave; t < ftiles; t += x.nwaves) {
+        const uint32_t i = (uint32_t)t * TV + x.vi;
+        const bool okv = i < ncand;
+        const uint32_t v = okv ? p.qc[i] : 0;
+        bool any = false;
+        for (uint32_t g = 0; g < WG; ++g) {
+            const uint32_t w = g * TW + x.wi;
+            u64 n = 0;
+            if (okv && w < W) {
+                const uint64_t idx = (uint64_t)v * W + w;
+                const u64 xw = p.nx[idx];
+                if (xw) {
+                    p.nx[idx] = 0;
+                    const u64 s = p.seen[idx];
+                    n = xw & ~s;
+                    if (n) {
+                        p.seen[idx] = s | n;
+                        nxt[idx] = n;
+                        if (p.out) wide_write_lane(p, w, v, n, nd);
+                    }
+                }
+            }
+            any |= n != 0;
+            my_bits += __popcll(n);
+        }
+        const u64 bal = __ballot(any);
+        bool vany = false;
+#pragma unroll
+        for (uint32_t k = 0; k < TW; ++k) vany |= ((bal >> (k * TV + x.vi)) & 1ull) != 0;
+        const bool app = x.wi == 0 && okv && vany;
+        const uint32_t pos = wave_reserve(&c->next_count, app);
+        if (app) {
+            qnext[pos] = v;
+            mark_frontier(fb_next, v);
+            my_deg += p.ro[v + 1] - p.ro[v];
+        }
+    }
+    for (uint64_t k = gthread; k < n_front; k += nthreads) {
+        const uint32_t u = qcur[k];
+        fb_cur[u >> 5] = 0;
+        if (p.ro[u + 1] - p.ro[u] > p.heavy_t)
+            for (uint32_t w = 0; w < W; ++w) cur[(uint64_t)u * W + w] = 0;
+    }
+}
+
+// Pull: each workgroup takes blocks of NB consecutive vertices.  Every
+// (vertex, word) with unseen bits ORs the frontier words of its parents
+// (only those set in the frontier bitmap are read) until all still-needed
+// bits are found (early exit); the new bits go to LDS, and the block's
+// distances are then stored one source row at a time with lane = vertex,
+// so each store covers NB consecutive entries of a row.  Heavy in-lists
+// come in chunks, one wave per chunk; a chunk's bits are ORed into nx and
+// resolved by wide_pull_resolve after the grid barrier.
+template <int TW>
+__device__ __forceinline__ void wide_pull(const WideParams &p, const WaveCtx &x, const u64 *cur,
+                                          const uint32_t *fb_cur, u64 *nxt, uint32_t *qnext,
+                                          uint32_t *fb_next, WideCounters *c, uint32_t nd,
+                                          u64 *lds_n, u64 &my_bits, u64 &my_deg) {
+    constexpr uint32_t TV = 64 / TW;
+    const uint32_t V = p.V, W = p.W, WG = (W + TW - 1) / TW;
+    const uint32_t NB = p.pull_nb, LW = W + 1;  // LDS row stride (padded)
+    const uint32_t wib = threadIdx.x / 64, nwb = blockDim.x / 64;
+    const uint64_t nblk = (V + NB - 1) / NB;
+    const bool u16 = p.out_u16 != 0;
+    for (uint64_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+        const uint32_t v0 = (uint32_t)blk * NB;
+        for (uint32_t st = wib; st < NB / TV; st += nwb) {
+            const uint32_t vl = st * TV + x.vi;
+            const uint32_t v = v0 + vl;
+            bool okv = v < V;
+            uint32_t e0 = 0, e1 = 0;
+            if (okv) {
+                e0 = p.co[v];
+                e1 = p.co[v + 1];
+                if (e1 - e0 > p.heavy_t) okv = false;  // chunked below
+            }
+            bool any = false;
+            for (uint32_t g = 0; g < WG; ++g) {
+                const uint32_t w = g * TW + x.wi;
+                u64 n = 0;
+                if (okv && w < W) {
+                    const uint64_t idx = (uint64_t)v * W + w;
+                    const u64 s = p.seen[idx];
+                    const u64 need = ~s & word_valid(p.S, w);
+                    if (need) {
+                        u64 acc = 0;
+                        for (uint32_t e = e0; e < e1; e += 4) {
+                            const uint32_t u0 = p.ri[e];
+                            const uint32_t u1 = e + 1 < e1 ? p.ri[e + 1] : u0;
+                            const uint32_t u2 = e + 2 < e1 ? p.ri[e + 2] : u0;
+                            const uint32_t u3 = e + 3 < e1 ? p.ri[e + 3] : u0;
+                            if (in_frontier(fb_cur, u0)) acc |= cur[(uint64_t)u0 * W + w];
+                            if (in_frontier(fb_cur, u1)) acc |= cur[(uint64_t)u1 * W + w];
+                            if (in_frontier(fb_cur, u2)) acc |= cur[(uint64_t)u2 * W + w];
+                            if (in_frontier(fb_cur, u3)) acc |= cur[(uint64_t)u3 * W + w];
+                            if ((acc & need) == need) break;
+                        }
+                        n = acc & need;
+                        if (n) {
+                            p.seen[idx] = s | n;
+                            nxt[idx] = n;
+                        }
+                    }
+                }
+                if (w < W) lds_n[vl * LW + w] = n;
+                any |= n != 0;
+                my_bits += __popcll(n);
+            }
+            const u64 bal = __ballot(any);
+            bool vany = false;
+#pragma unroll
+            for (uint32_t k = 0; k < TW; ++k) vany |= ((bal >> (k * TV + x.vi)) & 1ull) != 0;
+            const bool app = x.wi == 0 && okv && vany;
+            const uint32_t pos = wave_reserve(&c->next_count, app);
+            if (app) {
+                qnext[pos] = v;
+                mark_frontier(fb_next, v);
+                my_deg += p.ro[v + 1] - p.ro[v];
+            }
+        }
+        __syncthreads();
+        if (p.out) {
+            for (uint32_t w = wib; w < W; w += nwb) {
+                const bool okl = x.lane < NB && v0 + x.lane < V;
+                const u64 n = okl ? lds_n[x.lane * LW + w] : 0ull;
+                u64 un = wave_or64(n);
+                while (un) {
+                    const uint32_t b = (uint32_t)__builtin_ctzll(un);
+                    un &= un - 1;
+                    if ((n >> b) & 1ull)
+                        store_dist(p.out, u16, ((uint64_t)w * 64 + b) * V + v0 + x.lane, nd,
+                                   p.flags);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // Heavy in-lists: one wave per (chunk, word group); lanes vi take every
+    // TV-th in-edge, U parents per lane per step; a chunk stops once its
+    // words' needed bits (less those other chunks already found) are in.
+    const uint64_t nh = (uint64_t)p.n_hchunk_in * WG;
+    constexpr uint32_t U = TV >= 16 ? 1 : 16 / TV;
+    for (uint64_t t = x.gwave; t < nh; t += x.nwaves) {
+        const uint4 ch = p.hchunk_in[t / WG];
+        const uint32_t h = ch.x, e0 = ch.y, e1 = ch.z;
+        const uint32_t w = (uint32_t)(t % WG) * TW + x.wi;
+        const uint64_t idx = (uint64_t)h * W + w;
+        u64 need = 0;
+        if (w < W) need = ~p.seen[idx] & word_valid(p.S, w) & ~p.nx[idx];
+        if (!__any(need != 0)) continue;  // wave-uniform
+        u64 acc = 0;
+        for (uint32_t base = e0; base < e1; base += TV * U) {
+            if (need) {
+#pragma unroll
+                for (uint32_t k = 0; k < U; ++k) {
+                    const uint32_t e = base + k * TV + x.vi;
+                    if (e < e1) {
+                        const uint32_t u = p.ri[e];
+                        if (in_frontier(fb_cur, u)) acc |= cur[(uint64_t)u * W + w];
+                    }
+                }
+            }
+            const u64 g = group_or64<TV>(acc);
+            if (__all((g & need) == need)) break;
+        }
+        const u64 g = group_or64<TV>(acc) & need;
+        if (x.vi == 0 && g) atomicOr(&p.nx[idx], g);
+    }
+}
+
+// After a pull level's barrier: the old frontier's words and bitmap are
+// cleared, and each heavy vertex's chunk results (ORed into nx) become its
+// new bits, distances and next-frontier membership.
+template <int TW>
+__device__ __forceinline__ void wide_pull_resolve(const WideParams &p, const WaveCtx &x, u64 *cur,
+                                                  const uint32_t *qcur, uint32_t n_front,
+                                                  uint32_t *fb_cur, u64 *nxt, uint32_t *qnext,
+                                                  uint32_t *fb_next, WideCounters *c,
+                                                  uint32_t nd, uint32_t tick, uint64_t gthread,
+                                                  uint64_t nthreads, u64 &my_bits,
+                                                  u64 &my_deg) {
+    const uint32_t W = p.W;
+    const uint64_t nclear = (uint64_t)n_front * W;
+    for (uint64_t k = gthread; k < nclear; k += nthreads) {
+        const uint32_t u = qcur[k / W];
+        cur[(uint64_t)u * W + (k % W)] = 0;
+        if (k % W == 0) fb_cur[u >> 5] = 0;
+    }
+    const uint64_t nhw = (uint64_t)p.n_heavy_in * W;
+    const uint64_t tiles = (nhw + 63) / 64;
+    for (uint64_t t = x.gwave; t < tiles; t += x.nwaves) {
+        const uint64_t k = t * 64 + x.lane;
+        bool app = false;
+        uint32_t h = 0;
+        if (k < nhw) {
+            h = p.heavy_in[k / W];
+            const uint32_t w = (uint32_t)(k % W);
+            const uint64_t idx = (uint64_t)h * W + w;
+            const u64 xw = p.nx[idx];
+            if (xw) {
+                p.nx[idx] = 0;
+                const u64 s = p.seen[idx];
+                const u64 n = xw & ~s & word_valid(p.S, w);
+                if (n) {
+                    p.seen[idx] = s | n;
+                    nxt[idx] = n;
+                    my_bits += __popcll(n);
+                    if (p.out) wide_write_lane(p, w, h, n, nd);
+                    app = atomicExch(&p.stamp[h], tick) != tick;
+                }
+            }
+        }
+        const uint32_t pos = wave_reserve(&c->next_count, app);
+        if (app) {
+            qnext[pos] = h;
+            mark_frontier(fb_next, h);
+            my_deg += p.ro[h + 1] - p.ro[h];
+        }
+    }
+}
+
 template <int TW>
 __global__ void __launch_bounds__(kWideThreads) wide_bfs_kernel(WideParams p) {
     cg::grid_group grid = cg::this_grid();
+    extern __shared__ u64 lds_n[];
     constexpr uint32_t TV = 64 / TW;
-    const uint32_t lane = lane_id();
-    const uint32_t vi = lane % TV, wi = lane / TV;
-    const uint64_t gwave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / 64;
-    const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / 64;
+    WaveCtx x;
+    x.lane = lane_id();
+    x.vi = x.lane % TV;
+    x.wi = x.lane / TV;
+    x.gwave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / 64;
+    x.nwaves = (uint64_t)gridDim.x * blockDim.x / 64;
     const uint64_t gthread = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     const uint64_t nthreads = (uint64_t)gridDim.x * blockDim.x;
     const uint32_t V = p.V, W = p.W;
-    const uint32_t WG = (W + TW - 1) / TW;
     const bool first = blockIdx.x == 0 && threadIdx.x == 0;
 
     u64 *cur = p.frA, *nxt = p.frB;
+    uint32_t *fb_cur = p.fbA, *fb_next = p.fbB;
     uint32_t *qcur = p.qf0, *qnext = p.qf1;
     uint32_t n_front = ld_dev_u32(&p.ctr[3].next_count);
     u64 prev_bits = p.S, prev_deg = 0;
@@ -503,10 +855,12 @@ __global__ void __launch_bounds__(kWideThreads) wide_bfs_kernel(WideParams p) {
             // Direction controller (crates/gpu_bfs/src/dispatch/push_pull.rs:483-498):
             // Push->Pull when m_f > |E|/alpha, Pull->Push when n_f < |V|/beta.
             u64 n_f, m_f;
-            if (p.rule == 0) {
+            if (p.rule == 1) {
+                // the reference's estimate from the new-bit count
                 n_f = (prev_bits + p.S - 1) / p.S;
                 m_f = V ? (n_f * p.E) / V : 0;
             } else {
+                // measured: union frontier size and its out-degree sum
                 n_f = n_front;
                 m_f = prev_deg;
             }
@@ -518,151 +872,31 @@ __global__ void __launch_bounds__(kWideThreads) wide_bfs_kernel(WideParams p) {
         }
         const uint32_t nd = L + 1;
         const uint32_t tick = p.tick_base + L + 1;
-        uint64_t my_bits = 0, my_deg = 0;
+        u64 my_bits = 0, my_deg = 0;
+        if (p.trace && first && L < p.trace_cap) {
+            p.trace[4 * L] = wall_clock64();
+            p.trace[4 * L + 1] = ((u64)dir << 32) | n_front;
+        }
 
         if (dir == DirPush) {
             ++push_levels;
-            // Expand: each wave takes TV frontier vertices x TW words.
-            const uint64_t ntiles = (uint64_t)((n_front + TV - 1) / TV) * WG;
-            for (uint64_t t = gwave; t < ntiles; t += nwaves) {
-                const uint32_t i = (uint32_t)(t / WG) * TV + vi;
-                const uint32_t w = (uint32_t)(t % WG) * TW + wi;
-                const bool ok = i < n_front && w < W;
-                uint32_t u = 0, e = 0, eend = 0;
-                u64 f = 0;
-                if (ok) {
-                    u = qcur[i];
-                    const uint64_t idx = (uint64_t)w * V + u;
-                    f = cur[idx];
-                    if (f) {
-                        cur[idx] = 0;
-                        e = p.ro[u];
-                        eend = p.ro[u + 1];
-                    }
-                }
-                while (__any(e < eend)) {
-                    bool app = false;
-                    uint32_t v = 0;
-                    if (e < eend) {
-                        v = p.ci[e];
-                        const uint64_t idx = (uint64_t)w * V + v;
-                        const u64 m = f & ~p.seen[idx];
-                        if (m) {
-                            atomicOr(&p.nx[idx], m);
-                            if (atomicExch(&p.stamp[v], tick) != tick) app = true;
-                        }
-                        ++e;
-                    }
-                    const uint32_t pos = wave_reserve(&c->cand_count, app);
-                    if (app) p.qc[pos] = v;
-                }
-            }
+            wide_push_expand<TW>(p, x, cur, qcur, n_front, fb_cur, c, tick);
             grid.sync();
-            // Filter: new bits of each candidate, next frontier, distances.
-            const uint32_t ncand = ld_dev_u32(&c->cand_count);
-            const uint64_t ftiles = (ncand + TV - 1) / TV;
-            for (uint64_t t = gwave; t < ftiles; t += nwaves) {
-                const uint32_t i = (uint32_t)t * TV + vi;
-                const bool okv = i < ncand;
-                const uint32_t v = okv ? p.qc[i] : 0;
-                bool any = false;
-                for (uint32_t g = 0; g < WG; ++g) {
-                    const uint32_t w = g * TW + wi;
-                    const bool ok = okv && w < W;
-                    u64 n = 0;
-                    if (ok) {
-                        const uint64_t idx = (uint64_t)w * V + v;
-                        const u64 x = p.nx[idx];
-                        if (x) {
-                            p.nx[idx] = 0;
-                            const u64 s = p.seen[idx];
-                            n = x & ~s;
-                            if (n) {
-                                p.seen[idx] = s | n;
-                                nxt[idx] = n;
-                            }
-                        }
-                    }
-                    if (p.out) wide_write_bits(p, ok, w, v, n, nd);
-                    any |= n != 0;
-                    my_bits += __popcll(n);
-                }
-                const u64 bal = __ballot(any);
-                bool vany = false;
-#pragma unroll
-                for (uint32_t k = 0; k < TW; ++k) vany |= ((bal >> (k * TV + vi)) & 1ull) != 0;
-                const bool app = wi == 0 && okv && vany;
-                const uint32_t pos = wave_reserve(&c->next_count, app);
-                if (app) {
-                    qnext[pos] = v;
-                    my_deg += p.ro[v + 1] - p.ro[v];
-                }
-            }
+            if (p.trace && first && L < p.trace_cap) p.trace[4 * L + 2] = wall_clock64();
+            wide_push_filter<TW>(p, x, cur, qcur, n_front, fb_cur, nxt, qnext, fb_next, c, nd,
+                                 gthread, nthreads, my_bits, my_deg);
         } else {
             ++pull_levels;
-            // Pull: each wave takes TV consecutive vertices x TW words and
-            // gathers parent frontier words until every needed bit is found.
-            const uint64_t ntiles = (V + TV - 1) / TV;
-            for (uint64_t t = gwave; t < ntiles; t += nwaves) {
-                const uint32_t v = (uint32_t)t * TV + vi;
-                const bool okv = v < V;
-                uint32_t e0 = 0, e1 = 0;
-                if (okv) {
-                    e0 = p.co[v];
-                    e1 = p.co[v + 1];
-                }
-                bool any = false;
-                for (uint32_t g = 0; g < WG; ++g) {
-                    const uint32_t w = g * TW + wi;
-                    const bool ok = okv && w < W;
-                    u64 n = 0;
-                    if (ok) {
-                        const uint64_t idx = (uint64_t)w * V + v;
-                        const u64 s = p.seen[idx];
-                        const u64 need = ~s & word_valid(p.S, w);
-                        if (need) {
-                            u64 acc = 0;
-                            const u64 *cw = cur + (uint64_t)w * V;
-                            for (uint32_t e = e0; e < e1; ++e) {
-                                acc |= cw[p.ri[e]];
-                                if ((acc & need) == need) break;
-                            }
-                            n = acc & need;
-                            if (n) {
-                                p.seen[idx] = s | n;
-                                nxt[idx] = n;
-                            }
-                        }
-                    }
-                    if (p.out) wide_write_bits(p, ok, w, v, n, nd);
-                    any |= n != 0;
-                    my_bits += __popcll(n);
-                }
-                const u64 bal = __ballot(any);
-                bool vany = false;
-#pragma unroll
-                for (uint32_t k = 0; k < TW; ++k) vany |= ((bal >> (k * TV + vi)) & 1ull) != 0;
-                const bool app = wi == 0 && okv && vany;
-                const uint32_t pos = wave_reserve(&c->next_count, app);
-                if (app) {
-                    qnext[pos] = v;
-                    my_deg += p.ro[v + 1] - p.ro[v];
-                }
-            }
+            wide_pull<TW>(p, x, cur, fb_cur, nxt, qnext, fb_next, c, nd, lds_n, my_bits, my_deg);
             grid.sync();
-            // The consumed frontier words are cleared before this array is
-            // written again as the frontier after next.
-            const uint64_t nclear = (uint64_t)n_front * W;
-            for (uint64_t k = gthread; k < nclear; k += nthreads) {
-                const uint32_t w = (uint32_t)(k / n_front);
-                const uint32_t i = (uint32_t)(k % n_front);
-                cur[(uint64_t)w * V + qcur[i]] = 0;
-            }
+            if (p.trace && first && L < p.trace_cap) p.trace[4 * L + 2] = wall_clock64();
+            wide_pull_resolve<TW>(p, x, cur, qcur, n_front, fb_cur, nxt, qnext, fb_next, c, nd,
+                                  tick, gthread, nthreads, my_bits, my_deg);
         }
         // Level counters for the direction controller: one atomic per wave.
         {
             const u64 sb = wave_sum64(my_bits), sd = wave_sum64(my_deg);
-            if (lane == 0) {
+            if (x.lane == 0) {
                 if (sb) atomicAdd(&c->active_bits, sb);
                 if (sd) atomicAdd(&c->deg_sum, sd);
             }
@@ -678,9 +912,13 @@ __global__ void __launch_bounds__(kWideThreads) wide_bfs_kernel(WideParams p) {
         n_front = ld_dev_u32(&c->next_count);
         prev_bits = ld_dev_u64(&c->active_bits);
         prev_deg = ld_dev_u64(&c->deg_sum);
+        if (p.trace && first && L < p.trace_cap) p.trace[4 * L + 3] = prev_bits;
         u64 *tf = cur;
         cur = nxt;
         nxt = tf;
+        uint32_t *tb = fb_cur;
+        fb_cur = fb_next;
+        fb_next = tb;
         uint32_t *tq = qcur;
         qcur = qnext;
         qnext = tq;
@@ -690,70 +928,79 @@ __global__ void __launch_bounds__(kWideThreads) wide_bfs_kernel(WideParams p) {
         p.flags[FlagLevels] = L;
         p.flags[FlagPushLevels] = push_levels;
         p.flags[FlagPullLevels] = pull_levels;
+        if (p.trace && L < p.trace_cap) p.trace[4 * L] = wall_clock64();
     }
 
-    // Final sweep: sentinels for unreached pairs, reachability words, and
-    // the visited words cleared so the workspace starts clean next time.
-    const uint64_t ntiles = (uint64_t)((V + TV - 1) / TV) * WG;
-    for (uint64_t t = gwave; t < ntiles; t += nwaves) {
-        const uint32_t v = (uint32_t)(t / WG) * TV + vi;
-        const uint32_t w = (uint32_t)(t % WG) * TW + wi;
-        const bool ok = v < V && w < W;
-        u64 s = 0, miss = 0;
-        if (ok) {
-            const uint64_t idx = (uint64_t)w * V + v;
-            s = p.seen[idx];
-            if (s) p.seen[idx] = 0;
-            miss = ~s & word_valid(p.S, w);
-        }
-        if (p.out) {
-            u64 un = wave_or64(miss);
-            const bool u16 = p.out_u16 != 0;
-            while (un) {
-                const uint32_t b = (uint32_t)__builtin_ctzll(un);
-                un &= un - 1;
-                if ((miss >> b) & 1ull) store_sentinel(p.out, u16, ((uint64_t)w * 64 + b) * V + v);
+    // Final sweep (lane = vertex, so sentinel stores of one source row
+    // coalesce): sentinels for unreached pairs, reachability words, and the
+    // visited words cleared so the workspace starts clean next time.
+    const uint64_t nvt = (V + 63) / 64;
+    for (uint64_t t = x.gwave; t < nvt; t += x.nwaves) {
+        const uint32_t v = (uint32_t)t * 64 + x.lane;
+        const bool ok = v < V;
+        for (uint32_t w = 0; w < W; ++w) {
+            u64 s = 0, miss = 0;
+            if (ok) {
+                const uint64_t idx = (uint64_t)v * W + w;
+                s = p.seen[idx];
+                if (s) p.seen[idx] = 0;
+                miss = ~s & word_valid(p.S, w);
+            }
+            if (p.out) {
+                u64 un = wave_or64(miss);
+                const bool u16 = p.out_u16 != 0;
+                while (un) {
+                    const uint32_t b = (uint32_t)__builtin_ctzll(un);
+                    un &= un - 1;
+                    if ((miss >> b) & 1ull)
+                        store_sentinel(p.out, u16, ((uint64_t)w * 64 + b) * V + v);
+                }
+            }
+            if (p.reach && ok) {
+                const u64 r = s & word_valid(p.S, w);
+                const uint64_t base = (uint64_t)v * p.reach_wpv + 2 * w;
+                p.reach[base] = (uint32_t)r;
+                if (2 * w + 1 < p.reach_wpv) p.reach[base + 1] = (uint32_t)(r >> 32);
             }
         }
-        if (p.reach && ok) {
-            const u64 r = s & word_valid(p.S, w);
-            const uint64_t base = (uint64_t)v * p.reach_wpv + 2 * w;
-            p.reach[base] = (uint32_t)r;
-            if (2 * w + 1 < p.reach_wpv) p.reach[base + 1] = (uint32_t)(r >> 32);
-        }
     }
 }
 
 template <int TW>
-static hipError_t wide_launch_t(const WideParams &p, int blocks, int threads, hipStream_t st) {
+static hipError_t wide_launch_t(const WideParams &p, int blocks, int threads, size_t lds,
+                                hipStream_t st) {
     void *args[] = {(void *)&p};
     return hipLaunchCooperativeKernel((const void *)wide_bfs_kernel<TW>, dim3(blocks),
-                                      dim3(threads), args, 0, st);
+                                      dim3(threads), args, lds, st);
 }
 
 template <int TW>
-static hipError_t wide_occ_t(int threads, int *out) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, wide_bfs_kernel<TW>, threads, 0);
+static hipError_t wide_occ_t(int threads, size_t lds, int *out) {
+    hipError_t e = hipFuncSetAttribute((const void *)wide_bfs_kernel<TW>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, wide_bfs_kernel<TW>, threads, lds);
 }
 
-hipError_t launch_wide(int tw, const WideParams &p, int blocks, int threads, hipStream_t st) {
-    switch (tw) {
-    case 1: return wide_launch_t<1>(p, blocks, threads, st);
-    case 2: return wide_launch_t<2>(p, blocks, threads, st);
-    case 4: return wide_launch_t<4>(p, blocks, threads, st);
-    case 8: return wide_launch_t<8>(p, blocks, threads, st);
-    default: return hipErrorInvalidValue;
+#define WIDE_SWITCH(FN, ...)                  \
+    switch (tw) {                             \
+    case 1: return FN<1>(__VA_ARGS__);        \
+    case 2: return FN<2>(__VA_ARGS__);        \
+    case 4: return FN<4>(__VA_ARGS__);        \
+    case 8: return FN<8>(__VA_ARGS__);        \
+    case 16: return FN<16>(__VA_ARGS__);      \
+    case 32: return FN<32>(__VA_ARGS__);      \
+    case 64: return FN<64>(__VA_ARGS__);      \
+    default: return hipErrorInvalidValue;     \
     }
+
+hipError_t launch_wide(int tw, const WideParams &p, int blocks, int threads, size_t lds,
+                       hipStream_t st) {
+    WIDE_SWITCH(wide_launch_t, p, blocks, threads, lds, st)
 }
 
-hipError_t wide_max_blocks_per_cu(int tw, int threads, int *out) {
-    switch (tw) {
-    case 1: return wide_occ_t<1>(threads, out);
-    case 2: return wide_occ_t<2>(threads, out);
-    case 4: return wide_occ_t<4>(threads, out);
-    case 8: return wide_occ_t<8>(threads, out);
-    default: return hipErrorInvalidValue;
-    }
+hipError_t wide_max_blocks_per_cu(int tw, int threads, size_t lds, int *out) {
+    WIDE_SWITCH(wide_occ_t, threads, lds, out)
 }
 
 }  // namespace msbfs

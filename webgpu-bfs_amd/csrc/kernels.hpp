@@ -59,7 +59,7 @@ struct WideParams {
     uint64_t S, E;
     const uint32_t *ro, *ci;  // CSR (push)
     const uint32_t *co, *ri;  // CSC (pull); alias CSR when symmetric
-    unsigned long long *seen, *frA, *frB, *nx;  // word-major [W][V]
+    unsigned long long *seen, *frA, *frB, *nx;  // vertex-major [V][W]
     uint32_t *qf0, *qf1, *qc, *stamp;           // [V]
     const uint32_t *sources;
     WideCounters *ctr;      // ring of 4
@@ -71,9 +71,25 @@ struct WideParams {
     uint32_t out_u16;
     uint32_t policy;        // msbfs_direction_policy
     uint32_t alpha, beta;
-    uint32_t rule;          // 0 reference estimate, 1 exact degree sum
-    uint32_t pad;
+    uint32_t rule;          // 0 measured frontier size / degree sum, 1 reference estimate
+    uint32_t heavy_t;       // degree above which a vertex is handled by a whole wave
+    const uint32_t *heavy_out;  // vertices with out-degree > heavy_t (CSR)
+    const uint32_t *heavy_in;   // vertices with in-degree > heavy_t (CSC)
+    uint32_t n_heavy_out, n_heavy_in;
+    uint32_t *fbA, *fbB;    // frontier bitmaps (1 bit per vertex), ping-pong with frA/frB
+    const uint4 *hchunk_out;  // (vertex, first edge, end edge, 0): heavy out-lists in chunks
+    const uint4 *hchunk_in;   // same for heavy in-lists (CSC)
+    uint32_t n_hchunk_out, n_hchunk_in;
+    uint32_t pull_nb;       // vertices per workgroup block in pull levels
+    uint32_t trace_cap;     // levels the trace buffer holds
+    unsigned long long *trace;  // per level: start clock, dir<<32|frontier, mid clock, new bits
 };
+
+// Degree above which the wide family gives a vertex's edge list to a whole
+// wave instead of one lane.
+constexpr uint32_t kHeavyDegree = 128;
+// Edges per work item when a heavy vertex's list is split across waves.
+constexpr uint32_t kHeavyChunk = 2048;
 
 // flags[] layout (device, u32)
 enum : int {
@@ -90,13 +106,16 @@ hipError_t launch_deep(int bits, int ell, const DeepParams &p, int blocks, int t
 hipError_t deep_max_blocks_per_cu(int bits, int ell, int threads, size_t lds_bytes, int *out);
 hipError_t deep_prepare(int bits, int ell, size_t lds_bytes);
 
-hipError_t launch_wide(int tile_words, const WideParams &p, int blocks, int threads,
+hipError_t launch_wide(int tile_words, const WideParams &p, int blocks, int threads, size_t lds,
                        hipStream_t st);
-hipError_t wide_max_blocks_per_cu(int tile_words, int threads, int *out);
+hipError_t wide_max_blocks_per_cu(int tile_words, int threads, size_t lds, int *out);
 hipError_t launch_wide_seed(const WideParams &p, hipStream_t st);
 
 hipError_t launch_zero_u32(uint32_t *ptr, uint64_t n, hipStream_t st);
 
 constexpr int kWideThreads = 512;
+
+// LDS bytes of the pull block buffer: nb vertices x (W + 1) words.
+inline size_t wide_pull_lds(uint32_t nb, uint32_t W) { return (size_t)nb * (W + 1) * 8; }
 
 }  // namespace msbfs

@@ -61,7 +61,14 @@ class GraphInfo(ctypes.Structure):
 class Tuning(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint32) for n in (
         "alpha", "beta", "tile_words", "words_per_lane", "deep_bits", "deep_threads",
-        "wide_blocks_per_cu", "direction_rule")]
+        "wide_blocks_per_cu", "direction_rule", "trace_levels")]
+
+
+class LevelRecord(ctypes.Structure):
+    _fields_ = [("level", ctypes.c_uint32), ("direction", ctypes.c_uint32),
+                ("frontier", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("new_bits", ctypes.c_uint64), ("start_us", ctypes.c_double),
+                ("mid_us", ctypes.c_double), ("end_us", ctypes.c_double)]
 
 
 class RunStats(ctypes.Structure):
@@ -101,6 +108,7 @@ SIGNATURES = {
     "msbfs_engine_set_tuning": (I32, [P, ctypes.POINTER(Tuning)]),
     "msbfs_engine_get_tuning": (I32, [P, ctypes.POINTER(Tuning)]),
     "msbfs_engine_last_stats": (I32, [P, ctypes.POINTER(RunStats)]),
+    "msbfs_engine_level_trace": (I32, [P, P, U32, ctypes.POINTER(U32)]),
     "msbfs_select_auto_method": (I32, [U64, U64, U32, U64]),
     "msbfs_engine_resolve": (I32, [P, P, U64, ctypes.POINTER(I32)]),
     "msbfs_run_ms_bfs": (I32, [P, P, P, U64, I32, P, U64]),
@@ -125,6 +133,19 @@ SIGNATURES = {
 _lib = None
 
 
+def _bind_one_hip_runtime() -> None:
+    """PyTorch-ROCm wheels ship their own libamdhip64.so.7, with the same
+    SONAME as the system ROCm's.  The dynamic loader keeps whichever copy is
+    loaded first for the whole process, and torch fails to initialise on the
+    other copy; device pointers and hipStream_t handles are also only valid
+    within one runtime.  Loading torch first (when it is installed) makes
+    libmsbfs bind to torch's runtime, so tensors and streams can be shared."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def lib():
     """Returns the loaded libmsbfs, raising if it has not been built."""
     global _lib
@@ -133,6 +154,7 @@ def lib():
             raise ImportError(
                 f"libmsbfs.so not found at {LIB_PATH}; build it with "
                 "`make -C webgpu-bfs_amd/csrc` (hipcc, gfx950)")
+        _bind_one_hip_runtime()
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

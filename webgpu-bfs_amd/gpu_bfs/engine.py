@@ -217,6 +217,16 @@ class GpuBfsEngine:
         L.check(L.lib().msbfs_engine_last_stats(self._h, ctypes.byref(st)))
         return st.as_dict()
 
+    def level_trace(self) -> list:
+        """Per-level records of the last wide-family run (needs
+        set_tuning(trace_levels=N) before the run)."""
+        cap = max(1, self.get_tuning()["trace_levels"])
+        recs = (L.LevelRecord * cap)()
+        n = ctypes.c_uint32()
+        L.check(L.lib().msbfs_engine_level_trace(self._h, recs, cap, ctypes.byref(n)))
+        return [{k: getattr(recs[i], k) for k, _ in L.LevelRecord._fields_ if k != "reserved"}
+                for i in range(n.value)]
+
     def _device_graph(self, graph) -> DeviceGraph:
         if isinstance(graph, DeviceGraph):
             return graph
