@@ -34,6 +34,7 @@ CONFIGS = {
     "grid-s64": ("grid256", 64, "u32", False),
     "rmat-s64": ("rmat", 64, "u16", True),
     "rmat-s2048": ("rmat", 2048, "u16", True),
+    "batch512": ("batch512", 0, "u32", False),
 }
 
 
@@ -123,8 +124,101 @@ def calibrate_copy():
                       "launches": 3}))
 
 
+def bench_batch(args):
+    """512 G(n,p) graphs of uneven size, each with its own source set, in one
+    persistent hipGraph-replayed launch, against one engine call per graph."""
+    import numpy as np
+    import torch
+
+    import gpu_bfs as gb
+    from gpu_bfs import workloads as wl
+
+    t0 = time.time()
+    pairs = wl.batch512()
+    graphs = [g for g, _ in pairs]
+    sources = [s for _, s in pairs]
+    print(f"[bench] batch512: {len(graphs)} graphs, V={sum(g.num_vertices for g in graphs)} "
+          f"E={sum(g.num_edges for g in graphs)} S={sum(len(s) for s in sources)} "
+          f"(built in {time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+    ctx = gb.GpuContext(0)
+    eng = gb.GpuBfsEngine(ctx, gb.GpuBfsMethod.Auto)
+    dgs = [gb.DeviceGraph(ctx, g) for g in graphs]
+    b = gb.Batch(eng, dgs, sources)
+    out = torch.empty(b.total, dtype=torch.int32, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+    tuned, tuned_ms = b.autotune(out.data_ptr(), b.total)
+    print(f"[bench] autotune: {tuned['deep_bits']} sources/task, {tuned['deep_threads']} threads, "
+          f"{tuned_ms:.3f} ms", file=sys.stderr, flush=True)
+    b.capture(out.data_ptr(), b.total, stream=stream)
+    for _ in range(args.warmup):
+        b.replay(stream=stream)
+    kernel_ms = []
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        b.replay(stream=stream)
+        kernel_ms.append(b.stats()["kernel_ms"])
+    torch.cuda.synchronize()
+    ms_per_step = (time.perf_counter() - t_start) * 1e3 / args.steps
+    st = b.stats()
+
+    # traversed edges per graph, on the device
+    te = 0
+    for g, (off, S, V) in enumerate(zip(b.offsets, b.counts, b.num_vertices)):
+        if S == 0:
+            continue
+        d = out[int(off):int(off) + int(S) * V].view(int(S), V)
+        deg = torch.as_tensor(graphs[g].degrees(), device="cuda")
+        te += int(((d != -1).to(torch.int64) * deg[None, :]).sum())
+
+    # plain path: one engine call per graph, same outputs
+    plain = torch.empty_like(out)
+    eng_plain = gb.GpuBfsEngine(ctx, gb.GpuBfsMethod.Auto)
+    torch.cuda.synchronize()
+    for g in range(len(dgs)):  # warm every per-graph workspace once
+        if len(sources[g]):
+            eng_plain.run_ms_bfs_device(dgs[g], sources[g], plain.data_ptr() + 4 * int(b.offsets[g]),
+                                        int(b.counts[g]) * b.num_vertices[g], stream=stream)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for g in range(len(dgs)):
+        if len(sources[g]):
+            eng_plain.run_ms_bfs_device(dgs[g], sources[g], plain.data_ptr() + 4 * int(b.offsets[g]),
+                                        int(b.counts[g]) * b.num_vertices[g], stream=stream)
+    torch.cuda.synchronize()
+    plain_ms = (time.perf_counter() - t) * 1e3
+    same = bool(torch.equal(plain, out))
+
+    result = {
+        "metric": "batched MS-BFS: per-batch wall time and GTEPS vs one launch per graph",
+        "value": round(te / (ms_per_step / 1e3) / 1e9, 3), "unit": "GTEPS", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u32 distances", "data": "synthetic (512 seeded G(n,p) graphs)",
+        "config": {"workload": "batch512", "graphs": len(graphs),
+                   "num_vertices": int(sum(g.num_vertices for g in graphs)),
+                   "num_edges": int(sum(g.num_edges for g in graphs)),
+                   "num_sources": int(sum(len(s) for s in sources)),
+                   "tasks": int(st["instances"]), "sources_per_task": int(st["bits_per_instance"]),
+                   "threads": int(tuned["deep_threads"]), "grid_blocks": int(st["grid_blocks"]),
+                   "hipgraph": True},
+        "kernel_ms": round(float(np.mean(kernel_ms)), 4),
+        "traversed_edges": te,
+        "plain_per_graph_ms": round(plain_ms, 3),
+        "speedup_vs_plain": round(plain_ms / ms_per_step, 2),
+        "identical_to_plain": same,
+    }
+    print(json.dumps(result))
+
+
 def main():
     args = parse_args()
+    if args.config == "batch512":
+        if args.gpus != 1:
+            print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+            sys.exit(2)
+        bench_batch(args)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
                           "gpus_requested": args.gpus}))

@@ -217,6 +217,33 @@ int msbfs_engine_check(msbfs_engine *e);
 int msbfs_run_ms_reachability(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources,
                               uint64_t num_sources, uint32_t *words, uint64_t words_len);
 
+/* ---- batched launch ----
+ * Many resident graphs, each with its own source list, traversed by one
+ * persistent launch.  No counterpart in the reference, which makes one
+ * engine call per graph (crates/gpu_bfs/src/engine.rs:219-297).  Sources
+ * are concatenated graph by graph (source_counts[g] entries for graph g);
+ * graph g's distances occupy the output elements
+ * [offsets[g], offsets[g] + source_counts[g]*|V_g|) in the usual
+ * dist[s*|V_g| + v] layout. */
+int msbfs_batch_create(msbfs_engine *e, uint32_t num_graphs, msbfs_graph *const *graphs,
+                       const uint64_t *source_counts, const uint32_t *sources, int dist_type,
+                       msbfs_batch **out);
+int msbfs_batch_destroy(msbfs_batch *b);
+int msbfs_batch_layout(const msbfs_batch *b, uint64_t *total_elements, uint64_t *offsets);
+/* Replaces the sources (same counts); a captured graph sees the new ones. */
+int msbfs_batch_set_sources(msbfs_batch *b, const uint32_t *sources);
+int msbfs_batch_run(msbfs_batch *b, void *out_device, uint64_t out_len, void *stream, int sync);
+/* Records one run (output pointer fixed) into a hipGraph; replay launches it. */
+int msbfs_batch_capture(msbfs_batch *b, void *out_device, uint64_t out_len, void *stream);
+int msbfs_batch_replay(msbfs_batch *b, void *stream, int sync);
+int msbfs_batch_check(msbfs_batch *b);
+int msbfs_batch_stats(msbfs_batch *b, msbfs_run_stats *st);
+int msbfs_batch_set_tuning(msbfs_batch *b, const msbfs_tuning *t);
+/* Autotuner: times every feasible (deep_bits, deep_threads) pair on the
+ * batch and keeps the fastest; returns it and its time. */
+int msbfs_batch_autotune(msbfs_batch *b, void *out_device, uint64_t out_len, msbfs_tuning *best,
+                         float *best_ms);
+
 /* ---- host-side graph utilities (crates/core) ----
  * Library-owned host CSR produced by the generators and converters below;
  * read it with msbfs_host_graph_get and free it with msbfs_host_graph_free. */

@@ -1,0 +1,120 @@
+"""GPU parity of the batched multi-graph launch (msbfs_batch_*): every graph
+of a batch must get exactly the oracle's distances, eagerly, through hipGraph
+replays (several, with other device work in between), after a source update,
+after autotuning, and in u16."""
+import numpy as np
+import pytest
+
+import gpu_bfs as gb
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+def mixed_graphs(n=24, seed=1):
+    rng = np.random.default_rng(seed)
+    graphs, sources = [], []
+    for g in range(n):
+        nv = int(rng.integers(50, 4000))
+        graphs.append(gb.generate_erdos_renyi_np(nv, 6 / (nv - 1), seed=seed * 100 + g))
+        k = int(rng.integers(0, 90)) if g % 7 else 0  # some graphs get no sources
+        sources.append(rng.integers(0, nv, size=k).astype(np.uint32))
+    graphs.append(gb.generate_grid2d(40, 30))
+    sources.append(np.array([0, 1199, 600, 600], np.uint32))
+    graphs.append(gb.generate_rmat(10, 6, directed=True))
+    sources.append(rng.integers(0, 1024, size=70).astype(np.uint32))
+    return graphs, sources
+
+
+def expect_all(graphs, sources):
+    return [orc.ms_bfs_parallel(g.num_vertices, g.row_offsets, g.col_indices, s, 8)
+            .reshape(len(s), g.num_vertices) if len(s) else np.zeros((0, g.num_vertices), np.uint32)
+            for g, s in zip(graphs, sources)]
+
+
+def check(batch, out, expect, u16=False):
+    host = out.cpu().numpy().view(np.uint16 if u16 else np.uint32)
+    for g, (got, exp) in enumerate(zip(batch.split(host), expect)):
+        if u16:
+            exp = np.where(exp == 0xFFFFFFFF, 0xFFFF, exp).astype(np.uint16)
+        assert np.array_equal(got, exp), f"graph {g}"
+
+
+@pytest.fixture(scope="module")
+def setup(ctx):
+    import torch
+
+    graphs, sources = mixed_graphs()
+    eng = gb.GpuBfsEngine(ctx, gb.GpuBfsMethod.Auto)
+    b = gb.Batch(eng, graphs, sources)
+    out = torch.empty(max(b.total, 1), dtype=torch.int32, device="cuda")
+    return graphs, sources, eng, b, out
+
+
+def test_batch_eager(setup):
+    graphs, sources, eng, b, out = setup
+    say("batch eager", len(graphs), "graphs", b.total, "outputs")
+    b.run(out.data_ptr(), b.total)
+    check(b, out, expect_all(graphs, sources))
+    st = b.stats()
+    assert st["family"] == int(gb.Family.DEEP) and st["instances"] > 0
+
+
+def test_batch_graph_replays(setup):
+    import torch
+
+    graphs, sources, eng, b, out = setup
+    expect = expect_all(graphs, sources)
+    b.capture(out.data_ptr(), b.total)
+    for r in range(3):
+        out.fill_(-7)  # eager device work between replays
+        torch.cuda.synchronize()
+        say("batch replay", r)
+        b.replay()
+        check(b, out, expect)
+
+
+def test_batch_set_sources_then_replay(setup):
+    graphs, sources, eng, b, out = setup
+    rng = np.random.default_rng(5)
+    new = [rng.integers(0, g.num_vertices, size=len(s)).astype(np.uint32)
+           for g, s in zip(graphs, sources)]
+    b.set_sources(new)
+    b.capture(out.data_ptr(), b.total)
+    b.replay()
+    check(b, out, expect_all(graphs, new))
+    b.set_sources(sources)
+    b.replay()
+    check(b, out, expect_all(graphs, sources))
+
+
+def test_batch_autotune_keeps_results(setup):
+    graphs, sources, eng, b, out = setup
+    best, ms = b.autotune(out.data_ptr(), b.total)
+    say("autotune", best, ms)
+    assert best["deep_bits"] in (1, 2, 4, 8, 16, 32) and ms > 0
+    b.run(out.data_ptr(), b.total)
+    check(b, out, expect_all(graphs, sources))
+
+
+def test_batch_u16_and_errors(ctx):
+    import torch
+
+    graphs, sources = mixed_graphs(8, seed=3)
+    eng = gb.GpuBfsEngine(ctx, gb.GpuBfsMethod.Auto)
+    b = gb.Batch(eng, graphs, sources, dist_dtype=np.uint16)
+    out = torch.empty(max(b.total, 1), dtype=torch.int16, device="cuda")
+    b.run(out.data_ptr(), b.total)
+    check(b, out, expect_all(graphs, sources), u16=True)
+    bad = [s.copy() for s in sources]
+    bad[2] = np.append(bad[2], np.uint32(graphs[2].num_vertices))
+    with pytest.raises(gb.SourceOutOfBoundsError) as ei:
+        gb.Batch(eng, graphs, bad)
+    assert "graph 2" in str(ei.value)
+    with pytest.raises(gb.MsbfsError) as ei:
+        b.run(out.data_ptr(), b.total - 1)
+    assert ei.value.code == 4

@@ -1,0 +1,454 @@
+// Batched multi-graph launch: many resident graphs, each with its own
+// (uneven) source set, traversed by one persistent deep-family launch.
+//
+// Every graph's sources are cut into tasks of up to B sources; all tasks of
+// all graphs go into one table, largest graphs first, and the persistent
+// workgroups pull them from a global counter.  One workgroup runs one task
+// at a time with the task's visited bits in LDS, so graphs of any size mix
+// freely inside the launch and no device-wide barrier is involved.  A run
+// can be recorded into a hipGraph once and replayed; an autotuner times the
+// candidate task widths and workgroup sizes on the batch itself.
+//
+// The reference has no batched path (one engine call per graph and source
+// set, crates/gpu_bfs/src/engine.rs:219-297).
+#include <numeric>
+
+#include "engine_internal.hpp"
+
+using namespace msbfs;
+
+struct msbfs_batch {
+    msbfs_engine *eng = nullptr;
+    int dtype = MSBFS_DIST_U32;
+    uint32_t n = 0;
+    std::vector<const msbfs_graph *> graphs;
+    std::vector<uint64_t> S, src_off, out_off;
+    uint64_t total_sources = 0, total_out = 0, max_v = 0;
+    int ell = 0;
+    msbfs_tuning tune{};
+    Plan plan;
+    uint64_t ntasks = 0;
+    int blocks = 0;
+    hipStream_t stream = nullptr;
+    DevBuf d_graphs, d_tasks, d_sources, queues, work, flags;
+    PinnedBuf h_flags, h_stage;
+    std::vector<uint32_t> h_sources;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    void *graph_out = nullptr;
+    msbfs_run_stats last{};
+    bool pending = false;
+    bool planned = false;
+};
+
+namespace {
+
+void drop_graph(msbfs_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->graph_out = nullptr;
+}
+
+// Sources per task for the batch: the widest B whose visited bits for the
+// largest graph fit in LDS while the task count still fills the device.
+int batch_bits(const msbfs_batch *b) {
+    const msbfs_context *c = b->eng->ctx;
+    auto fits = [&](int B) { return ((b->max_v * B + 31) / 32) * 4 + kDeepStaticLds <= c->lds_max; };
+    if (b->tune.deep_bits) return fits((int)b->tune.deep_bits) ? (int)b->tune.deep_bits : 0;
+    int best = 0;
+    for (int B : {1, 2, 4, 8, 16, 32}) {
+        if (!fits(B)) break;
+        uint64_t tasks = 0;
+        for (uint32_t g = 0; g < b->n; ++g) tasks += (b->S[g] + B - 1) / B;
+        if (best == 0 || tasks >= 2 * (uint64_t)c->cus) best = B;
+        if (B == 8) break;  // wider tasks only when the autotuner asks for them
+    }
+    return best;
+}
+
+// Builds the task table and device buffers for the current tuning.
+int batch_plan(msbfs_batch *b) {
+    msbfs_context *c = b->eng->ctx;
+    drop_graph(b);
+    Plan &p = b->plan;
+    p = Plan{};
+    p.family = MSBFS_FAMILY_DEEP;
+    p.method = MSBFS_METHOD_WORKGROUP_PERSISTENT_BFS;
+    p.bits = batch_bits(b);
+    if (p.bits == 0)
+        return fail(MSBFS_ERR_UNSUPPORTED,
+                    "batch: the largest graph (%llu vertices) does not fit one workgroup's LDS",
+                    (unsigned long long)b->max_v);
+    const int B = p.bits;
+    std::vector<DeepTask> tasks;
+    for (uint32_t g = 0; g < b->n; ++g)
+        for (uint64_t k = 0; k * B < b->S[g]; ++k) {
+            DeepTask t{};
+            t.graph = g;
+            t.src_off = b->src_off[g] + k * B;
+            t.nsrc = (uint32_t)std::min<uint64_t>(B, b->S[g] - k * B);
+            t.out_off = b->out_off[g] + k * B * b->graphs[g]->V;
+            t.reach_bit = 0;
+            tasks.push_back(t);
+        }
+    // Largest graphs first (longest-processing-time order for the queue).
+    std::stable_sort(tasks.begin(), tasks.end(), [&](const DeepTask &x, const DeepTask &y) {
+        return b->graphs[x.graph]->E > b->graphs[y.graph]->E;
+    });
+    b->ntasks = tasks.size();
+    deep_layout_v(c, b->tune, b->max_v, b->ell, b->ntasks, p);
+    HIP_TRY(deep_prepare(B, p.ell, p.lds));
+    int occ = 0;
+    HIP_TRY(deep_max_blocks_per_cu(B, p.ell, p.threads, p.lds, &occ));
+    if (occ < 1) return fail(MSBFS_ERR_HIP, "batch: deep kernel cannot be resident");
+    b->blocks = (int)std::min<uint64_t>(std::max<uint64_t>(b->ntasks, 1), (uint64_t)c->cus * occ);
+    const uint64_t cap = std::max<uint64_t>((uint64_t)B * b->max_v, (uint64_t)B);
+    const uint64_t spill = cap > p.qcap ? cap - p.qcap : 1;
+    const size_t esz = B <= 8 ? 4 : 8;
+    int rc;
+    if ((rc = ensure_plain(b->queues, (size_t)b->blocks * 2 * spill * esz))) return rc;
+    if ((rc = ensure_plain(b->d_tasks, std::max<size_t>(1, tasks.size()) * sizeof(DeepTask))))
+        return rc;
+    if (!tasks.empty())
+        HIP_TRY(hipMemcpy(b->d_tasks.p, tasks.data(), tasks.size() * sizeof(DeepTask),
+                          hipMemcpyHostToDevice));
+    b->last = msbfs_run_stats{};
+    b->last.method = p.method;
+    b->last.family = p.family;
+    b->last.instances = (uint32_t)b->ntasks;
+    b->last.bits_per_instance = (uint32_t)B;
+    b->last.grid_blocks = (uint32_t)b->blocks;
+    b->last.workspace_bytes = b->queues.bytes;
+    b->planned = true;
+    return MSBFS_OK;
+}
+
+DeepParams batch_params(msbfs_batch *b, void *out) {
+    const uint32_t B = (uint32_t)b->plan.bits;
+    const uint64_t cap = std::max<uint64_t>((uint64_t)B * b->max_v, (uint64_t)B);
+    const uint64_t spill = cap > b->plan.qcap ? cap - b->plan.qcap : 1;
+    DeepParams p{};
+    p.tasks = b->d_tasks.as<DeepTask>();
+    p.graphs = b->d_graphs.as<DeepGraph>();
+    p.sources = b->d_sources.as<uint32_t>();
+    p.out = out;
+    p.queues = b->queues.p;
+    p.cap = spill + b->plan.qcap;
+    p.lds_qcap = b->plan.qcap;
+    p.lds_vis_words = b->plan.vis_words;
+    p.work = b->work.as<uint32_t>();
+    p.flags = b->flags.as<uint32_t>();
+    p.reach = nullptr;
+    p.reach_wpv = 0;
+    p.ntasks = (uint32_t)b->ntasks;
+    p.out_u16 = b->dtype == MSBFS_DIST_U16;
+    return p;
+}
+
+// Enqueues one run: clear the task counter and flags with kernels (no
+// memset nodes, so the sequence can be captured), the persistent launch,
+// and the copy of the flags to the host.
+int batch_enqueue(msbfs_batch *b, void *out, hipStream_t st) {
+    const DeepParams p = batch_params(b, out);
+    HIP_TRY(launch_zero_u32(b->work.as<uint32_t>(), 4, st));
+    HIP_TRY(launch_zero_u32(b->flags.as<uint32_t>(), FlagCount, st));
+    if (b->ntasks)
+        HIP_TRY(launch_deep(b->plan.bits, b->plan.ell, p, b->blocks, b->plan.threads, b->plan.lds,
+                            st));
+    HIP_TRY(hipMemcpyAsync(b->h_flags.p, b->flags.p, FlagCount * 4, hipMemcpyDeviceToHost, st));
+    return MSBFS_OK;
+}
+
+int batch_finish(msbfs_batch *b) {
+    if (!b->pending) return MSBFS_OK;
+    b->pending = false;
+    HIP_TRY(hipEventSynchronize(b->ev[1]));
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, b->ev[0], b->ev[1]) == hipSuccess) {
+        b->last.kernel_ms = ms;
+        b->last.total_ms = ms;
+    }
+    const uint32_t *f = (const uint32_t *)b->h_flags.p;
+    b->last.levels = f[FlagLevels];
+    if (f[FlagQueueOverflow]) return fail(MSBFS_ERR_HIP, "batch: internal frontier queue overflow");
+    if (f[FlagU16Overflow] && b->dtype == MSBFS_DIST_U16)
+        return fail(MSBFS_ERR_DIAMETER_U16, "batch: a distance >= 65535 cannot be stored as u16");
+    return MSBFS_OK;
+}
+
+int batch_check_out(const msbfs_batch *b, const void *out, uint64_t out_len) {
+    if (!out && b->total_out) return fail(MSBFS_ERR_INVALID_ARG, "null output");
+    if (out_len < b->total_out)
+        return fail(MSBFS_ERR_OUT_TOO_SMALL, "batch output holds %llu elements, %llu needed",
+                    (unsigned long long)out_len, (unsigned long long)b->total_out);
+    return MSBFS_OK;
+}
+
+}  // namespace
+
+extern "C" int msbfs_batch_create(msbfs_engine *e, uint32_t n, msbfs_graph *const *graphs,
+                                  const uint64_t *counts, const uint32_t *sources, int dtype,
+                                  msbfs_batch **out) {
+    clear_error();
+    if (!e || !out || (n && (!graphs || !counts)))
+        return fail(MSBFS_ERR_INVALID_ARG, "null argument");
+    if (dtype != MSBFS_DIST_U32 && dtype != MSBFS_DIST_U16)
+        return fail(MSBFS_ERR_INVALID_ARG, "unknown dist type %d", dtype);
+    auto *b = new msbfs_batch;
+    e->refs.fetch_add(1);
+    b->eng = e;
+    b->dtype = dtype;
+    b->n = n;
+    b->tune = e->tune;
+    b->S.resize(n);
+    b->src_off.resize(n);
+    b->out_off.resize(n);
+    for (uint32_t g = 0; g < n; ++g) {
+        const msbfs_graph *G = graphs[g];
+        if (!G) {
+            msbfs_batch_destroy(b);
+            return fail(MSBFS_ERR_INVALID_ARG, "graph %u is null", g);
+        }
+        if (G->ctx != e->ctx) {
+            msbfs_batch_destroy(b);
+            return fail(MSBFS_ERR_INVALID_ARG, "graph %u belongs to another context", g);
+        }
+        const_cast<msbfs_graph *>(G)->refs.fetch_add(1);
+        b->graphs.push_back(G);
+        b->S[g] = counts[g];
+        b->src_off[g] = b->total_sources;
+        b->out_off[g] = b->total_out;
+        b->total_sources += counts[g];
+        b->total_out += counts[g] * G->V;
+        b->max_v = std::max<uint64_t>(b->max_v, G->V);
+    }
+    // One neighbour-table width for the whole launch: used only when every
+    // graph has a table of the same width.
+    b->ell = n ? b->graphs[0]->ell_width : 0;
+    for (uint32_t g = 0; g < n; ++g)
+        if (b->graphs[g]->ell_width != b->ell) b->ell = 0;
+    if (b->total_sources && !sources) {
+        msbfs_batch_destroy(b);
+        return fail(MSBFS_ERR_INVALID_ARG, "null sources");
+    }
+    for (uint32_t g = 0; g < n; ++g) {
+        const msbfs_graph *G = b->graphs[g];
+        for (uint64_t i = 0; i < b->S[g]; ++i) {
+            const uint32_t s = sources[b->src_off[g] + i];
+            if ((uint64_t)s >= G->V) {
+                msbfs_batch_destroy(b);
+                return fail(MSBFS_ERR_SOURCE_OOB,
+                            "graph %u: Source vertex %u at index %llu exceeds graph vertex count "
+                            "%llu",
+                            g, s, (unsigned long long)i, (unsigned long long)G->V);
+            }
+        }
+    }
+    b->h_sources.assign(sources, sources + b->total_sources);
+    msbfs_context *c = e->ctx;
+    hipError_t er = hipSetDevice(c->device);
+    if (er == hipSuccess) er = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (er == hipSuccess) er = hipEventCreate(&b->ev[0]);
+    if (er == hipSuccess) er = hipEventCreate(&b->ev[1]);
+    if (er != hipSuccess) {
+        msbfs_batch_destroy(b);
+        return hip_fail(er, "batch stream");
+    }
+    std::vector<DeepGraph> tab(std::max<uint32_t>(n, 1));
+    for (uint32_t g = 0; g < n; ++g) {
+        tab[g].ro = b->graphs[g]->ro;
+        tab[g].ci = b->graphs[g]->ci;
+        tab[g].ell = b->ell ? b->graphs[g]->ell : nullptr;
+        tab[g].V = (uint32_t)b->graphs[g]->V;
+        tab[g].ell_width = (uint32_t)b->ell;
+    }
+    int rc;
+    if ((rc = ensure_plain(b->d_graphs, tab.size() * sizeof(DeepGraph))) ||
+        (rc = ensure_plain(b->d_sources, std::max<uint64_t>(b->total_sources, 1) * 4)) ||
+        (rc = ensure_plain(b->work, 16)) || (rc = ensure_plain(b->flags, FlagCount * 4))) {
+        msbfs_batch_destroy(b);
+        return rc;
+    }
+    if ((rc = b->h_flags.ensure(FlagCount * 4)) < 0) {
+        msbfs_batch_destroy(b);
+        return -rc;
+    }
+    er = hipMemcpy(b->d_graphs.p, tab.data(), tab.size() * sizeof(DeepGraph), hipMemcpyHostToDevice);
+    if (er == hipSuccess && b->total_sources)
+        er = hipMemcpy(b->d_sources.p, sources, b->total_sources * 4, hipMemcpyHostToDevice);
+    if (er != hipSuccess) {
+        msbfs_batch_destroy(b);
+        return hip_fail(er, "batch upload");
+    }
+    if ((rc = batch_plan(b))) {
+        msbfs_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return MSBFS_OK;
+}
+
+extern "C" int msbfs_batch_destroy(msbfs_batch *b) {
+    if (!b) return MSBFS_OK;
+    (void)hipSetDevice(b->eng->ctx->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    drop_graph(b);
+    for (auto &ev : b->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    for (const msbfs_graph *g : b->graphs) graph_release(const_cast<msbfs_graph *>(g));
+    msbfs_engine *e = b->eng;
+    delete b;
+    engine_release(e);
+    return MSBFS_OK;
+}
+
+extern "C" int msbfs_batch_layout(const msbfs_batch *b, uint64_t *total, uint64_t *offsets) {
+    if (!b) return fail(MSBFS_ERR_INVALID_ARG, "null batch");
+    if (total) *total = b->total_out;
+    if (offsets)
+        for (uint32_t g = 0; g < b->n; ++g) offsets[g] = b->out_off[g];
+    return MSBFS_OK;
+}
+
+extern "C" int msbfs_batch_set_sources(msbfs_batch *b, const uint32_t *sources) {
+    clear_error();
+    if (!b || (b->total_sources && !sources)) return fail(MSBFS_ERR_INVALID_ARG, "null argument");
+    for (uint32_t g = 0; g < b->n; ++g)
+        for (uint64_t i = 0; i < b->S[g]; ++i)
+            if ((uint64_t)sources[b->src_off[g] + i] >= b->graphs[g]->V)
+                return fail(MSBFS_ERR_SOURCE_OOB,
+                            "graph %u: Source vertex %u at index %llu exceeds graph vertex count "
+                            "%llu",
+                            g, sources[b->src_off[g] + i], (unsigned long long)i,
+                            (unsigned long long)b->graphs[g]->V);
+    HIP_TRY(hipSetDevice(b->eng->ctx->device));
+    int rc = batch_finish(b);
+    if (rc) return rc;
+    b->h_sources.assign(sources, sources + b->total_sources);
+    if (b->total_sources)
+        HIP_TRY(hipMemcpy(b->d_sources.p, sources, b->total_sources * 4, hipMemcpyHostToDevice));
+    return MSBFS_OK;
+}
+
+extern "C" int msbfs_batch_run(msbfs_batch *b, void *out, uint64_t out_len, void *stream,
+                               int sync) {
+    clear_error();
+    if (!b) return fail(MSBFS_ERR_INVALID_ARG, "null batch");
+    int rc = batch_check_out(b, out, out_len);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(b->eng->ctx->device));
+    if ((rc = batch_finish(b))) return rc;
+    hipStream_t st = stream ? (hipStream_t)stream : b->stream;
+    HIP_TRY(hipEventRecord(b->ev[0], st));
+    if ((rc = batch_enqueue(b, out, st))) return rc;
+    HIP_TRY(hipEventRecord(b->ev[1], st));
+    b->pending = true;
+    return sync ? batch_finish(b) : MSBFS_OK;
+}
+
+extern "C" int msbfs_batch_capture(msbfs_batch *b, void *out, uint64_t out_len, void *stream) {
+    clear_error();
+    if (!b) return fail(MSBFS_ERR_INVALID_ARG, "null batch");
+    int rc = batch_check_out(b, out, out_len);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(b->eng->ctx->device));
+    if ((rc = batch_finish(b))) return rc;
+    drop_graph(b);
+    hipStream_t st = stream ? (hipStream_t)stream : b->stream;
+    HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    rc = batch_enqueue(b, out, st);
+    hipGraph_t graph = nullptr;
+    hipError_t er = hipStreamEndCapture(st, &graph);
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (er != hipSuccess) return hip_fail(er, "hipStreamEndCapture");
+    b->graph = graph;
+    er = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+    if (er != hipSuccess) {
+        drop_graph(b);
+        return hip_fail(er, "hipGraphInstantiate");
+    }
+    b->graph_out = out;
+    return MSBFS_OK;
+}
+
+extern "C" int msbfs_batch_replay(msbfs_batch *b, void *stream, int sync) {
+    clear_error();
+    if (!b) return fail(MSBFS_ERR_INVALID_ARG, "null batch");
+    if (!b->exec) return fail(MSBFS_ERR_INVALID_ARG, "batch has no captured graph");
+    HIP_TRY(hipSetDevice(b->eng->ctx->device));
+    int rc = batch_finish(b);
+    if (rc) return rc;
+    hipStream_t st = stream ? (hipStream_t)stream : b->stream;
+    HIP_TRY(hipEventRecord(b->ev[0], st));
+    HIP_TRY(hipGraphLaunch(b->exec, st));
+    HIP_TRY(hipEventRecord(b->ev[1], st));
+    b->pending = true;
+    return sync ? batch_finish(b) : MSBFS_OK;
+}
+
+extern "C" int msbfs_batch_check(msbfs_batch *b) {
+    if (!b) return fail(MSBFS_ERR_INVALID_ARG, "null batch");
+    return batch_finish(b);
+}
+
+extern "C" int msbfs_batch_stats(msbfs_batch *b, msbfs_run_stats *st) {
+    if (!b || !st) return fail(MSBFS_ERR_INVALID_ARG, "null argument");
+    int rc = batch_finish(b);
+    *st = b->last;
+    return rc;
+}
+
+extern "C" int msbfs_batch_set_tuning(msbfs_batch *b, const msbfs_tuning *t) {
+    if (!b || !t) return fail(MSBFS_ERR_INVALID_ARG, "null argument");
+    int rc = batch_finish(b);
+    if (rc) return rc;
+    b->tune = *t;
+    return batch_plan(b);
+}
+
+// Autotuner: every feasible (sources per task, workgroup size) pair is run
+// once to warm up and twice timed on the batch itself; the fastest stays.
+extern "C" int msbfs_batch_autotune(msbfs_batch *b, void *out, uint64_t out_len,
+                                    msbfs_tuning *best, float *best_ms) {
+    clear_error();
+    if (!b) return fail(MSBFS_ERR_INVALID_ARG, "null batch");
+    int rc = batch_check_out(b, out, out_len);
+    if (rc) return rc;
+    const msbfs_tuning base = b->tune;
+    msbfs_tuning winner = base;
+    float win_ms = 1e30f;
+    for (uint32_t bits : {1u, 2u, 4u, 8u, 16u, 32u}) {
+        if (((b->max_v * bits + 31) / 32) * 4 + kDeepStaticLds > b->eng->ctx->lds_max) break;
+        for (uint32_t threads : {256u, 512u, 1024u}) {
+            msbfs_tuning t = base;
+            t.deep_bits = bits;
+            t.deep_threads = threads;
+            b->tune = t;
+            if ((rc = batch_plan(b))) continue;
+            float ms = 0;
+            bool ok = true;
+            for (int it = 0; it < 3 && ok; ++it) {
+                if (msbfs_batch_run(b, out, out_len, nullptr, 1) != MSBFS_OK) ok = false;
+                if (it > 0) ms += b->last.kernel_ms;
+            }
+            if (ok && ms / 2 < win_ms) {
+                win_ms = ms / 2;
+                winner = t;
+            }
+        }
+    }
+    b->tune = winner;
+    if ((rc = batch_plan(b))) return rc;
+    if (win_ms >= 1e30f) return fail(MSBFS_ERR_HIP, "autotune: no candidate ran");
+    if (best) *best = winner;
+    if (best_ms) *best_ms = win_ms;
+    clear_error();
+    return MSBFS_OK;
+}

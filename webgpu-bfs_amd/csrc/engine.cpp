@@ -6,143 +6,11 @@
 // and error behaviour; underneath, the wgpu buffers/pipelines and the
 // host-driven level loops are replaced by device-resident graphs and
 // persistent HIP kernels that iterate every level on the device.
-#include <hip/hip_runtime.h>
-
-#include <algorithm>
-#include <cstring>
-#include <mutex>
-#include <vector>
-
-#include "common.hpp"
-#include "kernels.hpp"
+#include "engine_internal.hpp"
 
 using namespace msbfs;
 
 #define MSBFS_VERSION "0.1.0"
-
-namespace {
-
-int hip_fail(hipError_t e, const char *what) {
-    const int code = (e == hipErrorOutOfMemory) ? MSBFS_ERR_OOM : MSBFS_ERR_HIP;
-    return fail(code, "%s: %s", what, hipGetErrorString(e));
-}
-
-#define HIP_TRY(expr)                                      \
-    do {                                                   \
-        hipError_t _e = (expr);                            \
-        if (_e != hipSuccess) return hip_fail(_e, #expr);  \
-    } while (0)
-
-struct DevBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
-    ~DevBuf() { release(); }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    // Grows the buffer to at least `need` bytes.  Returns 1 when it was
-    // (re)allocated (contents undefined), 0 when kept, <0 on failure.
-    int ensure(size_t need) {
-        if (need == 0) need = 16;
-        if (bytes >= need) return 0;
-        release();
-        size_t want = need + need / 8;
-        hipError_t e = hipMalloc(&p, want);
-        if (e != hipSuccess) {
-            p = nullptr;
-            return -hip_fail(e, "hipMalloc workspace");
-        }
-        bytes = want;
-        return 1;
-    }
-    template <class T>
-    T *as() const { return (T *)p; }
-};
-
-struct PinnedBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
-    ~PinnedBuf() {
-        if (p) (void)hipHostFree(p);
-    }
-    int ensure(size_t need) {
-        if (need == 0) need = 16;
-        if (bytes >= need) return 0;
-        if (p) (void)hipHostFree(p);
-        p = nullptr;
-        bytes = 0;
-        hipError_t e = hipHostMalloc(&p, need, hipHostMallocDefault);
-        if (e != hipSuccess) return -hip_fail(e, "hipHostMalloc");
-        bytes = need;
-        return 1;
-    }
-};
-
-constexpr uint32_t kDefaultAlpha = 14;  // push_pull.rs:29-35
-constexpr uint32_t kDefaultBeta = 24;
-
-bool is_pull_only(int m) {
-    return m == MSBFS_METHOD_BIT_PARALLEL_64_PULL || m == MSBFS_METHOD_MULTI_WORD_BIT_PARALLEL ||
-           m == MSBFS_METHOD_INDIRECT_MACRO_BATCH;
-}
-bool is_push_only(int m) {
-    return m == MSBFS_METHOD_BIT_PARALLEL_32 || m == MSBFS_METHOD_BIT_PARALLEL_64 ||
-           m == MSBFS_METHOD_BOOLEAN_REACHABILITY || m == MSBFS_METHOD_SPARSE_QUEUE ||
-           m == MSBFS_METHOD_SPARSE_BIT_PARALLEL_64 ||
-           m == MSBFS_METHOD_MULTI_WORD_BIT_PARALLEL_PUSH;
-}
-
-}  // namespace
-
-struct msbfs_context {
-    int device = 0;
-    hipDeviceProp_t prop{};
-    hipStream_t stream = nullptr;
-    int cus = 0;
-    size_t lds_max = 0;
-    int coop = 0;
-};
-
-struct msbfs_graph {
-    msbfs_context *ctx = nullptr;
-    uint64_t V = 0, E = 0;
-    uint32_t max_degree = 0;
-    int symmetric = 0;
-    uint32_t *ro = nullptr, *ci = nullptr, *co = nullptr, *ri = nullptr;
-    uint32_t *ell = nullptr;  // V x ell_width neighbour slots, 0xFFFFFFFF padded
-    int ell_width = 0;        // 4 or 8 when max degree allows, else 0
-    uint32_t *heavy_out = nullptr, *heavy_in = nullptr;  // degree > kHeavyDegree
-    uint32_t n_heavy_out = 0, n_heavy_in = 0;
-    uint32_t *hchunk_out = nullptr, *hchunk_in = nullptr;  // uint4 (v, e0, e1, 0) items
-    uint32_t n_hchunk_out = 0, n_hchunk_in = 0;
-    uint64_t bytes = 0;
-};
-
-struct msbfs_engine {
-    msbfs_context *ctx = nullptr;
-    int method = MSBFS_METHOD_AUTO;
-    msbfs_tuning tune{};
-    std::mutex mu;
-    // wide workspace (kept all-zero between runs by the kernels)
-    DevBuf seen, frA, frB, nx, qf0, qf1, qc, stamp, ctr, fbA, fbB;
-    uint64_t wide_words = 0;  // entries allocated per state array
-    uint32_t tick_base = 1;
-    bool dirty = true;
-    // deep workspace
-    DevBuf queues, tasks, graphs, work;
-    // shared
-    DevBuf flags, sources, out_tmp, reach_tmp, trace;
-    uint32_t trace_cap = 0;
-    bool trace_valid = false;
-    PinnedBuf h_sources, h_flags, h_tasks;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    msbfs_run_stats last{};
-    hipStream_t last_stream = nullptr;
-    bool pending = false;
-    int pending_dtype = 0;
-};
 
 extern "C" const char *msbfs_version(void) { return MSBFS_VERSION; }
 
@@ -180,11 +48,15 @@ extern "C" int msbfs_context_create(int device, msbfs_context **out) {
     return MSBFS_OK;
 }
 
-extern "C" int msbfs_context_destroy(msbfs_context *c) {
-    if (!c) return MSBFS_OK;
+void msbfs::context_release(msbfs_context *c) {
+    if (!c || c->refs.fetch_sub(1) != 1) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
+}
+
+extern "C" int msbfs_context_destroy(msbfs_context *c) {
+    context_release(c);
     return MSBFS_OK;
 }
 
@@ -219,6 +91,7 @@ extern "C" int msbfs_graph_create(msbfs_context *c, uint64_t V, uint64_t E, cons
     if (rc) return rc;
     HIP_TRY(hipSetDevice(c->device));
     auto *g = new msbfs_graph;
+    c->refs.fetch_add(1);
     g->ctx = c;
     g->V = V;
     g->E = E;
@@ -305,8 +178,14 @@ extern "C" int msbfs_graph_create(msbfs_context *c, uint64_t V, uint64_t E, cons
 }
 
 extern "C" int msbfs_graph_destroy(msbfs_graph *g) {
-    if (!g) return MSBFS_OK;
-    (void)hipSetDevice(g->ctx->device);
+    graph_release(g);
+    return MSBFS_OK;
+}
+
+void msbfs::graph_release(msbfs_graph *g) {
+    if (!g || g->refs.fetch_sub(1) != 1) return;
+    msbfs_context *ctx = g->ctx;
+    (void)hipSetDevice(ctx->device);
     if (g->co && g->co != g->ro) (void)hipFree(g->co);
     if (g->ri && g->ri != g->ci) (void)hipFree(g->ri);
     if (g->ro) (void)hipFree(g->ro);
@@ -317,7 +196,7 @@ extern "C" int msbfs_graph_destroy(msbfs_graph *g) {
     if (g->hchunk_in && g->hchunk_in != g->hchunk_out) (void)hipFree(g->hchunk_in);
     if (g->hchunk_out) (void)hipFree(g->hchunk_out);
     delete g;
-    return MSBFS_OK;
+    context_release(ctx);
 }
 
 extern "C" int msbfs_graph_get_info(const msbfs_graph *g, msbfs_graph_info *info) {
@@ -339,6 +218,7 @@ extern "C" int msbfs_engine_create(msbfs_context *c, int method, msbfs_engine **
         return fail(MSBFS_ERR_INVALID_ARG, "unknown method %d", method);
     HIP_TRY(hipSetDevice(c->device));
     auto *e = new msbfs_engine;
+    c->refs.fetch_add(1);
     e->ctx = c;
     e->method = method;
     for (auto &ev : e->ev) {
@@ -352,13 +232,19 @@ extern "C" int msbfs_engine_create(msbfs_context *c, int method, msbfs_engine **
     return MSBFS_OK;
 }
 
-extern "C" int msbfs_engine_destroy(msbfs_engine *e) {
-    if (!e) return MSBFS_OK;
-    (void)hipSetDevice(e->ctx->device);
+void msbfs::engine_release(msbfs_engine *e) {
+    if (!e || e->refs.fetch_sub(1) != 1) return;
+    msbfs_context *ctx = e->ctx;
+    (void)hipSetDevice(ctx->device);
     (void)hipDeviceSynchronize();
     for (auto &ev : e->ev)
         if (ev) (void)hipEventDestroy(ev);
     delete e;
+    context_release(ctx);
+}
+
+extern "C" int msbfs_engine_destroy(msbfs_engine *e) {
+    engine_release(e);
     return MSBFS_OK;
 }
 
@@ -404,23 +290,9 @@ extern "C" int msbfs_select_auto_method(uint64_t V, uint64_t E, uint32_t max_deg
     return MSBFS_METHOD_SPARSE_QUEUE;
 }
 
-namespace {
+namespace msbfs {
 
-struct Plan {
-    int method = 0;
-    int family = MSBFS_FAMILY_NONE;
-    int policy = MSBFS_DIR_ADAPTIVE;
-    int bits = 0;       // deep: sources per task
-    int ell = 0;        // deep: neighbour-table width (0: CSR)
-    int threads = 0;    // deep: workgroup size
-    uint32_t vis_words = 0;  // deep: LDS words of visited bits
-    uint32_t qcap = 0;  // deep: LDS entries per frontier queue
-    size_t lds = 0;     // deep: dynamic LDS bytes per workgroup
-};
-
-size_t deep_lds_bytes(uint64_t V, int bits) { return ((V * bits + 31) / 32) * 4; }
-
-constexpr size_t kDeepStaticLds = 64;
+static size_t deep_lds_bytes(uint64_t V, int bits) { return ((V * bits + 31) / 32) * 4; }
 
 // Deep family: sources per task so that the task count fills the device,
 // with the task's visited bits fitting in one workgroup's LDS.
@@ -440,15 +312,14 @@ int deep_bits_for(const msbfs_context *c, const msbfs_tuning &t, uint64_t V, uin
 // LDS split of a deep-family workgroup: visited bits first, the rest as two
 // frontier queues.  Enough workgroups per CU are planned to hold every task
 // at once when LDS and threads allow.
-void deep_layout(const msbfs_context *c, const msbfs_tuning &t, const msbfs_graph *g, uint64_t S,
-                 Plan &p) {
+void deep_layout_v(const msbfs_context *c, const msbfs_tuning &t, uint64_t max_v, int ell,
+                   uint64_t ntasks, Plan &p) {
     const int B = p.bits;
-    p.ell = g->ell_width;
+    p.ell = ell;
     p.threads = t.deep_threads ? (int)t.deep_threads : (B >= 4 ? 1024 : 256);
-    p.vis_words = (uint32_t)((g->V * B + 31) / 32);
+    p.vis_words = (uint32_t)((max_v * B + 31) / 32);
     const size_t esz = B <= 8 ? 4 : 8;
     const size_t budget = c->lds_max - kDeepStaticLds;
-    const uint64_t ntasks = (S + B - 1) / B;
     const int max_by_threads = std::max(1, 2048 / p.threads);
     int per_cu = (int)std::min<uint64_t>((ntasks + c->cus - 1) / c->cus, max_by_threads);
     per_cu = std::max(per_cu, 1);
@@ -457,11 +328,27 @@ void deep_layout(const msbfs_context *c, const msbfs_tuning &t, const msbfs_grap
     const size_t per_wg = budget / per_cu;
     size_t qbytes = per_wg > p.vis_words * 4 ? per_wg - p.vis_words * 4 : 0;
     uint64_t qcap = qbytes / (2 * esz);
-    qcap = std::min<uint64_t>(qcap, std::max<uint64_t>((uint64_t)B * g->V, 1));
+    qcap = std::min<uint64_t>(qcap, std::max<uint64_t>((uint64_t)B * max_v, 1));
     qcap = std::min<uint64_t>(qcap, 1u << 20);
     p.qcap = (uint32_t)qcap;
     p.lds = p.vis_words * 4 + 2 * qcap * esz;
 }
+
+int ensure_zeroed(DevBuf &b, size_t bytes, bool force, hipStream_t st) {
+    int r = b.ensure(bytes);
+    if (r < 0) return -r;
+    if (r == 1 || force) HIP_TRY(hipMemsetAsync(b.p, 0, b.bytes, st));
+    return MSBFS_OK;
+}
+
+int ensure_plain(DevBuf &b, size_t bytes) {
+    int r = b.ensure(bytes);
+    return r < 0 ? -r : MSBFS_OK;
+}
+
+}  // namespace msbfs
+
+namespace {
 
 Plan plan_for(const msbfs_engine *e, const msbfs_graph *g, uint64_t S) {
     Plan p;
@@ -496,7 +383,7 @@ Plan plan_for(const msbfs_engine *e, const msbfs_graph *g, uint64_t S) {
             p.family = MSBFS_FAMILY_WIDE;
             p.policy = MSBFS_DIR_ADAPTIVE;
         } else {
-            deep_layout(e->ctx, e->tune, g, S, p);
+            deep_layout_v(e->ctx, e->tune, g->V, g->ell_width, (S + p.bits - 1) / p.bits, p);
         }
     }
     return p;
@@ -513,18 +400,6 @@ int check_sources(const msbfs_graph *g, const uint32_t *sources, uint64_t S) {
                         "Source vertex %u at index %llu exceeds graph vertex count %llu",
                         sources[i], (unsigned long long)i, (unsigned long long)g->V);
     return MSBFS_OK;
-}
-
-int ensure_zeroed(DevBuf &b, size_t bytes, bool force, hipStream_t st) {
-    int r = b.ensure(bytes);
-    if (r < 0) return -r;
-    if (r == 1 || force) HIP_TRY(hipMemsetAsync(b.p, 0, b.bytes, st));
-    return MSBFS_OK;
-}
-
-int ensure_plain(DevBuf &b, size_t bytes) {
-    int r = b.ensure(bytes);
-    return r < 0 ? -r : MSBFS_OK;
 }
 
 // Enqueues one traversal.  `out` (device) may be null when only the

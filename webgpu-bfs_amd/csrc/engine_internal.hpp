@@ -1,0 +1,175 @@
+// Internal host-side types of libmsbfs shared by the engine (engine.cpp)
+// and the batched and serving front ends (batch.cpp, server.cpp): device
+// buffers, the context / graph / engine objects and the run plan.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace msbfs {
+
+inline int hip_fail(hipError_t e, const char *what) {
+    const int code = (e == hipErrorOutOfMemory) ? MSBFS_ERR_OOM : MSBFS_ERR_HIP;
+    return fail(code, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr)                                      \
+    do {                                                   \
+        hipError_t _e = (expr);                            \
+        if (_e != hipSuccess) return hip_fail(_e, #expr);  \
+    } while (0)
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    // Grows the buffer to at least `need` bytes.  Returns 1 when it was
+    // (re)allocated (contents undefined), 0 when kept, <0 on failure.
+    int ensure(size_t need) {
+        if (need == 0) need = 16;
+        if (bytes >= need) return 0;
+        release();
+        size_t want = need + need / 8;
+        hipError_t e = hipMalloc(&p, want);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return -hip_fail(e, "hipMalloc workspace");
+        }
+        bytes = want;
+        return 1;
+    }
+    template <class T>
+    T *as() const { return (T *)p; }
+};
+
+struct PinnedBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    ~PinnedBuf() {
+        if (p) (void)hipHostFree(p);
+    }
+    int ensure(size_t need) {
+        if (need == 0) need = 16;
+        if (bytes >= need) return 0;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        bytes = 0;
+        hipError_t e = hipHostMalloc(&p, need, hipHostMallocDefault);
+        if (e != hipSuccess) return -hip_fail(e, "hipHostMalloc");
+        bytes = need;
+        return 1;
+    }
+};
+
+constexpr uint32_t kDefaultAlpha = 14;  // push_pull.rs:29-35
+constexpr uint32_t kDefaultBeta = 24;
+
+inline bool is_pull_only(int m) {
+    return m == MSBFS_METHOD_BIT_PARALLEL_64_PULL || m == MSBFS_METHOD_MULTI_WORD_BIT_PARALLEL ||
+           m == MSBFS_METHOD_INDIRECT_MACRO_BATCH;
+}
+inline bool is_push_only(int m) {
+    return m == MSBFS_METHOD_BIT_PARALLEL_32 || m == MSBFS_METHOD_BIT_PARALLEL_64 ||
+           m == MSBFS_METHOD_BOOLEAN_REACHABILITY || m == MSBFS_METHOD_SPARSE_QUEUE ||
+           m == MSBFS_METHOD_SPARSE_BIT_PARALLEL_64 ||
+           m == MSBFS_METHOD_MULTI_WORD_BIT_PARALLEL_PUSH;
+}
+
+}  // namespace msbfs
+
+// Context, graphs and engines are reference counted: a graph or engine
+// holds its context, a batch holds its engine and graphs, so the public
+// destroy calls may come in any order.
+struct msbfs_context {
+    std::atomic<int> refs{1};
+    int device = 0;
+    hipDeviceProp_t prop{};
+    hipStream_t stream = nullptr;
+    int cus = 0;
+    size_t lds_max = 0;
+    int coop = 0;
+};
+
+struct msbfs_graph {
+    std::atomic<int> refs{1};
+    msbfs_context *ctx = nullptr;
+    uint64_t V = 0, E = 0;
+    uint32_t max_degree = 0;
+    int symmetric = 0;
+    uint32_t *ro = nullptr, *ci = nullptr, *co = nullptr, *ri = nullptr;
+    uint32_t *ell = nullptr;  // V x ell_width neighbour slots, 0xFFFFFFFF padded
+    int ell_width = 0;        // 4 or 8 when max degree allows, else 0
+    uint32_t *heavy_out = nullptr, *heavy_in = nullptr;  // degree > kHeavyDegree
+    uint32_t n_heavy_out = 0, n_heavy_in = 0;
+    uint32_t *hchunk_out = nullptr, *hchunk_in = nullptr;  // uint4 (v, e0, e1, 0) items
+    uint32_t n_hchunk_out = 0, n_hchunk_in = 0;
+    uint64_t bytes = 0;
+};
+
+struct msbfs_engine {
+    std::atomic<int> refs{1};
+    msbfs_context *ctx = nullptr;
+    int method = MSBFS_METHOD_AUTO;
+    msbfs_tuning tune{};
+    std::mutex mu;
+    // wide workspace (kept all-zero between runs by the kernels)
+    msbfs::DevBuf seen, frA, frB, nx, qf0, qf1, qc, stamp, ctr, fbA, fbB;
+    uint64_t wide_words = 0;  // entries allocated per state array
+    uint32_t tick_base = 1;
+    bool dirty = true;
+    // deep workspace
+    msbfs::DevBuf queues, tasks, graphs, work;
+    // shared
+    msbfs::DevBuf flags, sources, out_tmp, reach_tmp, trace;
+    uint32_t trace_cap = 0;
+    bool trace_valid = false;
+    msbfs::PinnedBuf h_sources, h_flags, h_tasks;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    msbfs_run_stats last{};
+    hipStream_t last_stream = nullptr;
+    bool pending = false;
+    int pending_dtype = 0;
+};
+
+
+namespace msbfs {
+
+struct Plan {
+    int method = 0;
+    int family = MSBFS_FAMILY_NONE;
+    int policy = MSBFS_DIR_ADAPTIVE;
+    int bits = 0;       // deep: sources per task
+    int ell = 0;        // deep: neighbour-table width (0: CSR)
+    int threads = 0;    // deep: workgroup size
+    uint32_t vis_words = 0;  // deep: LDS words of visited bits
+    uint32_t qcap = 0;  // deep: LDS entries per frontier queue
+    size_t lds = 0;     // deep: dynamic LDS bytes per workgroup
+};
+
+constexpr size_t kDeepStaticLds = 64;
+
+void context_release(msbfs_context *c);
+void graph_release(msbfs_graph *g);
+void engine_release(msbfs_engine *e);
+
+int ensure_zeroed(DevBuf &b, size_t bytes, bool force, hipStream_t st);
+int ensure_plain(DevBuf &b, size_t bytes);
+// LDS split of a deep-family workgroup for graphs of up to max_v vertices.
+void deep_layout_v(const msbfs_context *c, const msbfs_tuning &t, uint64_t max_v, int ell,
+                   uint64_t ntasks, Plan &p);
+int deep_bits_for(const msbfs_context *c, const msbfs_tuning &t, uint64_t V, uint64_t S);
+
+}  // namespace msbfs

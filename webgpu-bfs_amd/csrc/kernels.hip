@@ -81,9 +81,11 @@ __device__ __forceinline__ u64 wave_sum64(u64 x) {
     return ((u64)hi << 32) | lo;
 }
 
-// Distance stores are streamed (non-temporal): the output is written once
-// and never read back by the kernels, so it should not evict the graph and
-// frontier state from L2.
+// Distance stores.  NT = true streams the store past L2 (non-temporal): used
+// for the wide family's row-coalesced stores, which fill whole lines and are
+// never read back.  Scattered single-entry stores stay cached so that L2 can
+// merge the partial lines they write.
+template <bool NT = false>
 __device__ __forceinline__ void store_dist(void *out, bool u16, uint64_t idx, uint32_t d,
                                            uint32_t *flags) {
     if (u16) {
@@ -91,17 +93,31 @@ __device__ __forceinline__ void store_dist(void *out, bool u16, uint64_t idx, ui
             atomicOr(&flags[FlagU16Overflow], 1u);
             d = 0xFFFEu;
         }
-        __builtin_nontemporal_store((uint16_t)d, (uint16_t *)out + idx);
+        if (NT)
+            __builtin_nontemporal_store((uint16_t)d, (uint16_t *)out + idx);
+        else
+            ((uint16_t *)out)[idx] = (uint16_t)d;
     } else {
-        __builtin_nontemporal_store(d, (uint32_t *)out + idx);
+        if (NT)
+            __builtin_nontemporal_store(d, (uint32_t *)out + idx);
+        else
+            ((uint32_t *)out)[idx] = d;
     }
 }
 
+template <bool NT = false>
 __device__ __forceinline__ void store_sentinel(void *out, bool u16, uint64_t idx) {
-    if (u16)
-        __builtin_nontemporal_store((uint16_t)0xFFFFu, (uint16_t *)out + idx);
-    else
-        __builtin_nontemporal_store(0xFFFFFFFFu, (uint32_t *)out + idx);
+    if (u16) {
+        if (NT)
+            __builtin_nontemporal_store((uint16_t)0xFFFFu, (uint16_t *)out + idx);
+        else
+            ((uint16_t *)out)[idx] = 0xFFFFu;
+    } else {
+        if (NT)
+            __builtin_nontemporal_store(0xFFFFFFFFu, (uint32_t *)out + idx);
+        else
+            ((uint32_t *)out)[idx] = 0xFFFFFFFFu;
+    }
 }
 
 __global__ void zero_u32_kernel(uint32_t *p, uint64_t n) {
@@ -732,8 +748,8 @@ __device__ __forceinline__ void wide_pull(const WideParams &p, const WaveCtx &x,
                     const uint32_t b = (uint32_t)__builtin_ctzll(un);
                     un &= un - 1;
                     if ((n >> b) & 1ull)
-                        store_dist(p.out, u16, ((uint64_t)w * 64 + b) * V + v0 + x.lane, nd,
-                                   p.flags);
+                        store_dist<true>(p.out, u16, ((uint64_t)w * 64 + b) * V + v0 + x.lane, nd,
+                                         p.flags);
                 }
             }
         }
@@ -953,7 +969,7 @@ __global__ void __launch_bounds__(kWideThreads) wide_bfs_kernel(WideParams p) {
                     const uint32_t b = (uint32_t)__builtin_ctzll(un);
                     un &= un - 1;
                     if ((miss >> b) & 1ull)
-                        store_sentinel(p.out, u16, ((uint64_t)w * 64 + b) * V + v);
+                        store_sentinel<true>(p.out, u16, ((uint64_t)w * 64 + b) * V + v);
                 }
             }
             if (p.reach && ok) {

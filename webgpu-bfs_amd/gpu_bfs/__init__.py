@@ -11,10 +11,11 @@ from .core import (CscGraph, CsrGraph, ReachabilityBitmask, csr_from_undirected_
                    generate_erdos_renyi_np, generate_graph500_rmat, generate_grid2d,
                    generate_rmat, generate_road_like, sample_sources, traversed_edges)
 from .engine import DeviceGraph, Family, GpuBfsEngine, GpuBfsMethod, GpuContext
+from .batch import Batch
 
 __all__ = [
     "CsrGraph", "CscGraph", "ReachabilityBitmask", "GpuContext", "GpuBfsEngine", "GpuBfsMethod",
-    "DeviceGraph", "Family", "MsbfsError", "SourceOutOfBoundsError", "NoDeviceError",
+    "DeviceGraph", "Family", "Batch", "MsbfsError", "SourceOutOfBoundsError", "NoDeviceError",
     "generate_grid2d", "generate_road_like", "generate_rmat", "generate_graph500_rmat",
     "generate_erdos_renyi_np", "csr_from_undirected_pairs", "sample_sources", "traversed_edges",
     "DIST_U32", "DIST_U16", "LIB_PATH", "lib",

@@ -115,6 +115,17 @@ SIGNATURES = {
     "msbfs_run_ms_bfs_device": (I32, [P, P, P, U64, I32, P, U64, P, I32]),
     "msbfs_engine_check": (I32, [P]),
     "msbfs_run_ms_reachability": (I32, [P, P, P, U64, P, U64]),
+    "msbfs_batch_create": (I32, [P, U32, P, P, P, I32, PP]),
+    "msbfs_batch_destroy": (I32, [P]),
+    "msbfs_batch_layout": (I32, [P, ctypes.POINTER(U64), P]),
+    "msbfs_batch_set_sources": (I32, [P, P]),
+    "msbfs_batch_run": (I32, [P, P, U64, P, I32]),
+    "msbfs_batch_capture": (I32, [P, P, U64, P]),
+    "msbfs_batch_replay": (I32, [P, P, I32]),
+    "msbfs_batch_check": (I32, [P]),
+    "msbfs_batch_stats": (I32, [P, ctypes.POINTER(RunStats)]),
+    "msbfs_batch_set_tuning": (I32, [P, ctypes.POINTER(Tuning)]),
+    "msbfs_batch_autotune": (I32, [P, P, U64, ctypes.POINTER(Tuning), ctypes.POINTER(ctypes.c_float)]),
     "msbfs_host_graph_get": (I32, [P, ctypes.POINTER(U64), ctypes.POINTER(U64), PP, PP]),
     "msbfs_host_graph_free": (None, [P]),
     "msbfs_csr_from_edge_list": (I32, [U64, U64, P, P, PP]),
