@@ -35,6 +35,7 @@ CONFIGS = {
     "rmat-s64": ("rmat", 64, "u16", True),
     "rmat-s2048": ("rmat", 2048, "u16", True),
     "batch512": ("batch512", 0, "u32", False),
+    "serve": ("serve", 0, "u16", True),
 }
 
 
@@ -50,6 +51,9 @@ def parse_args():
     ap.add_argument("--cpu-budget-s", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
+    ap.add_argument("--serve-queries", type=int, default=300)
+    ap.add_argument("--serve-rates", default="2000,20000", help="Poisson arrival rates (queries/s)")
+    ap.add_argument("--serve-rmat-scale", type=int, default=22)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain device copy of 256 MiB (counter calibration)")
     return ap.parse_args()
@@ -211,8 +215,135 @@ def bench_batch(args):
     print(json.dumps(result))
 
 
+def bench_serve(args):
+    """Serving: the road-like graph and an R-MAT graph (scale 22) resident;
+    Poisson arrivals, each query picks a graph 50/50 and brings k
+    log-uniform in [1, 512] sources; u16 distances kept on the device.
+    Packed continuous batching against one query per traversal."""
+    import threading
+
+    import numpy as np
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import connected_components
+
+    import gpu_bfs as gb
+    from gpu_bfs import workloads as wl
+
+    t0 = time.time()
+    graphs = [wl.road(), wl.rmat(args.serve_rmat_scale)]
+    print(f"[bench] serve: road V={graphs[0].num_vertices} E={graphs[0].num_edges}; "
+          f"rmat{args.serve_rmat_scale} V={graphs[1].num_vertices} E={graphs[1].num_edges} "
+          f"(built in {time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+    # traversed edges of a source = out-degree sum of its connected component
+    comp_te = []
+    for g in graphs:
+        m = csr_matrix((np.ones(g.num_edges, np.int8), g.col_indices, g.row_offsets),
+                       shape=(g.num_vertices, g.num_vertices))
+        _, lab = connected_components(m, directed=False)
+        tot = np.bincount(lab, weights=g.degrees()).astype(np.int64)
+        comp_te.append(tot[lab])
+    rng = np.random.default_rng(0x5EED)
+    n = args.serve_queries
+    queries = []
+    nonzero = [np.nonzero(g.degrees() > 0)[0].astype(np.uint32) for g in graphs]
+    for _ in range(n):
+        lane = int(rng.integers(0, 2))
+        k = int(np.exp(rng.uniform(0, np.log(512))))
+        k = max(1, min(512, k))
+        queries.append((lane, rng.choice(nonzero[lane], size=k)))
+    te_q = [int(comp_te[lane][src].sum()) for lane, src in queries]
+    ctx = gb.GpuContext(0)
+    dgs = [gb.DeviceGraph(ctx, g) for g in graphs]
+
+    # Output pool: room for a full 2048-source launch on the larger graph
+    # plus the same again for results not yet released.
+    pool = 2 * 2 * 2048 * max(g.num_vertices for g in graphs)
+    print(f"[bench] serve: {n} queries, output pool {pool / 2**30:.1f} GiB", file=sys.stderr,
+          flush=True)
+
+    def run(rate, pack):
+        print(f"[bench] serve: rate {rate:.0f} q/s, pack={pack}", file=sys.stderr, flush=True)
+        srv = gb.Server(ctx, pool_bytes=pool, dist_dtype=np.uint16, pack=pack)
+        for dg in dgs:
+            srv.add_graph(dg)
+        srv.start()
+        # warm both lanes
+        for lane in (0, 1):
+            q = srv.submit(lane, nonzero[lane][:1])
+            assert srv.next_completed(120.0) == q
+            srv.release(q)
+        srv.reset_stats()
+        gaps = rng.exponential(1.0 / rate, size=n)
+        arrive = np.cumsum(gaps)
+        ids = [None] * n
+
+        def client():
+            t_base = time.perf_counter()
+            for i, (lane, src) in enumerate(queries):
+                delay = arrive[i] - (time.perf_counter() - t_base)
+                if delay > 0:
+                    time.sleep(delay)
+                ids[i] = srv.submit(lane, src)
+
+        th = threading.Thread(target=client)
+        t_start = time.perf_counter()
+        th.start()
+        # consume results in completion order and release them at once
+        lat = []
+        while len(lat) < n:
+            qid = srv.next_completed(120.0)
+            info = srv.info(qid)
+            if info["state"] != 2:
+                raise RuntimeError(f"query {qid} failed with status {info['status']}")
+            lat.append(info["latency_ms"])
+            srv.release(qid)
+            if len(lat) % 50 == 0:
+                print(f"[bench] serve: {len(lat)}/{n} done", file=sys.stderr, flush=True)
+        th.join()
+        wall = time.perf_counter() - t_start
+        st = srv.stats()
+        srv.stop()
+        lat = np.array(lat)
+        span = st["span_s"]
+        return {"offered_qps": rate, "queries": n, "launches": int(st["launches"]),
+                "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                "achieved_qps": round(n / span, 1),
+                "sources_per_s": round(sum(len(s) for _, s in queries) / span, 1),
+                "gteps": round(sum(te_q) / span / 1e9, 3), "span_s": round(span, 3),
+                "wall_s": round(wall, 3)}
+
+    rates = [float(r) for r in args.serve_rates.split(",")]
+    packed = [run(r, True) for r in rates]
+    for r in packed:
+        print(f"[bench] packed  {r}", file=sys.stderr, flush=True)
+    single = [run(r, False) for r in rates]
+    for r in single:
+        print(f"[bench] single  {r}", file=sys.stderr, flush=True)
+    best = max(packed, key=lambda r: r["gteps"])
+    result = {
+        "metric": "served MS-BFS: p50/p99 latency and throughput, packed vs one query per launch",
+        "value": best["gteps"], "unit": "GTEPS", "n_gpus": 1, "steps": len(rates),
+        "warmup": 1, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u64 source words / u16 distances",
+        "data": "synthetic (road-like + R-MAT, Poisson arrivals, seeded)",
+        "config": {"workload": "serve", "graphs": ["road-like 114,599 V",
+                                                   f"rmat{args.serve_rmat_scale}"],
+                   "queries_per_rate": n, "k": "log-uniform [1, 512]",
+                   "admission": "u16 x k x |V| output bytes", "pool_bytes": pool},
+        "packed": packed, "one_query_per_launch": single,
+    }
+    print(json.dumps(result))
+
+
 def main():
     args = parse_args()
+    if args.config == "serve":
+        if args.gpus != 1:
+            print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+            sys.exit(2)
+        bench_serve(args)
+        return
     if args.config == "batch512":
         if args.gpus != 1:
             print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))

@@ -244,6 +244,60 @@ int msbfs_batch_set_tuning(msbfs_batch *b, const msbfs_tuning *t);
 int msbfs_batch_autotune(msbfs_batch *b, void *out_device, uint64_t out_len, msbfs_tuning *best,
                          float *best_ms);
 
+/* ---- serving ----
+ * Resident graphs ("lanes") and a scheduler thread that packs queries of
+ * 1..512 sources into shared 64-bit source words, admitting them by the
+ * device memory their distance outputs need (continuous batching: queries
+ * arriving while a lane's traversal runs join its next traversal; lanes run
+ * concurrently).  Results stay in device memory until released.  No
+ * counterpart in the reference. */
+typedef struct msbfs_server_config {
+    uint64_t output_pool_bytes;       /* device memory for results; 0 = 16 GiB */
+    uint32_t max_sources_per_launch;  /* 0 = 2048 */
+    uint32_t max_queries_per_launch;  /* 0 = 512 */
+    int32_t dist_type;                /* MSBFS_DIST_U32 or MSBFS_DIST_U16 */
+    int32_t pack;                     /* 1: many queries per traversal; 0: one per traversal */
+} msbfs_server_config;
+
+typedef struct msbfs_query_info {
+    int32_t state;        /* 0 pending, 1 running, 2 done, 3 failed */
+    int32_t status;       /* msbfs_status of a failed query */
+    uint32_t lane;
+    uint32_t num_sources;
+    uint64_t launch;      /* sequence number of the traversal that ran it */
+    void *device_ptr;     /* dist[s*|V| + v] for its k sources, while held */
+    uint64_t elements;
+    double queue_ms;      /* submit -> launch */
+    double latency_ms;    /* submit -> result ready */
+} msbfs_query_info;
+
+typedef struct msbfs_server_stats {
+    uint64_t completed, failed, launches, sources, pending;
+    uint64_t pool_bytes, pool_used;
+    double span_s;        /* first submit -> last completion */
+    double p50_ms, p99_ms, mean_ms, max_ms;
+} msbfs_server_stats;
+
+int msbfs_server_create(msbfs_context *ctx, const msbfs_server_config *cfg, msbfs_server **out);
+int msbfs_server_add_graph(msbfs_server *s, msbfs_graph *g, int method, uint32_t *lane_id);
+int msbfs_server_set_tuning(msbfs_server *s, uint32_t lane, const msbfs_tuning *t);
+int msbfs_server_start(msbfs_server *s);
+int msbfs_server_submit(msbfs_server *s, uint32_t lane, const uint32_t *sources, uint32_t k,
+                        uint64_t *query_id);
+int msbfs_server_wait(msbfs_server *s, uint64_t query_id, double timeout_s);
+/* Completion queue: the oldest finished (done or failed) query not yet
+ * returned by this call.  Clients that hold results should consume them in
+ * this order and release them, or the output pool can fill with results
+ * while an older query waits for admission. */
+int msbfs_server_next_completed(msbfs_server *s, double timeout_s, uint64_t *query_id);
+int msbfs_server_query_info(msbfs_server *s, uint64_t query_id, msbfs_query_info *info);
+int msbfs_server_copy_result(msbfs_server *s, uint64_t query_id, void *host_out, uint64_t out_len);
+int msbfs_server_release(msbfs_server *s, uint64_t query_id);
+int msbfs_server_get_stats(msbfs_server *s, msbfs_server_stats *st);
+int msbfs_server_reset_stats(msbfs_server *s);
+int msbfs_server_stop(msbfs_server *s);
+int msbfs_server_destroy(msbfs_server *s);
+
 /* ---- host-side graph utilities (crates/core) ----
  * Library-owned host CSR produced by the generators and converters below;
  * read it with msbfs_host_graph_get and free it with msbfs_host_graph_free. */

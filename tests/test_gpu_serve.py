@@ -1,0 +1,153 @@
+"""GPU tests of the serving loop: every query's distances are the oracle's,
+whether it ran alone, packed with others (continuous batching, concurrent
+submission from another thread), or one query per traversal; admission by
+output memory makes queries wait, not fail, when the pool is full."""
+import threading
+
+import numpy as np
+import pytest
+
+import gpu_bfs as gb
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def graphs():
+    road = gb.generate_road_like(20_000, 160, 900, seed=4)
+    rmat = gb.generate_rmat(14, 16)
+    return [road, rmat]
+
+
+def make_queries(graphs, n, seed):
+    rng = np.random.default_rng(seed)
+    qs = []
+    for _ in range(n):
+        lane = int(rng.integers(0, 2))
+        g = graphs[lane]
+        k = int(np.exp(rng.uniform(0, np.log(300))))
+        pool = np.nonzero(g.degrees() > 0)[0]
+        qs.append((lane, rng.choice(pool, size=k).astype(np.uint32)))
+    return qs
+
+
+def expect(graphs, lane, src):
+    g = graphs[lane]
+    d = orc.ms_bfs_parallel(g.num_vertices, g.row_offsets, g.col_indices, src, 8)
+    return np.where(d == 0xFFFFFFFF, 0xFFFF, d).astype(np.uint16)
+
+
+def start_server(ctx, graphs, **kw):
+    s = gb.Server(ctx, dist_dtype=np.uint16, **kw)
+    for g in graphs:
+        s.add_graph(g)
+    s.start()
+    return s
+
+
+def test_mix_equals_alone_equals_oracle(ctx, graphs):
+    qs = make_queries(graphs, 40, seed=1)
+    want = [expect(graphs, lane, src) for lane, src in qs]
+    s = start_server(ctx, graphs, pool_bytes=1 << 30)
+    ids = [None] * len(qs)
+
+    def client():
+        for i, (lane, src) in enumerate(qs):
+            ids[i] = s.submit(lane, src)
+
+    t = threading.Thread(target=client)
+    t.start()
+    t.join()
+    # consume in completion order (the completion queue), release at once
+    got = {}
+    for _ in range(len(qs)):
+        qid = s.next_completed(60.0)
+        assert qid not in got
+        got[qid] = s.result(qid)
+        s.release(qid)
+    mixed = [got[qid] for qid in ids]
+    st = s.stats()
+    say("mix", st)
+    assert st["completed"] == len(qs) and st["failed"] == 0
+    assert st["launches"] < len(qs)  # queries were packed together
+    alone = []
+    for lane, src in qs:
+        qid = s.submit(lane, src)
+        s.wait(qid)
+        alone.append(s.result(qid))
+        s.release(qid)
+    s.stop()
+    for i in range(len(qs)):
+        assert np.array_equal(mixed[i], want[i]), f"query {i} in the mix"
+        assert np.array_equal(alone[i], want[i]), f"query {i} alone"
+
+
+def test_one_query_per_launch(ctx, graphs):
+    qs = make_queries(graphs, 12, seed=2)
+    s = start_server(ctx, graphs, pool_bytes=1 << 30, pack=False)
+    ids = [s.submit(lane, src) for lane, src in qs]
+    for qid, (lane, src) in zip(ids, qs):
+        s.wait(qid)
+        assert np.array_equal(s.result(qid), expect(graphs, lane, src))
+        s.release(qid)
+    assert s.stats()["launches"] == len(qs)
+    s.stop()
+
+
+def test_admission_waits_for_memory(ctx, graphs):
+    road = graphs[0]
+    k = 100
+    per_query = k * road.num_vertices * 2
+    s = start_server(ctx, [road], pool_bytes=int(per_query * 2.5))  # two fit at a time
+    rng = np.random.default_rng(3)
+    srcs = [rng.integers(0, road.num_vertices, size=k).astype(np.uint32) for _ in range(7)]
+    ids = [s.submit(0, src) for src in srcs]
+    for qid, src in zip(ids, srcs):
+        s.wait(qid)
+        info = s.info(qid)
+        assert info["state"] == 2
+        assert np.array_equal(s.result(qid), expect([road], 0, src))
+        s.release(qid)  # frees room for the waiting queries
+    st = s.stats()
+    assert st["completed"] == 7 and st["pool_used"] == 0
+    with pytest.raises(gb.MsbfsError):
+        s.submit(0, np.zeros(513, np.uint32))  # more than 512 sources
+    with pytest.raises(gb.SourceOutOfBoundsError):
+        s.submit(0, [road.num_vertices])
+    s.stop()
+
+
+def test_cooperative_lane_beside_deep_lanes(ctx, graphs):
+    """A lane on the cooperative (grid-barrier) family shares the server with
+    deep-family lanes; the library keeps each cooperative traversal alone on
+    the device, and every result is still the oracle's."""
+    road, rmat = graphs
+    s = gb.Server(ctx, dist_dtype=np.uint16, pool_bytes=1 << 30)
+    s.add_graph(road)                                     # deep
+    s.add_graph(rmat, method=gb.GpuBfsMethod.PushPullAdaptive)  # wide
+    s.add_graph(rmat)                                     # deep (Auto)
+    s.start()
+    rng = np.random.default_rng(7)
+    qs = []
+    for i in range(30):
+        lane = i % 3
+        g = graphs[0] if lane == 0 else rmat
+        pool = np.nonzero(g.degrees() > 0)[0]
+        qs.append((lane, rng.choice(pool, size=int(rng.integers(1, 200))).astype(np.uint32)))
+    say("coop lane beside deep lanes:", len(qs), "queries")
+    ids = [s.submit(lane, src) for lane, src in qs]
+    got = {}
+    for _ in range(len(qs)):
+        qid = s.next_completed(60.0)
+        got[qid] = s.result(qid)
+        s.release(qid)
+    st = s.stats()
+    s.stop()
+    assert st["completed"] == len(qs) and st["failed"] == 0
+    for qid, (lane, src) in zip(ids, qs):
+        assert np.array_equal(got[qid], expect([road, rmat, rmat], lane, src)), lane

@@ -343,9 +343,12 @@ extern "C" int msbfs_batch_run(msbfs_batch *b, void *out, uint64_t out_len, void
     HIP_TRY(hipSetDevice(b->eng->ctx->device));
     if ((rc = batch_finish(b))) return rc;
     hipStream_t st = stream ? (hipStream_t)stream : b->stream;
+    LaunchOrder order(b->eng->ctx, st, false);
+    if ((rc = order.begin())) return rc;
     HIP_TRY(hipEventRecord(b->ev[0], st));
     if ((rc = batch_enqueue(b, out, st))) return rc;
     HIP_TRY(hipEventRecord(b->ev[1], st));
+    if ((rc = order.end())) return rc;
     b->pending = true;
     return sync ? batch_finish(b) : MSBFS_OK;
 }
@@ -386,9 +389,12 @@ extern "C" int msbfs_batch_replay(msbfs_batch *b, void *stream, int sync) {
     int rc = batch_finish(b);
     if (rc) return rc;
     hipStream_t st = stream ? (hipStream_t)stream : b->stream;
+    LaunchOrder order(b->eng->ctx, st, false);
+    if ((rc = order.begin())) return rc;
     HIP_TRY(hipEventRecord(b->ev[0], st));
     HIP_TRY(hipGraphLaunch(b->exec, st));
     HIP_TRY(hipEventRecord(b->ev[1], st));
+    if ((rc = order.end())) return rc;
     b->pending = true;
     return sync ? batch_finish(b) : MSBFS_OK;
 }

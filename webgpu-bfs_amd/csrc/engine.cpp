@@ -51,8 +51,39 @@ extern "C" int msbfs_context_create(int device, msbfs_context **out) {
 void msbfs::context_release(msbfs_context *c) {
     if (!c || c->refs.fetch_sub(1) != 1) return;
     (void)hipSetDevice(c->device);
+    (void)hipDeviceSynchronize();
+    if (c->coop_done) (void)hipEventDestroy(c->coop_done);
+    for (auto &se : c->stream_done) (void)hipEventDestroy(se.second);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
+}
+
+int msbfs::LaunchOrder::begin() {
+    if (c_->coop_valid) HIP_TRY(hipStreamWaitEvent(st_, c_->coop_done, 0));
+    if (coop_)
+        for (auto &se : c_->stream_done)
+            if (se.first != st_) HIP_TRY(hipStreamWaitEvent(st_, se.second, 0));
+    return MSBFS_OK;
+}
+
+int msbfs::LaunchOrder::end() {
+    if (coop_) {
+        if (!c_->coop_done)
+            HIP_TRY(hipEventCreateWithFlags(&c_->coop_done, hipEventDisableTiming));
+        HIP_TRY(hipEventRecord(c_->coop_done, st_));
+        c_->coop_valid = true;
+        return MSBFS_OK;
+    }
+    for (auto &se : c_->stream_done)
+        if (se.first == st_) {
+            HIP_TRY(hipEventRecord(se.second, st_));
+            return MSBFS_OK;
+        }
+    hipEvent_t ev = nullptr;
+    HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    HIP_TRY(hipEventRecord(ev, st_));
+    c_->stream_done.emplace_back(st_, ev);
+    return MSBFS_OK;
 }
 
 extern "C" int msbfs_context_destroy(msbfs_context *c) {
@@ -416,6 +447,9 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
     if ((rc = ensure_plain(e->sources, S * 4))) return rc;
     if ((rc = ensure_plain(e->flags, FlagCount * 4))) return rc;
     if ((rc = e->h_flags.ensure(FlagCount * 4)) < 0) return -rc;
+    const bool coop = plan.family == MSBFS_FAMILY_WIDE;
+    LaunchOrder order(c, st, coop);
+    if ((rc = order.begin())) return rc;
     HIP_TRY(hipEventRecord(e->ev[0], st));
     HIP_TRY(hipMemcpyAsync(e->sources.p, e->h_sources.p, S * 4, hipMemcpyHostToDevice, st));
     HIP_TRY(launch_zero_u32(e->flags.as<uint32_t>(), FlagCount, st));
@@ -586,6 +620,7 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
     }
     HIP_TRY(hipMemcpyAsync(e->h_flags.p, e->flags.p, FlagCount * 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipEventRecord(e->ev[3], st));
+    if ((rc = order.end())) return rc;
     e->pending = true;
     e->last_stream = st;
     e->pending_dtype = dtype;

@@ -42,7 +42,7 @@ struct DevBuf {
         if (need == 0) need = 16;
         if (bytes >= need) return 0;
         release();
-        size_t want = need + need / 8;
+        size_t want = need + std::min<size_t>(need / 8, (size_t)1 << 30);
         hipError_t e = hipMalloc(&p, want);
         if (e != hipSuccess) {
             p = nullptr;
@@ -96,6 +96,15 @@ inline bool is_push_only(int m) {
 struct msbfs_context {
     std::atomic<int> refs{1};
     int device = 0;
+    // Launch ordering.  A cooperative (grid-barrier) traversal needs all its
+    // workgroups resident at once, so it must not share the device with
+    // other traversals of this library: it waits on the last launch of every
+    // other stream, and every later launch waits on it.  All of this is
+    // expressed with stream events; the host never blocks.
+    std::mutex order_mu;
+    hipEvent_t coop_done = nullptr;
+    bool coop_valid = false;
+    std::vector<std::pair<hipStream_t, hipEvent_t>> stream_done;
     hipDeviceProp_t prop{};
     hipStream_t stream = nullptr;
     int cus = 0;
@@ -162,6 +171,21 @@ struct Plan {
 constexpr size_t kDeepStaticLds = 64;
 
 void context_release(msbfs_context *c);
+// Brackets the enqueue of one traversal on stream `st` (coop = it contains a
+// cooperative launch).  The context's ordering lock is held from begin() to
+// the guard's end of life, so no other traversal is enqueued in between.
+class LaunchOrder {
+  public:
+    LaunchOrder(msbfs_context *c, hipStream_t st, bool coop)
+        : c_(c), st_(st), coop_(coop), lk_(c->order_mu) {}
+    int begin();  // stream waits that keep a cooperative kernel alone
+    int end();    // records this traversal's completion event
+  private:
+    msbfs_context *c_;
+    hipStream_t st_;
+    bool coop_;
+    std::unique_lock<std::mutex> lk_;
+};
 void graph_release(msbfs_graph *g);
 void engine_release(msbfs_engine *e);
 

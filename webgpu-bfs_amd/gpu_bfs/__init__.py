@@ -12,10 +12,11 @@ from .core import (CscGraph, CsrGraph, ReachabilityBitmask, csr_from_undirected_
                    generate_rmat, generate_road_like, sample_sources, traversed_edges)
 from .engine import DeviceGraph, Family, GpuBfsEngine, GpuBfsMethod, GpuContext
 from .batch import Batch
+from .server import Server
 
 __all__ = [
     "CsrGraph", "CscGraph", "ReachabilityBitmask", "GpuContext", "GpuBfsEngine", "GpuBfsMethod",
-    "DeviceGraph", "Family", "Batch", "MsbfsError", "SourceOutOfBoundsError", "NoDeviceError",
+    "DeviceGraph", "Family", "Batch", "Server", "MsbfsError", "SourceOutOfBoundsError", "NoDeviceError",
     "generate_grid2d", "generate_road_like", "generate_rmat", "generate_graph500_rmat",
     "generate_erdos_renyi_np", "csr_from_undirected_pairs", "sample_sources", "traversed_edges",
     "DIST_U32", "DIST_U16", "LIB_PATH", "lib",

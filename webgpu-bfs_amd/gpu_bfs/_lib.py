@@ -71,6 +71,25 @@ class LevelRecord(ctypes.Structure):
                 ("mid_us", ctypes.c_double), ("end_us", ctypes.c_double)]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("output_pool_bytes", ctypes.c_uint64), ("max_sources_per_launch", ctypes.c_uint32),
+                ("max_queries_per_launch", ctypes.c_uint32), ("dist_type", ctypes.c_int32),
+                ("pack", ctypes.c_int32)]
+
+
+class QueryInfo(ctypes.Structure):
+    _fields_ = [("state", ctypes.c_int32), ("status", ctypes.c_int32), ("lane", ctypes.c_uint32),
+                ("num_sources", ctypes.c_uint32), ("launch", ctypes.c_uint64),
+                ("device_ptr", ctypes.c_void_p), ("elements", ctypes.c_uint64),
+                ("queue_ms", ctypes.c_double), ("latency_ms", ctypes.c_double)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in ("completed", "failed", "launches", "sources",
+                                               "pending", "pool_bytes", "pool_used")] + \
+               [(n, ctypes.c_double) for n in ("span_s", "p50_ms", "p99_ms", "mean_ms", "max_ms")]
+
+
 class RunStats(ctypes.Structure):
     _fields_ = [("method", ctypes.c_int32), ("family", ctypes.c_int32),
                 ("levels", ctypes.c_uint32), ("push_levels", ctypes.c_uint32),
@@ -126,6 +145,20 @@ SIGNATURES = {
     "msbfs_batch_stats": (I32, [P, ctypes.POINTER(RunStats)]),
     "msbfs_batch_set_tuning": (I32, [P, ctypes.POINTER(Tuning)]),
     "msbfs_batch_autotune": (I32, [P, P, U64, ctypes.POINTER(Tuning), ctypes.POINTER(ctypes.c_float)]),
+    "msbfs_server_create": (I32, [P, ctypes.POINTER(ServerConfig), PP]),
+    "msbfs_server_add_graph": (I32, [P, P, I32, ctypes.POINTER(U32)]),
+    "msbfs_server_set_tuning": (I32, [P, U32, ctypes.POINTER(Tuning)]),
+    "msbfs_server_start": (I32, [P]),
+    "msbfs_server_submit": (I32, [P, U32, P, U32, ctypes.POINTER(U64)]),
+    "msbfs_server_wait": (I32, [P, U64, ctypes.c_double]),
+    "msbfs_server_next_completed": (I32, [P, ctypes.c_double, ctypes.POINTER(U64)]),
+    "msbfs_server_query_info": (I32, [P, U64, ctypes.POINTER(QueryInfo)]),
+    "msbfs_server_copy_result": (I32, [P, U64, P, U64]),
+    "msbfs_server_release": (I32, [P, U64]),
+    "msbfs_server_get_stats": (I32, [P, ctypes.POINTER(ServerStats)]),
+    "msbfs_server_reset_stats": (I32, [P]),
+    "msbfs_server_stop": (I32, [P]),
+    "msbfs_server_destroy": (I32, [P]),
     "msbfs_host_graph_get": (I32, [P, ctypes.POINTER(U64), ctypes.POINTER(U64), PP, PP]),
     "msbfs_host_graph_free": (None, [P]),
     "msbfs_csr_from_edge_list": (I32, [U64, U64, P, P, PP]),
