@@ -168,8 +168,6 @@ struct Plan {
     size_t lds = 0;     // deep: dynamic LDS bytes per workgroup
 };
 
-constexpr size_t kDeepStaticLds = 64;
-
 void context_release(msbfs_context *c);
 // Brackets the enqueue of one traversal on stream `st` (coop = it contains a
 // cooperative launch).  The context's ordering lock is held from begin() to

@@ -33,6 +33,8 @@
 
 #include "kernels.hpp"
 
+#include <type_traits>
+
 namespace cg = cooperative_groups;
 
 namespace msbfs {
@@ -52,7 +54,7 @@ __device__ __forceinline__ uint32_t wave_reserve(Ctr *ctr, bool pred) {
     if (bal) {
         const int leader = __ffsll((long long)bal) - 1;
         if ((int)lane_id() == leader) base = atomicAdd(ctr, (uint32_t)__popcll(bal));
-        base = __shfl(base, leader);
+        base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
     }
     const u64 below = lane_id() ? (bal & ((~0ull) >> (64 - lane_id()))) : 0ull;
     return base + (uint32_t)__popcll(below);
@@ -170,6 +172,9 @@ __device__ __forceinline__ void unpack_entry(u64 x, uint32_t &v, uint32_t &m) {
 
 // A frontier queue: the first `lcap` entries live in LDS, the rest spill to
 // this block's region of global memory.
+// Entries are read through address-space-qualified pointers, so the LDS
+// part is a ds_read (waits on LDS traffic only) and the spill part a global
+// load, never one flat load that waits on both.
 template <typename Entry>
 struct DeepQueue {
     Entry *lds;
@@ -177,7 +182,8 @@ struct DeepQueue {
     uint32_t lcap;
     uint64_t gcap;
     __device__ __forceinline__ Entry get(uint32_t i) const {
-        return i < lcap ? lds[i] : glb[i - lcap];
+        if (i < lcap) return ((__attribute__((address_space(3))) Entry *)lds)[i];
+        return ((__attribute__((address_space(1))) Entry *)glb)[i - lcap];
     }
     __device__ __forceinline__ bool put(uint32_t i, Entry x) const {
         if (i < lcap) {
@@ -192,13 +198,18 @@ struct DeepQueue {
     }
 };
 
-template <int B>
+// Distance d (already clamped for u16) for every source bit of `bits` at
+// vertex v.
+template <int B, bool U16>
 __device__ __forceinline__ void deep_write_bits(const DeepParams &p, uint64_t out_off, uint32_t V,
                                                 uint32_t v, uint32_t bits, uint32_t d) {
-    const bool u16 = p.out_u16 != 0;
+    constexpr bool u16 = U16;
     while (bits) {
-        const uint32_t b = __builtin_ctz(bits);
-        store_dist(p.out, u16, out_off + (uint64_t)b * V + v, d, p.flags);
+        const uint64_t idx = out_off + (uint64_t)__builtin_ctz(bits) * V + v;
+        if (u16)
+            ((uint16_t *)p.out)[idx] = (uint16_t)d;
+        else
+            ((uint32_t *)p.out)[idx] = d;
         bits &= bits - 1;
     }
 }
@@ -208,14 +219,17 @@ __device__ __forceinline__ void deep_write_bits(const DeepParams &p, uint64_t ou
 // neighbour table (graphs of degree <= ELL) a queue entry costs one vector
 // load of its neighbour slots, its claims are independent LDS atomics, and
 // the wave appends all its new entries with one LDS atomic.
-template <int B, int ELL>
+template <int B, int ELL, int DT>
 __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
+    // DT: 0 = no distances (reachability only), 1 = u16, 2 = u32.
+    constexpr bool OUT = DT != 0;
     typedef typename DeepEntry<B>::T Entry;
     constexpr uint32_t VPW = 32 / B;  // vertices per LDS word
     constexpr uint32_t FULL = (B == 32) ? 0xFFFFFFFFu : ((1u << B) - 1u);
     extern __shared__ uint32_t lds[];
     __shared__ uint32_t s_task;
-    __shared__ uint32_t s_cnt[2];
+    __shared__ uint32_t s_cnt[3];
+    __shared__ uint32_t s_sink[64];  // claim target of empty neighbour slots
 
     uint32_t *const vis = lds;
     const uint32_t tid = threadIdx.x, nth = blockDim.x;
@@ -226,7 +240,7 @@ __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
     const uint64_t gcap = p.cap - p.lds_qcap;  // spill entries per queue
     Entry *const gq0 = (Entry *)p.queues + (uint64_t)blockIdx.x * 2 * gcap;
     Entry *const gq1 = gq0 + gcap;
-    const bool u16 = p.out_u16 != 0;
+    constexpr bool u16 = DT == 1;
 
     for (;;) {
         if (tid == 0) s_task = atomicAdd(p.work, 1u);
@@ -241,7 +255,9 @@ __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
         if (tid == 0) {
             s_cnt[0] = task.nsrc;
             s_cnt[1] = 0;
+            s_cnt[2] = 0;
         }
+        if (tid < 64) s_sink[tid] = 0;
         __syncthreads();
 
         DeepQueue<Entry> qa{lq0, gq0, p.lds_qcap, gcap};
@@ -251,78 +267,141 @@ __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
             const uint32_t bit = 1u << tid;
             atomicOr(&vis[s / VPW], bit << ((s % VPW) * B));
             qa.put(tid, pack_entry(s, bit, (Entry *)nullptr));
-            if (p.out) store_dist(p.out, u16, task.out_off + (uint64_t)tid * V + s, 0, p.flags);
         }
         __syncthreads();
 
+        // Distances are written when an entry is expanded: the entries of
+        // level L are the (vertex, source bits) pairs first reached at
+        // distance L.  The store of an entry's lowest bit is issued right
+        // after its neighbour-row load, so it has a whole level to complete
+        // before the next level's load waits on the memory counter.
         uint32_t level = 0;
         for (;;) {
-            const uint32_t n = s_cnt[level & 1];
+            // Three rotating counters: level L reads s_cnt[L%3], appends to
+            // s_cnt[(L+1)%3] and clears s_cnt[(L+2)%3], last read at L-1, so
+            // one barrier per level orders everything.
+            const uint32_t n = s_cnt[level % 3];
             if (n == 0) break;
-            uint32_t *const cnt_next = &s_cnt[(level + 1) & 1];
-            const uint32_t nd = level + 1;
+            uint32_t *const cnt_next = &s_cnt[(level + 1) % 3];
+            if (tid == 0) {
+                s_cnt[(level + 2) % 3] = 0;
+                if (u16 && level >= 0xFFFFu) atomicOr(&p.flags[FlagU16Overflow], 1u);
+            }
+            const uint32_t dval = u16 ? min(level, 0xFFFEu) : level;
             for (uint32_t base = 0; base < n; base += nth) {
                 const uint32_t i = base + tid;
                 uint32_t u = 0, f = 0;
-                if (i < n) unpack_entry(qa.get(i), u, f);
                 if constexpr (ELL > 0) {
-                    // All neighbour slots in one or two 16-byte loads.
-                    uint32_t nb[ELL];
-                    if (i < n) {
-                        const uint4 *row = (const uint4 *)(g.ell + (uint64_t)u * ELL);
-#pragma unroll
-                        for (int k = 0; k < ELL / 4; ++k) {
-                            const uint4 x = row[k];
-                            nb[4 * k] = x.x;
-                            nb[4 * k + 1] = x.y;
-                            nb[4 * k + 2] = x.z;
-                            nb[4 * k + 3] = x.w;
+                    if (base + (tid & ~63u) >= n) continue;  // wave-uniform: no entries
+                    // A pass within the LDS part of the queue reads it with
+                    // ds_reads only, so the row load is not held behind a
+                    // wait on outstanding stores.
+                    auto pass = [&](auto spill) {
+                        if (i < n) {
+                            Entry x;
+                            if constexpr (decltype(spill)::value)
+                                x = qa.get(i);
+                            else
+                                x = ((__attribute__((address_space(3))) Entry *)qa.lds)[i];
+                            unpack_entry(x, u, f);
                         }
-                    } else {
+                        // All neighbour slots in one or two 16-byte global loads.
+                        typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+                        uint32_t nb[ELL];
+                        if (i < n) {
+                            const __attribute__((address_space(1))) u32x4 *row =
+                                (const __attribute__((address_space(1))) u32x4 *)(g.ell +
+                                                                                  (uint64_t)u * ELL);
 #pragma unroll
-                        for (int k = 0; k < ELL; ++k) nb[k] = 0xFFFFFFFFu;
-                    }
-                    uint32_t newly[ELL];
-                    uint32_t c = 0;
+                            for (int k = 0; k < ELL / 4; ++k) {
+                                const u32x4 x = row[k];
+                                nb[4 * k] = x.x;
+                                nb[4 * k + 1] = x.y;
+                                nb[4 * k + 2] = x.z;
+                                nb[4 * k + 3] = x.w;
+                            }
+                        } else {
 #pragma unroll
-                    for (int k = 0; k < ELL; ++k) {
-                        newly[k] = 0;
-                        const uint32_t v = nb[k];
-                        if (v != 0xFFFFFFFFu) {
-                            const uint32_t wi = v / VPW, sh = (v % VPW) * B;
-                            const uint32_t m = f & ~((vis[wi] >> sh) & FULL);
-                            if (m) {
-                                const uint32_t old = atomicOr(&vis[wi], m << sh);
-                                newly[k] = m & ~((old >> sh) & FULL);
+                            for (int k = 0; k < ELL; ++k) nb[k] = 0xFFFFFFFFu;
+                        }
+                        // This entry's lowest source bit, one straight-line
+                        // store per lane; a lane without an entry repeats lane
+                        // 0's store (entries are contiguous, so lane 0 has one).
+                        if constexpr (OUT) {
+                            const uint32_t b0 = __builtin_ctz(f | 0x80000000u);
+                            const uint32_t u_l0 = (uint32_t)__builtin_amdgcn_readlane((int)u, 0);
+                            const uint32_t b_l0 = (uint32_t)__builtin_amdgcn_readlane((int)b0, 0);
+                            const uint32_t su = i < n ? u : u_l0;
+                            const uint32_t sb = i < n ? b0 : b_l0;
+                            const uint64_t idx = task.out_off + (uint64_t)sb * V + su;
+                            if constexpr (u16)
+                                ((__attribute__((address_space(1))) uint16_t *)p.out)[idx] = (uint16_t)dval;
+                            else
+                                ((__attribute__((address_space(1))) uint32_t *)p.out)[idx] = dval;
+                        }
+                        // Claims: every slot issues its LDS atomic at once (an
+                        // empty slot ORs 0 into this lane's sink word), then one
+                        // wait; the returned words give the bits that are new.
+                        __attribute__((address_space(3))) uint32_t *const lv =
+                            (__attribute__((address_space(3))) uint32_t *)vis;
+                        __attribute__((address_space(3))) uint32_t *const sink =
+                            (__attribute__((address_space(3))) uint32_t *)s_sink;
+                        uint32_t old[ELL], sh[ELL];
+#pragma unroll
+                        for (int k = 0; k < ELL; ++k) {
+                            const uint32_t v = nb[k];
+                            const bool ok = v != 0xFFFFFFFFu;
+                            sh[k] = ok ? (v % VPW) * B : 0u;
+                            __attribute__((address_space(3))) uint32_t *const w =
+                                ok ? lv + v / VPW : sink + lane;
+                            old[k] = __hip_atomic_fetch_or(w, ok ? f << sh[k] : 0u,
+                                                           __ATOMIC_RELAXED,
+                                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                        }
+                        uint32_t newly[ELL];
+                        uint32_t c = 0;
+#pragma unroll
+                        for (int k = 0; k < ELL; ++k) {
+                            newly[k] = nb[k] != 0xFFFFFFFFu ? f & ~((old[k] >> sh[k]) & FULL) : 0u;
+                            c += newly[k] != 0;
+                        }
+                        // Exclusive prefix of c over the wave from ELL ballots:
+                        // lanes below with c > j, summed over j.
+                        uint32_t pre = 0, tot = 0;
+#pragma unroll
+                        for (int j = 0; j < ELL; ++j) {
+                            const u64 bj = __ballot(c > (uint32_t)j);
+                            pre += (uint32_t)__popcll(bj & lt_mask);
+                            tot += (uint32_t)__popcll(bj);
+                        }
+                        uint32_t wbase = 0;
+                        if (tot) {
+                            if (lane == 0) wbase = atomicAdd(cnt_next, tot);
+                            wbase = (uint32_t)__builtin_amdgcn_readfirstlane((int)wbase);
+                        }
+                        uint32_t pos = wbase + pre;
+#pragma unroll
+                        for (int k = 0; k < ELL; ++k) {
+                            if (newly[k]) {
+                                if (!qb.put(pos, pack_entry(nb[k], newly[k], (Entry *)nullptr)))
+                                    atomicOr(&p.flags[FlagQueueOverflow], 1u);
+                                ++pos;
                             }
                         }
-                        c += newly[k] != 0;
-                    }
-                    // Exclusive prefix of c over the wave from ELL ballots:
-                    // lanes below with c > j, summed over j.
-                    uint32_t pre = 0, tot = 0;
-#pragma unroll
-                    for (int j = 0; j < ELL; ++j) {
-                        const u64 bj = __ballot(c > (uint32_t)j);
-                        pre += (uint32_t)__popcll(bj & lt_mask);
-                        tot += (uint32_t)__popcll(bj);
-                    }
-                    uint32_t wbase = 0;
-                    if (tot) {
-                        if (lane == 0) wbase = atomicAdd(cnt_next, tot);
-                        wbase = __shfl(wbase, 0);
-                    }
-                    uint32_t pos = wbase + pre;
-#pragma unroll
-                    for (int k = 0; k < ELL; ++k) {
-                        if (newly[k]) {
-                            if (!qb.put(pos, pack_entry(nb[k], newly[k], (Entry *)nullptr)))
-                                atomicOr(&p.flags[FlagQueueOverflow], 1u);
-                            ++pos;
-                            if (p.out) deep_write_bits<B>(p, task.out_off, V, nb[k], newly[k], nd);
-                        }
-                    }
+                        // The entry's other source bits (rare: sources whose
+                        // searches reach the vertex at the same level).
+                        if (OUT && i < n)
+                            deep_write_bits<B, u16>(p, task.out_off, V, u, f & (f - 1), dval);
+                    };
+                    if (base + nth <= qa.lcap)
+                        pass(std::false_type{});
+                    else
+                        pass(std::true_type{});
                 } else {
+                    if (i < n) {
+                        unpack_entry(qa.get(i), u, f);
+                        if (OUT) deep_write_bits<B, u16>(p, task.out_off, V, u, f, dval);
+                    }
                     uint32_t e = 0, eend = 0;
                     if (i < n) {
                         e = g.ro[u];
@@ -343,16 +422,11 @@ __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
                             ++e;
                         }
                         const uint32_t pos = wave_reserve(cnt_next, newly != 0);
-                        if (newly) {
-                            if (!qb.put(pos, pack_entry(v, newly, (Entry *)nullptr)))
-                                atomicOr(&p.flags[FlagQueueOverflow], 1u);
-                            if (p.out) deep_write_bits<B>(p, task.out_off, V, v, newly, nd);
-                        }
+                        if (newly && !qb.put(pos, pack_entry(v, newly, (Entry *)nullptr)))
+                            atomicOr(&p.flags[FlagQueueOverflow], 1u);
                     }
                 }
             }
-            __syncthreads();
-            if (tid == 0) s_cnt[level & 1] = 0;
             DeepQueue<Entry> tq = qa;
             qa = qb;
             qb = tq;
@@ -363,7 +437,7 @@ __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
 
         // Unreached (source, vertex) pairs get the sentinel; rows are written
         // by consecutive threads, so the stores coalesce.
-        if (p.out) {
+        if constexpr (OUT) {
             for (uint32_t b = 0; b < task.nsrc; ++b) {
                 const uint64_t row = task.out_off + (uint64_t)b * V;
                 for (uint32_t v = tid; v < V; v += nth)
@@ -388,19 +462,47 @@ __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
 template <int B, int ELL>
 static hipError_t deep_launch_t(const DeepParams &p, int blocks, int threads, size_t lds,
                                 hipStream_t st) {
-    hipLaunchKernelGGL((deep_bfs_kernel<B, ELL>), dim3(blocks), dim3(threads), lds, st, p);
+    if (!p.out)
+        hipLaunchKernelGGL((deep_bfs_kernel<B, ELL, 0>), dim3(blocks), dim3(threads), lds, st, p);
+    else if (p.out_u16)
+        hipLaunchKernelGGL((deep_bfs_kernel<B, ELL, 1>), dim3(blocks), dim3(threads), lds, st, p);
+    else
+        hipLaunchKernelGGL((deep_bfs_kernel<B, ELL, 2>), dim3(blocks), dim3(threads), lds, st, p);
     return hipGetLastError();
 }
 
-template <int B, int ELL>
-static hipError_t deep_prepare_t(size_t lds) {
-    return hipFuncSetAttribute((const void *)deep_bfs_kernel<B, ELL>,
+template <int B, int ELL, int DT>
+static hipError_t deep_prepare_dt(size_t lds) {
+    hipFuncAttributes a;
+    hipError_t e = hipFuncGetAttributes(&a, (const void *)deep_bfs_kernel<B, ELL, DT>);
+    if (e != hipSuccess) return e;
+    if (a.sharedSizeBytes > kDeepStaticLds) return hipErrorInvalidConfiguration;
+    return hipFuncSetAttribute((const void *)deep_bfs_kernel<B, ELL, DT>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 }
 
 template <int B, int ELL>
+static hipError_t deep_prepare_t(size_t lds) {
+    hipError_t e = deep_prepare_dt<B, ELL, 0>(lds);
+    if (e == hipSuccess) e = deep_prepare_dt<B, ELL, 1>(lds);
+    if (e == hipSuccess) e = deep_prepare_dt<B, ELL, 2>(lds);
+    return e;
+}
+
+// Resident workgroups per CU: the least over the output variants.
+template <int B, int ELL>
 static hipError_t deep_occ_t(int threads, size_t lds, int *out) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, deep_bfs_kernel<B, ELL>, threads, lds);
+    int o[3] = {0, 0, 0};
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&o[0], deep_bfs_kernel<B, ELL, 0>,
+                                                                threads, lds);
+    if (e == hipSuccess)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&o[1], deep_bfs_kernel<B, ELL, 1>,
+                                                         threads, lds);
+    if (e == hipSuccess)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&o[2], deep_bfs_kernel<B, ELL, 2>,
+                                                         threads, lds);
+    *out = std::min(o[0], std::min(o[1], o[2]));
+    return e;
 }
 
 // Dispatches a templated call over bits in {1,2,4,8,16,32} x ell in {0,4,8}.

@@ -104,6 +104,10 @@ enum : int {
 hipError_t launch_deep(int bits, int ell, const DeepParams &p, int blocks, int threads,
                        size_t lds_bytes, hipStream_t st);
 hipError_t deep_max_blocks_per_cu(int bits, int ell, int threads, size_t lds_bytes, int *out);
+// LDS the deep kernel declares statically (task slot, level counters, claim
+// sink), reserved out of the per-workgroup budget; deep_prepare fails if the
+// compiled kernel needs more.
+constexpr size_t kDeepStaticLds = 1024;
 hipError_t deep_prepare(int bits, int ell, size_t lds_bytes);
 
 hipError_t launch_wide(int tile_words, const WideParams &p, int blocks, int threads, size_t lds,
