@@ -153,12 +153,17 @@ extern "C" int msbfs_graph_create(msbfs_context *c, uint64_t V, uint64_t E, cons
         }
     }
     // Fixed-width neighbour table for low-degree graphs: the deep family
-    // fetches a vertex's whole neighbourhood with one vector load.
+    // fetches a vertex's whole neighbourhood with one vector load.  Empty
+    // slots hold the vertex itself: a frontier vertex's own bits are already
+    // visited, so claiming it finds nothing new and needs no test.
     if (e == hipSuccess && V > 0 && g->max_degree <= 8) {
         g->ell_width = g->max_degree <= 4 ? 4 : 8;
-        std::vector<uint32_t> ell(V * g->ell_width, 0xFFFFFFFFu);
-        for (uint64_t v = 0; v < V; ++v)
-            for (uint32_t k = 0; k < ro[v + 1] - ro[v]; ++k) ell[v * g->ell_width + k] = ci[ro[v] + k];
+        std::vector<uint32_t> ell(V * g->ell_width);
+        for (uint64_t v = 0; v < V; ++v) {
+            const uint32_t d = ro[v + 1] - ro[v];
+            for (uint32_t k = 0; k < (uint32_t)g->ell_width; ++k)
+                ell[v * g->ell_width + k] = k < d ? ci[ro[v] + k] : (uint32_t)v;
+        }
         e = upload(&g->ell, ell.data(), ell.size());
     }
     // Vertices whose edge lists the wide family hands to a whole wave.
@@ -331,11 +336,21 @@ int deep_bits_for(const msbfs_context *c, const msbfs_tuning &t, uint64_t V, uin
     if (V >= (1ull << 24)) return 0;
     auto fits = [&](int b) { return deep_lds_bytes(V, b) + kDeepStaticLds <= c->lds_max; };
     if (t.deep_bits) return fits((int)t.deep_bits) ? (int)t.deep_bits : 0;
-    const uint64_t target = (uint64_t)std::max(1, c->cus);
+    // Several tasks per CU hide each other's per-level latency, so the
+    // width is the largest that still gives every CU k tasks at once, for
+    // k = 4, then 2, with k sets of visited bits and queues fitting its LDS
+    // (road-like |V| = 114,599, S = 2048, ms: k=4 3.73, k=2 3.91, k=1 4.22).
+    const uint64_t cus = (uint64_t)std::max(1, c->cus);
+    for (uint64_t k : {4u, 2u}) {
+        for (int b : {32, 16, 8, 4, 2, 1}) {
+            const size_t need = deep_lds_bytes(V, b) + kDeepStaticLds + 2 * 512 * (b <= 8 ? 4 : 8);
+            if ((S + b - 1) / b >= k * cus && k * need <= c->lds_max) return b;
+        }
+    }
     int best = 0;
     for (int b : {1, 2, 4, 8, 16, 32}) {
         if (!fits(b)) break;
-        if (best == 0 || (S + b - 1) / b >= target) best = b;
+        if (best == 0 || (S + b - 1) / b >= cus) best = b;
     }
     return best;
 }
@@ -347,16 +362,30 @@ void deep_layout_v(const msbfs_context *c, const msbfs_tuning &t, uint64_t max_v
                    uint64_t ntasks, Plan &p) {
     const int B = p.bits;
     p.ell = ell;
-    p.threads = t.deep_threads ? (int)t.deep_threads : (B >= 4 ? 1024 : 256);
+    // Workgroup size: the 2048 threads of a CU shared by the tasks it holds
+    // at once (capped by LDS), between 256 and 1024.
+    if (t.deep_threads) {
+        p.threads = (int)t.deep_threads;
+    } else {
+        const uint64_t vis = ((max_v * B + 31) / 32) * 4;
+        const uint64_t fit = std::max<uint64_t>(
+            1, c->lds_max / (vis + 2 * 512 * (B <= 8 ? 4 : 8) + kDeepStaticLds));
+        const uint64_t k = std::min<uint64_t>((ntasks + c->cus - 1) / c->cus, fit);
+        if (k <= 1)
+            p.threads = B >= 4 ? 1024 : 256;
+        else
+            p.threads = k < 4 ? 512 : 256;
+    }
     p.vis_words = (uint32_t)((max_v * B + 31) / 32);
     const size_t esz = B <= 8 ? 4 : 8;
-    const size_t budget = c->lds_max - kDeepStaticLds;
     const int max_by_threads = std::max(1, 2048 / p.threads);
     int per_cu = (int)std::min<uint64_t>((ntasks + c->cus - 1) / c->cus, max_by_threads);
     per_cu = std::max(per_cu, 1);
     const size_t min_q = 2 * 512 * esz;
-    while (per_cu > 1 && p.vis_words * 4 + min_q > budget / per_cu) --per_cu;
-    const size_t per_wg = budget / per_cu;
+    // each resident workgroup also holds the kernel's static LDS
+    auto dyn_budget = [&](int k) { return c->lds_max / k - kDeepStaticLds; };
+    while (per_cu > 1 && p.vis_words * 4 + min_q > dyn_budget(per_cu)) --per_cu;
+    const size_t per_wg = dyn_budget(per_cu);
     size_t qbytes = per_wg > p.vis_words * 4 ? per_wg - p.vis_words * 4 : 0;
     uint64_t qcap = qbytes / (2 * esz);
     qcap = std::min<uint64_t>(qcap, std::max<uint64_t>((uint64_t)B * max_v, 1));

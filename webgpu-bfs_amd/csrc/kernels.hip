@@ -200,16 +200,17 @@ struct DeepQueue {
 
 // Distance d (already clamped for u16) for every source bit of `bits` at
 // vertex v.
+// Rows of one task (< 2^29 entries: |V| < 2^24, <= 32 sources) are indexed
+// with 32-bit offsets from the task's first row.
 template <int B, bool U16>
-__device__ __forceinline__ void deep_write_bits(const DeepParams &p, uint64_t out_off, uint32_t V,
-                                                uint32_t v, uint32_t bits, uint32_t d) {
-    constexpr bool u16 = U16;
+__device__ __forceinline__ void deep_write_bits(void *out_task, uint32_t V, uint32_t v,
+                                                uint32_t bits, uint32_t d) {
     while (bits) {
-        const uint64_t idx = out_off + (uint64_t)__builtin_ctz(bits) * V + v;
-        if (u16)
-            ((uint16_t *)p.out)[idx] = (uint16_t)d;
+        const uint32_t off = (uint32_t)__builtin_ctz(bits) * V + v;
+        if constexpr (U16)
+            ((__attribute__((address_space(1))) uint16_t *)out_task)[off] = (uint16_t)d;
         else
-            ((uint32_t *)p.out)[idx] = d;
+            ((__attribute__((address_space(1))) uint32_t *)out_task)[off] = d;
         bits &= bits - 1;
     }
 }
@@ -228,8 +229,8 @@ __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
     constexpr uint32_t FULL = (B == 32) ? 0xFFFFFFFFu : ((1u << B) - 1u);
     extern __shared__ uint32_t lds[];
     __shared__ uint32_t s_task;
-    __shared__ uint32_t s_cnt[3];
-    __shared__ uint32_t s_sink[64];  // claim target of empty neighbour slots
+    __shared__ uint32_t s_cnt[4];
+    __shared__ Entry s_sink[64];  // target of the appends a lane does not make
 
     uint32_t *const vis = lds;
     const uint32_t tid = threadIdx.x, nth = blockDim.x;
@@ -256,8 +257,8 @@ __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
             s_cnt[0] = task.nsrc;
             s_cnt[1] = 0;
             s_cnt[2] = 0;
+            s_cnt[3] = 0;
         }
-        if (tid < 64) s_sink[tid] = 0;
         __syncthreads();
 
         DeepQueue<Entry> qa{lq0, gq0, p.lds_qcap, gcap};
@@ -270,6 +271,8 @@ __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
         }
         __syncthreads();
 
+        void *const out_task =
+            OUT ? (void *)((char *)p.out + task.out_off * (u16 ? 2 : 4)) : nullptr;
         // Distances are written when an entry is expanded: the entries of
         // level L are the (vertex, source bits) pairs first reached at
         // distance L.  The store of an entry's lowest bit is issued right
@@ -277,14 +280,19 @@ __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
         // before the next level's load waits on the memory counter.
         uint32_t level = 0;
         for (;;) {
-            // Three rotating counters: level L reads s_cnt[L%3], appends to
-            // s_cnt[(L+1)%3] and clears s_cnt[(L+2)%3], last read at L-1, so
+            // Rotating counters: level L reads s_cnt[L&3], appends to
+            // s_cnt[(L+1)&3] and clears s_cnt[(L+2)&3], last read at L-2, so
             // one barrier per level orders everything.
-            const uint32_t n = s_cnt[level % 3];
+            // The first pass's entry is read alongside the count (both LDS,
+            // one wait); the queue's LDS part always holds index tid.
+            Entry x0 = 0;
+            if constexpr (ELL > 0)
+                if (tid < qa.lcap) x0 = ((__attribute__((address_space(3))) Entry *)qa.lds)[tid];
+            const uint32_t n = s_cnt[level & 3];
             if (n == 0) break;
-            uint32_t *const cnt_next = &s_cnt[(level + 1) % 3];
+            uint32_t *const cnt_next = &s_cnt[(level + 1) & 3];
             if (tid == 0) {
-                s_cnt[(level + 2) % 3] = 0;
+                s_cnt[(level + 2) & 3] = 0;
                 if (u16 && level >= 0xFFFFu) atomicOr(&p.flags[FlagU16Overflow], 1u);
             }
             const uint32_t dval = u16 ? min(level, 0xFFFEu) : level;
@@ -297,18 +305,19 @@ __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
                     // ds_reads only, so the row load is not held behind a
                     // wait on outstanding stores.
                     auto pass = [&](auto spill) {
-                        if (i < n) {
+                        const bool act = i < n;
+                        if (act) {
                             Entry x;
                             if constexpr (decltype(spill)::value)
                                 x = qa.get(i);
                             else
-                                x = ((__attribute__((address_space(3))) Entry *)qa.lds)[i];
+                                x = base == 0 ? x0 : ((__attribute__((address_space(3))) Entry *)qa.lds)[i];
                             unpack_entry(x, u, f);
                         }
                         // All neighbour slots in one or two 16-byte global loads.
                         typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
                         uint32_t nb[ELL];
-                        if (i < n) {
+                        if (act) {
                             const __attribute__((address_space(1))) u32x4 *row =
                                 (const __attribute__((address_space(1))) u32x4 *)(g.ell +
                                                                                   (uint64_t)u * ELL);
@@ -320,9 +329,6 @@ __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
                                 nb[4 * k + 2] = x.z;
                                 nb[4 * k + 3] = x.w;
                             }
-                        } else {
-#pragma unroll
-                            for (int k = 0; k < ELL; ++k) nb[k] = 0xFFFFFFFFu;
                         }
                         // This entry's lowest source bit, one straight-line
                         // store per lane; a lane without an entry repeats lane
@@ -331,39 +337,35 @@ __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
                             const uint32_t b0 = __builtin_ctz(f | 0x80000000u);
                             const uint32_t u_l0 = (uint32_t)__builtin_amdgcn_readlane((int)u, 0);
                             const uint32_t b_l0 = (uint32_t)__builtin_amdgcn_readlane((int)b0, 0);
-                            const uint32_t su = i < n ? u : u_l0;
-                            const uint32_t sb = i < n ? b0 : b_l0;
-                            const uint64_t idx = task.out_off + (uint64_t)sb * V + su;
+                            const uint32_t off = (act ? b0 : b_l0) * V + (act ? u : u_l0);
                             if constexpr (u16)
-                                ((__attribute__((address_space(1))) uint16_t *)p.out)[idx] = (uint16_t)dval;
+                                ((__attribute__((address_space(1))) uint16_t *)out_task)[off] = (uint16_t)dval;
                             else
-                                ((__attribute__((address_space(1))) uint32_t *)p.out)[idx] = dval;
+                                ((__attribute__((address_space(1))) uint32_t *)out_task)[off] = dval;
                         }
-                        // Claims: every slot issues its LDS atomic at once (an
-                        // empty slot ORs 0 into this lane's sink word), then one
-                        // wait; the returned words give the bits that are new.
-                        __attribute__((address_space(3))) uint32_t *const lv =
-                            (__attribute__((address_space(3))) uint32_t *)vis;
-                        __attribute__((address_space(3))) uint32_t *const sink =
-                            (__attribute__((address_space(3))) uint32_t *)s_sink;
-                        uint32_t old[ELL], sh[ELL];
-#pragma unroll
-                        for (int k = 0; k < ELL; ++k) {
-                            const uint32_t v = nb[k];
-                            const bool ok = v != 0xFFFFFFFFu;
-                            sh[k] = ok ? (v % VPW) * B : 0u;
-                            __attribute__((address_space(3))) uint32_t *const w =
-                                ok ? lv + v / VPW : sink + lane;
-                            old[k] = __hip_atomic_fetch_or(w, ok ? f << sh[k] : 0u,
-                                                           __ATOMIC_RELAXED,
-                                                           __HIP_MEMORY_SCOPE_WORKGROUP);
-                        }
+                        // Claims: all slots' LDS atomics issued at once, then
+                        // one wait; the returned words give the new bits.
+                        // (Empty slots name the vertex itself: nothing new.)
                         uint32_t newly[ELL];
                         uint32_t c = 0;
+                        if (act) {
+                            __attribute__((address_space(3))) uint32_t *const lv =
+                                (__attribute__((address_space(3))) uint32_t *)vis;
+                            uint32_t old[ELL];
 #pragma unroll
-                        for (int k = 0; k < ELL; ++k) {
-                            newly[k] = nb[k] != 0xFFFFFFFFu ? f & ~((old[k] >> sh[k]) & FULL) : 0u;
-                            c += newly[k] != 0;
+                            for (int k = 0; k < ELL; ++k)
+                                old[k] = __hip_atomic_fetch_or(lv + nb[k] / VPW,
+                                                               f << ((nb[k] % VPW) * B),
+                                                               __ATOMIC_RELAXED,
+                                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+#pragma unroll
+                            for (int k = 0; k < ELL; ++k) {
+                                newly[k] = f & ~(old[k] >> ((nb[k] % VPW) * B));
+                                c += newly[k] != 0;
+                            }
+                        } else {
+#pragma unroll
+                            for (int k = 0; k < ELL; ++k) newly[k] = 0;
                         }
                         // Exclusive prefix of c over the wave from ELL ballots:
                         // lanes below with c > j, summed over j.
@@ -380,18 +382,33 @@ __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
                             wbase = (uint32_t)__builtin_amdgcn_readfirstlane((int)wbase);
                         }
                         uint32_t pos = wbase + pre;
+                        if (wbase + tot <= qb.lcap) {
+                            // The wave's appends all land in LDS: every slot
+                            // writes, the ones without new bits into this
+                            // lane's sink word, so there are no branches.
+                            __attribute__((address_space(3))) Entry *const q =
+                                (__attribute__((address_space(3))) Entry *)qb.lds;
+                            __attribute__((address_space(3))) Entry *const sk =
+                                (__attribute__((address_space(3))) Entry *)s_sink + lane;
 #pragma unroll
-                        for (int k = 0; k < ELL; ++k) {
-                            if (newly[k]) {
-                                if (!qb.put(pos, pack_entry(nb[k], newly[k], (Entry *)nullptr)))
-                                    atomicOr(&p.flags[FlagQueueOverflow], 1u);
-                                ++pos;
+                            for (int k = 0; k < ELL; ++k) {
+                                *(newly[k] ? q + pos : sk) = pack_entry(nb[k], newly[k], (Entry *)nullptr);
+                                pos += newly[k] != 0;
+                            }
+                        } else {
+#pragma unroll
+                            for (int k = 0; k < ELL; ++k) {
+                                if (newly[k]) {
+                                    if (!qb.put(pos, pack_entry(nb[k], newly[k], (Entry *)nullptr)))
+                                        atomicOr(&p.flags[FlagQueueOverflow], 1u);
+                                    ++pos;
+                                }
                             }
                         }
                         // The entry's other source bits (rare: sources whose
                         // searches reach the vertex at the same level).
-                        if (OUT && i < n)
-                            deep_write_bits<B, u16>(p, task.out_off, V, u, f & (f - 1), dval);
+                        if constexpr (OUT && B > 1)
+                            if (act) deep_write_bits<B, u16>(out_task, V, u, f & (f - 1), dval);
                     };
                     if (base + nth <= qa.lcap)
                         pass(std::false_type{});
@@ -400,7 +417,7 @@ __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
                 } else {
                     if (i < n) {
                         unpack_entry(qa.get(i), u, f);
-                        if (OUT) deep_write_bits<B, u16>(p, task.out_off, V, u, f, dval);
+                        if (OUT) deep_write_bits<B, u16>(out_task, V, u, f, dval);
                     }
                     uint32_t e = 0, eend = 0;
                     if (i < n) {
