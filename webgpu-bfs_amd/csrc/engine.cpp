@@ -509,6 +509,10 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
             e->tick_base = 1;
         }
         if ((rc = ensure_plain(e->ctr, 4 * sizeof(WideCounters)))) return rc;
+        // zero row (one word per source word), read in place of frontier
+        // words of parents outside the frontier; never written
+        if ((rc = ensure_zeroed(e->zrow, std::max<uint64_t>(W, 64) * 8, false, st))) return rc;
+        if ((rc = ensure_plain(e->act, 3 * W * 8))) return rc;  // set up by the seed kernel
         e->dirty = false;
 
         WideParams p{};
@@ -564,6 +568,8 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
         }
         if (nb < tv) nb = tv;
         p.pull_nb = nb;
+        p.zrow = e->zrow.as<unsigned long long>();
+        p.act = e->act.as<unsigned long long>();
         const size_t lds = wide_pull_lds(nb, (uint32_t)W);
         if (e->tune.trace_levels) {
             e->trace_cap = e->tune.trace_levels;

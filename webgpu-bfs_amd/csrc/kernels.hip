@@ -613,6 +613,10 @@ __global__ void wide_seed_kernel(WideParams p) {
     atomicOr(&p.frA[idx], bit);
     atomicOr(&p.fbA[s >> 5], 1u << (s & 31));
     if (p.out) store_dist(p.out, p.out_u16 != 0, i * p.V + s, 0, p.flags);
+    if (i < p.W) {  // active sources: all of them at level 0
+        p.act[i] = word_valid(p.S, (uint32_t)i);
+        p.act[p.W + i] = 0;
+    }
     if (atomicExch(&p.stamp[s], p.tick_base) != p.tick_base) {
         const uint32_t pos = atomicAdd(&p.ctr[3].next_count, 1u);
         p.qf0[pos] = s;
@@ -631,6 +635,10 @@ enum { DirPush = 0, DirPull = 1 };
 struct WaveCtx {
     uint32_t lane, vi, wi;
     uint64_t gwave, nwaves;
+    u64 *act_lds;        // this workgroup's new-bit sources of the level (LDS, W words)
+    const u64 *act_cur;  // sources whose frontier is not empty at this level (W words)
+    uint32_t *blk;       // LDS: pull block's new frontier vertices: [0] count, [1..3]
+                         // bitmap words from the block's first, [4..68) the vertices
 };
 
 __device__ __forceinline__ bool in_frontier(const uint32_t *fb, uint32_t u) {
@@ -753,6 +761,7 @@ __device__ __forceinline__ void wide_push_filter(const WideParams &p, const Wave
                     const u64 s = p.seen[idx];
                     n = xw & ~s;
                     if (n) {
+                        atomicOr(&x.act_lds[w], n);
                         p.seen[idx] = s | n;
                         nxt[idx] = n;
                         if (p.out) wide_write_lane(p, w, v, n, nd);
@@ -820,22 +829,35 @@ __device__ __forceinline__ void wide_pull(const WideParams &p, const WaveCtx &x,
                 if (okv && w < W) {
                     const uint64_t idx = (uint64_t)v * W + w;
                     const u64 s = p.seen[idx];
-                    const u64 need = ~s & word_valid(p.S, w);
+                    const u64 need = ~s & word_valid(p.S, w) & x.act_cur[w];
                     if (need) {
+                        // Parents in steps of 8 with no branches inside a
+                        // step: indices past the end repeat the last parent,
+                        // parents outside the frontier read the zero row, so
+                        // each step is three rounds of independent loads
+                        // (indices, bitmap words, frontier words).
+                        typedef const __attribute__((address_space(1))) uint32_t *g32;
+                        typedef const __attribute__((address_space(1))) u64 *g64;
+                        constexpr int U = 8;
                         u64 acc = 0;
-                        for (uint32_t e = e0; e < e1; e += 4) {
-                            const uint32_t u0 = p.ri[e];
-                            const uint32_t u1 = e + 1 < e1 ? p.ri[e + 1] : u0;
-                            const uint32_t u2 = e + 2 < e1 ? p.ri[e + 2] : u0;
-                            const uint32_t u3 = e + 3 < e1 ? p.ri[e + 3] : u0;
-                            if (in_frontier(fb_cur, u0)) acc |= cur[(uint64_t)u0 * W + w];
-                            if (in_frontier(fb_cur, u1)) acc |= cur[(uint64_t)u1 * W + w];
-                            if (in_frontier(fb_cur, u2)) acc |= cur[(uint64_t)u2 * W + w];
-                            if (in_frontier(fb_cur, u3)) acc |= cur[(uint64_t)u3 * W + w];
+                        for (uint32_t e = e0; e < e1; e += U) {
+                            uint32_t uu[U];
+#pragma unroll
+                            for (int k = 0; k < U; ++k)
+                                uu[k] = ((g32)p.ri)[min(e + (uint32_t)k, e1 - 1)];
+                            uint32_t fw[U];
+#pragma unroll
+                            for (int k = 0; k < U; ++k) fw[k] = ((g32)fb_cur)[uu[k] >> 5];
+#pragma unroll
+                            for (int k = 0; k < U; ++k) {
+                                const bool inf = (fw[k] >> (uu[k] & 31)) & 1u;
+                                acc |= *(g64)(inf ? cur + (uint64_t)uu[k] * W + w : p.zrow + w);
+                            }
                             if ((acc & need) == need) break;
                         }
                         n = acc & need;
                         if (n) {
+                            atomicOr(&x.act_lds[w], n);
                             p.seen[idx] = s | n;
                             nxt[idx] = n;
                         }
@@ -849,15 +871,36 @@ __device__ __forceinline__ void wide_pull(const WideParams &p, const WaveCtx &x,
             bool vany = false;
 #pragma unroll
             for (uint32_t k = 0; k < TW; ++k) vany |= ((bal >> (k * TV + x.vi)) & 1ull) != 0;
+            // New frontier vertices are gathered in LDS and published once
+            // per block (one global atomic instead of one per wave).
             const bool app = x.wi == 0 && okv && vany;
-            const uint32_t pos = wave_reserve(&c->next_count, app);
-            if (app) {
-                qnext[pos] = v;
-                mark_frontier(fb_next, v);
-                my_deg += p.ro[v + 1] - p.ro[v];
+            const u64 ab = __ballot(app);
+            if (ab) {
+                uint32_t lb = 0;
+                const int leader = __ffsll((long long)ab) - 1;
+                if ((int)x.lane == leader) lb = atomicAdd(&x.blk[0], (uint32_t)__popcll(ab));
+                lb = (uint32_t)__builtin_amdgcn_readlane((int)lb, leader);
+                if (app) {
+                    x.blk[4 + lb + (uint32_t)__popcll(ab & ((1ull << x.lane) - 1ull))] = v;
+                    atomicOr(&x.blk[1 + ((v >> 5) - (v0 >> 5))], 1u << (v & 31));
+                    my_deg += p.ro[v + 1] - p.ro[v];
+                }
             }
         }
         __syncthreads();
+        if (wib == 0) {
+            const uint32_t cnt = x.blk[0];
+            uint32_t gb = 0;
+            if (cnt) {
+                if (x.lane == 0) gb = atomicAdd(&c->next_count, cnt);
+                gb = (uint32_t)__builtin_amdgcn_readfirstlane((int)gb);
+            }
+            if (x.lane < cnt) qnext[gb + x.lane] = x.blk[4 + x.lane];
+            if (x.lane < 3) {
+                const uint32_t m = x.blk[1 + x.lane];
+                if (m) atomicOr(&fb_next[(v0 >> 5) + x.lane], m);
+            }
+        }
         if (p.out) {
             for (uint32_t w = wib; w < W; w += nwb) {
                 const bool okl = x.lane < NB && v0 + x.lane < V;
@@ -872,6 +915,7 @@ __device__ __forceinline__ void wide_pull(const WideParams &p, const WaveCtx &x,
                 }
             }
         }
+        if (wib == 0 && x.lane < 4) x.blk[x.lane] = 0;  // wave 0 read them above
         __syncthreads();
     }
     // Heavy in-lists: one wave per (chunk, word group); lanes vi take every
@@ -885,19 +929,26 @@ __device__ __forceinline__ void wide_pull(const WideParams &p, const WaveCtx &x,
         const uint32_t w = (uint32_t)(t % WG) * TW + x.wi;
         const uint64_t idx = (uint64_t)h * W + w;
         u64 need = 0;
-        if (w < W) need = ~p.seen[idx] & word_valid(p.S, w) & ~p.nx[idx];
+        if (w < W) need = ~p.seen[idx] & word_valid(p.S, w) & ~p.nx[idx] & x.act_cur[w];
         if (!__any(need != 0)) continue;  // wave-uniform
+        // Branch-free steps as in the light pull: indices past the chunk
+        // repeat its last parent, parents outside the frontier (and lanes
+        // past the last word) read the zero row.
+        typedef const __attribute__((address_space(1))) uint32_t *g32;
+        typedef const __attribute__((address_space(1))) u64 *g64;
+        const bool wok = w < W;
         u64 acc = 0;
         for (uint32_t base = e0; base < e1; base += TV * U) {
-            if (need) {
+            uint32_t uu[U];
 #pragma unroll
-                for (uint32_t k = 0; k < U; ++k) {
-                    const uint32_t e = base + k * TV + x.vi;
-                    if (e < e1) {
-                        const uint32_t u = p.ri[e];
-                        if (in_frontier(fb_cur, u)) acc |= cur[(uint64_t)u * W + w];
-                    }
-                }
+            for (uint32_t k = 0; k < U; ++k) uu[k] = ((g32)p.ri)[min(base + k * TV + x.vi, e1 - 1)];
+            uint32_t fw[U];
+#pragma unroll
+            for (uint32_t k = 0; k < U; ++k) fw[k] = ((g32)fb_cur)[uu[k] >> 5];
+#pragma unroll
+            for (uint32_t k = 0; k < U; ++k) {
+                const bool inf = wok && ((fw[k] >> (uu[k] & 31)) & 1u);
+                acc |= *(g64)(inf ? cur + (uint64_t)uu[k] * W + w : p.zrow + (w & 63));
             }
             const u64 g = group_or64<TV>(acc);
             if (__all((g & need) == need)) break;
@@ -941,6 +992,7 @@ __device__ __forceinline__ void wide_pull_resolve(const WideParams &p, const Wav
                 const u64 s = p.seen[idx];
                 const u64 n = xw & ~s & word_valid(p.S, w);
                 if (n) {
+                    atomicOr(&x.act_lds[w], n);
                     p.seen[idx] = s | n;
                     nxt[idx] = n;
                     my_bits += __popcll(n);
@@ -973,6 +1025,12 @@ __global__ void __launch_bounds__(kWideThreads) wide_bfs_kernel(WideParams p) {
     const uint64_t nthreads = (uint64_t)gridDim.x * blockDim.x;
     const uint32_t V = p.V, W = p.W;
     const bool first = blockIdx.x == 0 && threadIdx.x == 0;
+    x.act_lds = lds_n + (size_t)p.pull_nb * (W + 1);
+    for (uint32_t w = threadIdx.x; w < W; w += blockDim.x) x.act_lds[w] = 0;
+    __shared__ uint32_t s_blk[4 + 64];
+    x.blk = s_blk;
+    if (threadIdx.x < 4) s_blk[threadIdx.x] = 0;
+    __syncthreads();
 
     u64 *cur = p.frA, *nxt = p.frB;
     uint32_t *fb_cur = p.fbA, *fb_next = p.fbB;
@@ -1007,6 +1065,7 @@ __global__ void __launch_bounds__(kWideThreads) wide_bfs_kernel(WideParams p) {
         }
         const uint32_t nd = L + 1;
         const uint32_t tick = p.tick_base + L + 1;
+        x.act_cur = p.act + (size_t)(L % 3) * W;
         u64 my_bits = 0, my_deg = 0;
         if (p.trace && first && L < p.trace_cap) {
             p.trace[4 * L] = wall_clock64();
@@ -1042,6 +1101,18 @@ __global__ void __launch_bounds__(kWideThreads) wide_bfs_kernel(WideParams p) {
             z->cand_count = 0;
             z->active_bits = 0;
             z->deg_sum = 0;
+        }
+        // Sources that reached new vertices at this level are the ones with a
+        // frontier at the next: one atomic per word per workgroup.  The ring
+        // slot of level L+2 was last read at level L-1 and is cleared here.
+        __syncthreads();
+        for (uint32_t w = threadIdx.x; w < W; w += blockDim.x) {
+            const u64 a = x.act_lds[w];
+            if (a) {
+                atomicOr(&p.act[(size_t)((L + 1) % 3) * W + w], a);
+                x.act_lds[w] = 0;
+            }
+            if (blockIdx.x == 0) p.act[(size_t)((L + 2) % 3) * W + w] = 0;
         }
         grid.sync();
         n_front = ld_dev_u32(&c->next_count);

@@ -81,6 +81,8 @@ struct WideParams {
     const uint4 *hchunk_in;   // same for heavy in-lists (CSC)
     uint32_t n_hchunk_out, n_hchunk_in;
     uint32_t pull_nb;       // vertices per workgroup block in pull levels
+    const unsigned long long *zrow;  // >= W zero words: gather target of parents not in the frontier
+    unsigned long long *act;  // ring of 3 x W words: sources whose frontier is not empty
     uint32_t trace_cap;     // levels the trace buffer holds
     unsigned long long *trace;  // per level: start clock, dir<<32|frontier, mid clock, new bits
 };
@@ -120,6 +122,8 @@ hipError_t launch_zero_u32(uint32_t *ptr, uint64_t n, hipStream_t st);
 constexpr int kWideThreads = 512;
 
 // LDS bytes of the pull block buffer: nb vertices x (W + 1) words.
-inline size_t wide_pull_lds(uint32_t nb, uint32_t W) { return (size_t)nb * (W + 1) * 8; }
+// Dynamic LDS of the wide kernel: a pull block's new words (nb rows of W+1)
+// and the workgroup's active-source words (W).
+inline size_t wide_pull_lds(uint32_t nb, uint32_t W) { return ((size_t)nb * (W + 1) + W) * 8; }
 
 }  // namespace msbfs
