@@ -791,6 +791,50 @@ __device__ __forceinline__ void wide_push_filter(const WideParams &p, const Wave
     }
 }
 
+// Scan-then-gather step for tiles of TW >= 8 words (TV = 64/TW lane
+// groups, group vi = lanes wi*TV + vi): each lane of a group loads one
+// in-edge of [ebase, eend) (TW edges per group), tests its frontier bit, and
+// the group's ballot lists the parents in the frontier; their frontier rows
+// are then gathered four at a time, parent index broadcast from the lane that
+// loaded it.  Returns the OR of this lane's word over those parents.
+template <int TW>
+__device__ __forceinline__ u64 scan_gather(const WideParams &p, const WaveCtx &x, const u64 *cur,
+                                           const uint32_t *fb, uint32_t ebase, uint32_t eend,
+                                           uint32_t w, bool wok) {
+    constexpr uint32_t TV = 64 / TW;
+    static_assert(TV <= 8, "scan_gather needs tiles of 8 or more words");
+    typedef const __attribute__((address_space(1))) uint32_t *g32;
+    typedef const __attribute__((address_space(1))) u64 *g64;
+    const u64 gmask = (~0ull / ((1ull << TV) - 1ull)) << x.vi;
+    const uint32_t e = ebase + x.wi;
+    uint32_t uu = 0;
+    bool inf = false;
+    if (e < eend) {
+        uu = ((g32)p.ri)[e];
+        inf = (((g32)fb)[uu >> 5] >> (uu & 31)) & 1u;
+    }
+    u64 m = __ballot(inf) & gmask;
+    u64 acc = 0;
+    const uint64_t W = p.W;
+    while (__any(m != 0)) {
+        uint32_t j[4];
+        bool ok[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            ok[k] = m != 0;
+            j[k] = ok[k] ? (uint32_t)__builtin_ctzll(m) : x.lane;
+            m &= m - 1;
+        }
+        uint32_t u[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) u[k] = (uint32_t)__shfl((int)uu, (int)j[k]);
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            acc |= *(g64)((ok[k] && wok) ? cur + (uint64_t)u[k] * W + w : p.zrow + (w & 63));
+    }
+    return acc;
+}
+
 // Pull: each workgroup takes blocks of NB consecutive vertices.  Every
 // (vertex, word) with unseen bits ORs the frontier words of its parents
 // (only those set in the frontier bitmap are read) until all still-needed
@@ -826,7 +870,29 @@ __device__ __forceinline__ void wide_pull(const WideParams &p, const WaveCtx &x,
             for (uint32_t g = 0; g < WG; ++g) {
                 const uint32_t w = g * TW + x.wi;
                 u64 n = 0;
-                if (okv && w < W) {
+                if constexpr (TW >= 8) {
+                    // The vertex's TW lanes scan its in-list together.
+                    const bool wok = okv && w < W;
+                    const uint64_t idx = (uint64_t)v * W + w;
+                    const u64 s = wok ? p.seen[idx] : ~0ull;
+                    const u64 need = wok ? ~s & word_valid(p.S, w) & x.act_cur[w] : 0ull;
+                    const u64 gmask = (~0ull / ((1ull << TV) - 1ull)) << x.vi;
+                    bool gon = (__ballot(need != 0) & gmask) != 0 && e0 < e1;
+                    u64 acc = 0;
+                    uint32_t eb = e0;
+                    while (__any(gon)) {
+                        acc |= scan_gather<TW>(p, x, cur, fb_cur, gon ? eb : e1, e1, w, wok);
+                        eb += TW;
+                        const bool sat = (acc & need) == need;
+                        gon = gon && eb < e1 && (__ballot(!sat) & gmask) != 0;
+                    }
+                    n = acc & need;
+                    if (n) {
+                        atomicOr(&x.act_lds[w], n);
+                        p.seen[idx] = s | n;
+                        nxt[idx] = n;
+                    }
+                } else if (okv && w < W) {
                     const uint64_t idx = (uint64_t)v * W + w;
                     const u64 s = p.seen[idx];
                     const u64 need = ~s & word_valid(p.S, w) & x.act_cur[w];
@@ -938,6 +1004,14 @@ __device__ __forceinline__ void wide_pull(const WideParams &p, const WaveCtx &x,
         typedef const __attribute__((address_space(1))) u64 *g64;
         const bool wok = w < W;
         u64 acc = 0;
+        if constexpr (TW >= 8) {
+            // each lane group scans its own TW-edge slice of every 64 edges
+            for (uint32_t base = e0; base < e1; base += 64) {
+                acc |= scan_gather<TW>(p, x, cur, fb_cur, base + x.vi * TW, e1, w, wok);
+                const u64 g = group_or64<TV>(acc);
+                if (__all((g & need) == need)) break;
+            }
+        } else
         for (uint32_t base = e0; base < e1; base += TV * U) {
             uint32_t uu[U];
 #pragma unroll
@@ -976,29 +1050,43 @@ __device__ __forceinline__ void wide_pull_resolve(const WideParams &p, const Wav
         cur[(uint64_t)u * W + (k % W)] = 0;
         if (k % W == 0) fb_cur[u >> 5] = 0;
     }
-    const uint64_t nhw = (uint64_t)p.n_heavy_in * W;
-    const uint64_t tiles = (nhw + 63) / 64;
+    // Tiles of (word, 64 heavy vertices), lane = vertex: a source bit's
+    // distances for the tile's vertices go out in one store instruction
+    // (heavy vertices are listed in increasing order).
+    const uint32_t nh = p.n_heavy_in;
+    const uint64_t hg = (nh + 63) / 64;
+    const uint64_t tiles = hg * W;
+    const bool u16 = p.out_u16 != 0;
     for (uint64_t t = x.gwave; t < tiles; t += x.nwaves) {
-        const uint64_t k = t * 64 + x.lane;
+        const uint32_t w = (uint32_t)(t / hg);
+        const uint32_t k = (uint32_t)(t % hg) * 64 + x.lane;
         bool app = false;
         uint32_t h = 0;
-        if (k < nhw) {
-            h = p.heavy_in[k / W];
-            const uint32_t w = (uint32_t)(k % W);
+        u64 n = 0;
+        if (k < nh) {
+            h = p.heavy_in[k];
             const uint64_t idx = (uint64_t)h * W + w;
             const u64 xw = p.nx[idx];
             if (xw) {
                 p.nx[idx] = 0;
                 const u64 s = p.seen[idx];
-                const u64 n = xw & ~s & word_valid(p.S, w);
+                n = xw & ~s & word_valid(p.S, w);
                 if (n) {
                     atomicOr(&x.act_lds[w], n);
                     p.seen[idx] = s | n;
                     nxt[idx] = n;
                     my_bits += __popcll(n);
-                    if (p.out) wide_write_lane(p, w, h, n, nd);
                     app = atomicExch(&p.stamp[h], tick) != tick;
                 }
+            }
+        }
+        if (p.out) {
+            u64 un = wave_or64(n);
+            while (un) {
+                const uint32_t b = (uint32_t)__builtin_ctzll(un);
+                un &= un - 1;
+                if ((n >> b) & 1ull)
+                    store_dist(p.out, u16, ((uint64_t)w * 64 + b) * p.V + h, nd, p.flags);
             }
         }
         const uint32_t pos = wave_reserve(&c->next_count, app);
