@@ -39,7 +39,8 @@ typedef enum msbfs_status {
     MSBFS_ERR_INVALID_GRAPH = 8,
     MSBFS_ERR_UNSUPPORTED = 9,
     MSBFS_ERR_NO_DEVICE = 10,
-    MSBFS_ERR_BUSY = 11 /* serving: admission refused, retry later */
+    MSBFS_ERR_BUSY = 11, /* serving: admission refused, retry later */
+    MSBFS_ERR_PARSE = 12 /* Matrix Market text rejected; msbfs_mtx_last_error has the detail */
 } msbfs_status;
 
 /* Method selector: same names and order as GpuBfsMethod
@@ -322,6 +323,38 @@ int msbfs_csr_is_symmetric(uint64_t num_vertices, uint64_t num_edges,
 /* CsrGraph::validate (graph.rs:226-278): MSBFS_OK or MSBFS_ERR_INVALID_GRAPH. */
 int msbfs_csr_validate(uint64_t num_vertices, uint64_t num_edges, const uint32_t *row_offsets,
                        const uint32_t *col_indices);
+
+/* ---- Matrix Market reader (crates/core/src/mtx.rs:124-287, CsrGraph::from_mtx_reader) ----
+ * Coordinate files only; values ignored; self loops dropped; symmetric, skew-symmetric and
+ * hermitian files mirrored; |V| = max(rows, cols); result is the canonical CSR of
+ * CsrGraph::from_edge_list (graph.rs:51-80).  On rejected text the call returns
+ * MSBFS_ERR_PARSE, msbfs_last_error() holds the reference's message (error.rs:45-81) and
+ * msbfs_mtx_last_error() the variant and its fields. */
+typedef enum msbfs_mtx_error_kind {
+    MSBFS_MTX_OK = 0,
+    MSBFS_MTX_IO = 1,                    /* MtxError::Io */
+    MSBFS_MTX_INVALID_HEADER = 2,        /* MtxError::InvalidHeader */
+    MSBFS_MTX_UNSUPPORTED_FORMAT = 3,    /* MtxError::UnsupportedFormat ("array") */
+    MSBFS_MTX_INVALID_DIMENSIONS = 4,    /* MtxError::InvalidDimensions */
+    MSBFS_MTX_INVALID_COORDINATE_LINE = 5, /* MtxError::InvalidCoordinateLine */
+    MSBFS_MTX_ZERO_INDEX = 6,            /* MtxError::ZeroIndex */
+    MSBFS_MTX_INDEX_OUT_OF_BOUNDS = 7    /* MtxError::IndexOutOfBounds */
+} msbfs_mtx_error_kind;
+
+typedef struct msbfs_mtx_error {
+    int32_t kind;          /* msbfs_mtx_error_kind */
+    uint32_t reserved;
+    uint64_t line_number;  /* 1-based line of the file */
+    uint64_t row, col;     /* ZeroIndex / IndexOutOfBounds */
+    uint64_t max_rows, max_cols; /* IndexOutOfBounds */
+} msbfs_mtx_error;
+
+/* Parses `len` bytes of Matrix Market text (MtxReader::parse). */
+int msbfs_mtx_parse(const char *text, uint64_t len, msbfs_host_graph **out);
+/* Reads and parses a file (CsrGraph::from_mtx_reader over a BufReader). */
+int msbfs_mtx_read(const char *path, msbfs_host_graph **out);
+/* Detail of this thread's last failed msbfs_mtx_parse / msbfs_mtx_read. */
+int msbfs_mtx_last_error(msbfs_mtx_error *out);
 
 /* Generators.  Grid2D follows crates/core/src/generators.rs:28-80 exactly.
  * The random generators restate the reference's algorithms

@@ -293,3 +293,17 @@ def test_interleaved_graphs_same_size_do_not_alias(engines):
         for _ in range(2):
             assert np.array_equal(engines[m].run_ms_bfs(a, src), ea)
             assert np.array_equal(engines[m].run_ms_bfs(b, src), eb)
+
+
+def test_matrix_market_file_end_to_end(engines, tmp_path):
+    """A graph read from a Matrix Market file (the reference CLI's `.mtx`
+    path, crates/core/src/mtx.rs) runs bit-exact on both families."""
+    g = gb.generate_road_like(8_000, 90, 500, seed=3)
+    p = tmp_path / "road.mtx"
+    gb.write_mtx(g, p, symmetric=True)
+    h = gb.read_mtx(p)
+    src = gb.sample_sources(h, 130, seed=5)
+    expect = oracle(h, src)
+    for m in (M.Auto, M.WorkgroupPersistentBfs, M.PushPullAdaptive):
+        say("mtx road", m.name)
+        assert np.array_equal(engines[m].run_ms_bfs(h, src), expect), m.name

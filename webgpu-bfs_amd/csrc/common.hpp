@@ -75,6 +75,8 @@ void csr_transpose(uint64_t V, const uint32_t *ro, const uint32_t *ci, uint32_t 
                    uint32_t *ri);
 bool csr_is_symmetric(uint64_t V, uint64_t E, const uint32_t *ro, const uint32_t *ci);
 int host_threads();
+// Hands a built CSR to the C API (msbfs_host_graph_get / _free).
+msbfs_host_graph *wrap_host_graph(HostCsr &&c);
 
 }  // namespace msbfs
 

@@ -107,7 +107,7 @@ bool csr_is_symmetric(uint64_t V, uint64_t E, const uint32_t *ro, const uint32_t
 
 using namespace msbfs;
 
-static msbfs_host_graph *wrap(HostCsr &&c) {
+msbfs_host_graph *msbfs::wrap_host_graph(HostCsr &&c) {
     auto *h = new msbfs_host_graph;
     h->csr = std::move(c);
     return h;
@@ -131,6 +131,7 @@ const char *msbfs_status_string(int s) {
     case MSBFS_ERR_UNSUPPORTED: return "unsupported";
     case MSBFS_ERR_NO_DEVICE: return "no device";
     case MSBFS_ERR_BUSY: return "busy";
+    case MSBFS_ERR_PARSE: return "Matrix Market parse error";
     default: return "unknown status";
     }
 }
@@ -155,7 +156,7 @@ int msbfs_csr_from_edge_list(uint64_t V, uint64_t n, const uint32_t *us, const u
     for (uint64_t i = 0; i < n; ++i) pairs[i] = ((uint64_t)us[i] << 32) | vs[i];
     HostCsr c;
     csr_from_pairs(V, pairs, c, host_threads());
-    *out = wrap(std::move(c));
+    *out = wrap_host_graph(std::move(c));
     return MSBFS_OK;
 }
 
@@ -171,7 +172,7 @@ int msbfs_csr_from_undirected_pairs(uint64_t V, uint64_t n, const uint32_t *us,
     }
     HostCsr c;
     csr_from_pairs(V, pairs, c, host_threads());
-    *out = wrap(std::move(c));
+    *out = wrap_host_graph(std::move(c));
     return MSBFS_OK;
 }
 
@@ -232,7 +233,7 @@ int msbfs_gen_grid2d(uint64_t width, uint64_t height, msbfs_host_graph **out) {
             if (y + 1 < height) c.ci.push_back((uint32_t)(v + width));
             c.ro.push_back((uint32_t)c.ci.size());
         }
-    *out = wrap(std::move(c));
+    *out = wrap_host_graph(std::move(c));
     return MSBFS_OK;
 }
 
@@ -288,7 +289,7 @@ int msbfs_gen_road_like(uint64_t V, uint64_t width, uint64_t extra, uint64_t see
     }
     HostCsr c;
     csr_from_pairs(V, pairs, c, host_threads());
-    *out = wrap(std::move(c));
+    *out = wrap_host_graph(std::move(c));
     return MSBFS_OK;
 }
 
@@ -352,7 +353,7 @@ int msbfs_gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, dou
     raw.shrink_to_fit();
     HostCsr cs;
     csr_from_pairs(V, pairs, cs, T);
-    *out = wrap(std::move(cs));
+    *out = wrap_host_graph(std::move(cs));
     return MSBFS_OK;
 }
 
@@ -402,7 +403,7 @@ int msbfs_gen_erdos_renyi_np(uint64_t V, double p, uint64_t seed, msbfs_host_gra
     }
     HostCsr c;
     csr_from_pairs(V, pairs, c, 1);
-    *out = wrap(std::move(c));
+    *out = wrap_host_graph(std::move(c));
     return MSBFS_OK;
 }
 

@@ -24,6 +24,7 @@ ERR_INVALID_GRAPH = 8
 ERR_UNSUPPORTED = 9
 ERR_NO_DEVICE = 10
 ERR_BUSY = 11
+ERR_PARSE = 12
 
 DIST_U32 = 0
 DIST_U16 = 1
@@ -43,6 +44,29 @@ class SourceOutOfBoundsError(MsbfsError, ValueError):
 
 class NoDeviceError(MsbfsError):
     pass
+
+
+class MtxParseError(MsbfsError, ValueError):
+    """Matrix Market text rejected (crates/core/src/error.rs:45-81 MtxError).
+    `kind` names the variant; `line_number`, `row`, `col`, `max_rows`,
+    `max_cols` carry its fields where the variant has them."""
+
+    KINDS = {1: "Io", 2: "InvalidHeader", 3: "UnsupportedFormat", 4: "InvalidDimensions",
+             5: "InvalidCoordinateLine", 6: "ZeroIndex", 7: "IndexOutOfBounds"}
+
+    def __init__(self, code: int, message: str, detail):
+        super().__init__(code, message)
+        self.kind = self.KINDS.get(detail.kind, "Unknown")
+        self.line_number = detail.line_number
+        self.row, self.col = detail.row, detail.col
+        self.max_rows, self.max_cols = detail.max_rows, detail.max_cols
+
+
+class MtxErrorDetail(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("reserved", ctypes.c_uint32),
+                ("line_number", ctypes.c_uint64), ("row", ctypes.c_uint64),
+                ("col", ctypes.c_uint64), ("max_rows", ctypes.c_uint64),
+                ("max_cols", ctypes.c_uint64)]
 
 
 class DeviceInfo(ctypes.Structure):
@@ -172,6 +196,9 @@ SIGNATURES = {
                              ctypes.c_double, U64, I32, PP]),
     "msbfs_gen_erdos_renyi_np": (I32, [U64, ctypes.c_double, U64, PP]),
     "msbfs_sample_sources": (I32, [P, U64, U64, U64, I32, P]),
+    "msbfs_mtx_parse": (I32, [ctypes.c_char_p, U64, PP]),
+    "msbfs_mtx_read": (I32, [ctypes.c_char_p, PP]),
+    "msbfs_mtx_last_error": (I32, [P]),
 }
 
 _lib = None
@@ -220,4 +247,8 @@ def check(rc: int) -> None:
         raise SourceOutOfBoundsError(rc, msg)
     if rc == ERR_NO_DEVICE:
         raise NoDeviceError(rc, msg)
+    if rc == ERR_PARSE:
+        d = MtxErrorDetail()
+        lib().msbfs_mtx_last_error(ctypes.byref(d))
+        raise MtxParseError(rc, msg, d)
     raise MsbfsError(rc, msg)

@@ -7,6 +7,7 @@ work is done in libmsbfs (C++), so the large benchmark graphs build quickly.
 from __future__ import annotations
 
 import ctypes
+import os
 from dataclasses import dataclass
 
 import numpy as np
@@ -184,6 +185,42 @@ class ReachabilityBitmask:
 
 
 # ---------------------------------------------------------------- generators
+
+def parse_mtx(text) -> CsrGraph:
+    """MtxReader::parse (crates/core/src/mtx.rs:124-287) of Matrix Market text
+    (str or bytes): coordinate files only, values ignored, self loops dropped,
+    symmetric files mirrored, canonical CSR.  Raises MtxParseError."""
+    data = text.encode() if isinstance(text, str) else bytes(text)
+    h = ctypes.c_void_p()
+    L.check(L.lib().msbfs_mtx_parse(data, len(data), ctypes.byref(h)))
+    return _take_host_graph(h)
+
+
+def read_mtx(path) -> CsrGraph:
+    """CsrGraph::from_mtx_reader over a file (the CLI's `.mtx` loader)."""
+    h = ctypes.c_void_p()
+    L.check(L.lib().msbfs_mtx_read(os.fsencode(path), ctypes.byref(h)))
+    return _take_host_graph(h)
+
+
+def write_mtx(graph: CsrGraph, path, symmetric: bool = False) -> None:
+    """Writes `graph` as a coordinate pattern file (1-based).  With
+    `symmetric=True` only entries u > v are written and the banner says
+    symmetric, so a symmetric graph reads back identical."""
+    ro, ci = graph.row_offsets.astype(np.int64), graph.col_indices.astype(np.int64)
+    rows = np.repeat(np.arange(graph.num_vertices, dtype=np.int64), np.diff(ro))
+    if symmetric:
+        keep = rows > ci
+        rows, cols = rows[keep], ci[keep]
+    else:
+        cols = ci
+    with open(path, "w") as f:
+        f.write("%%MatrixMarket matrix coordinate pattern "
+                + ("symmetric" if symmetric else "general") + "\n")
+        f.write(f"{graph.num_vertices} {graph.num_vertices} {len(rows)}\n")
+        if len(rows):
+            np.savetxt(f, np.stack([rows + 1, cols + 1], 1), fmt="%d")
+
 
 def generate_grid2d(width: int, height: int) -> CsrGraph:
     """Grid2D::new(width, height).graph (crates/core/src/generators.rs:28-80)."""
