@@ -40,7 +40,8 @@ typedef enum msbfs_status {
     MSBFS_ERR_UNSUPPORTED = 9,
     MSBFS_ERR_NO_DEVICE = 10,
     MSBFS_ERR_BUSY = 11, /* serving: admission refused, retry later */
-    MSBFS_ERR_PARSE = 12 /* Matrix Market text rejected; msbfs_mtx_last_error has the detail */
+    MSBFS_ERR_PARSE = 12, /* Matrix Market text rejected; msbfs_mtx_last_error has the detail */
+    MSBFS_ERR_DIAMETER_U8 = 13 /* a distance >= 0xFF cannot be stored as u8 */
 } msbfs_status;
 
 /* Method selector: same names and order as GpuBfsMethod
@@ -64,9 +65,14 @@ typedef enum msbfs_method {
     MSBFS_METHOD_AUTO = 14
 } msbfs_method;
 
+/* Output element types.  Unreached pairs hold the all-ones sentinel of the type
+ * (u32::MAX, 0xFFFF, 0xFF), so u16 holds distances up to 65,534 and u8 up to 254; a
+ * larger distance is an error (MSBFS_ERR_DIAMETER_U16 / _U8), never a silent alias of
+ * the sentinel.  u8 is the compact wire format of pack_distances_u32_to_u8.wgsl:18-53. */
 typedef enum msbfs_dist_type {
     MSBFS_DIST_U32 = 0,
-    MSBFS_DIST_U16 = 1
+    MSBFS_DIST_U16 = 1,
+    MSBFS_DIST_U8 = 2
 } msbfs_dist_type;
 
 /* Traversal families (what a method resolves to). */
@@ -199,8 +205,13 @@ int msbfs_engine_resolve(const msbfs_engine *e, const msbfs_graph *g, uint64_t n
 /* ---- traversal ----
  * BfsEngine::run_ms_bfs / GpuBfsEngine::run_ms_bfs_into
  * (crates/core/src/lib.rs:21-39, crates/gpu_bfs/src/engine.rs:219-297).
- * `out` is host memory of out_len elements (u32 or u16 by dist_type);
- * out_len must be >= S*|V|. */
+ * `out` is host memory of out_len elements (u32, u16 or u8 by dist_type);
+ * out_len must be >= S*|V|.  The distances cross PCIe in the most compact
+ * type that holds them (the reference's u8/u16 packing and host unpack,
+ * indirect_sparse_push.rs:1580-1967, 2922-3017): computed on the device as u16
+ * (u32 only if a distance reaches 65,535), packed to u8 on the device when
+ * every distance is below 255, and widened into `out` by host threads while
+ * later chunks are still being copied. */
 int msbfs_run_ms_bfs(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources,
                      uint64_t num_sources, int dist_type, void *out, uint64_t out_len);
 /* Same, with the output left in device memory `out_device` and work

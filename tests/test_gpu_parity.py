@@ -307,3 +307,54 @@ def test_matrix_market_file_end_to_end(engines, tmp_path):
     for m in (M.Auto, M.WorkgroupPersistentBfs, M.PushPullAdaptive):
         say("mtx road", m.name)
         assert np.array_equal(engines[m].run_ms_bfs(h, src), expect), m.name
+
+
+def to_u8_expect(d32):
+    return np.where(d32 == INF32, 0xFF, d32).astype(np.uint8)
+
+
+def test_u8_output_and_boundaries(engines):
+    """u8 distances (pack_distances_u32_to_u8.wgsl:18-53: u8 iff every level
+    < 255): paths around the boundary, and graphs with unreachable pairs on
+    every family."""
+    for n in kats()["quantisation_boundary"]["path_lengths"]:
+        g = gb.csr_from_undirected_pairs(n, [(i, i + 1) for i in range(n - 1)])
+        expect = oracle(g, [0, n - 1])
+        for m in (M.Auto, M.PushPullAdaptive):
+            say("u8 path", n, m.name)
+            if n - 1 <= 254:
+                got = engines[m].run_ms_bfs(g, [0, n - 1], dist_dtype=np.uint8)
+                assert np.array_equal(got, to_u8_expect(expect)), (n, m.name)
+            else:
+                with pytest.raises(gb.MsbfsError) as ei:
+                    engines[m].run_ms_bfs(g, [0, n - 1], dist_dtype=np.uint8)
+                assert ei.value.code == 13
+    g = gb.generate_erdos_renyi_np(3000, 1.5 / 2999, seed=9)  # many components
+    src = gb.sample_sources(g, 150, seed=3)
+    expect = to_u8_expect(oracle(g, src))
+    for m in (M.Auto, M.WorkgroupPersistentBfs, M.PushPullAdaptive):
+        say("u8 ER3000", m.name)
+        assert np.array_equal(engines[m].run_ms_bfs(g, src, dist_dtype=np.uint8), expect), m.name
+
+
+def test_compact_host_transfer(engines, ctx):
+    """Host outputs cross PCIe as u8 when every distance is below 255 (here
+    in two 32 MiB chunks), as u16 otherwise, and as u32 after a rerun when a
+    distance reaches 65,535; every path widens to the caller's type exactly."""
+    import torch
+    g = gb.generate_rmat(16, 16)
+    src = gb.sample_sources(g, 600, seed=6, non_isolated=True)
+    expect = oracle(g, src)
+    e = engines[M.Auto]
+    say("compact transfer rmat16 S=600")
+    assert np.array_equal(e.run_ms_bfs(g, src), expect)
+    assert np.array_equal(e.run_ms_bfs(g, src, dist_dtype=np.uint16), to_u16_expect(expect))
+    assert np.array_equal(e.run_ms_bfs(g, src, dist_dtype=np.uint8), to_u8_expect(expect))
+    out = torch.empty(len(src) * g.num_vertices, dtype=torch.uint8, device="cuda")
+    e.run_ms_bfs_device(g, src, out.data_ptr(), out.numel(), dist_dtype=np.uint8)
+    torch.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy(), to_u8_expect(expect))
+    road = gb.generate_road_like(20_000, 160, 900, seed=4)  # eccentricities > 255: u16 wire
+    rs = gb.sample_sources(road, 300, seed=8)
+    say("compact transfer road S=300")
+    assert np.array_equal(e.run_ms_bfs(road, rs), oracle(road, rs))

@@ -144,7 +144,7 @@ DeepParams batch_params(msbfs_batch *b, void *out) {
     p.reach = nullptr;
     p.reach_wpv = 0;
     p.ntasks = (uint32_t)b->ntasks;
-    p.out_u16 = b->dtype == MSBFS_DIST_U16;
+    p.out_kind = b->dtype == MSBFS_DIST_U16 ? 1u : 0u;
     return p;
 }
 

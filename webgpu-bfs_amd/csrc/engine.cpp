@@ -8,6 +8,9 @@
 // persistent HIP kernels that iterate every level on the device.
 #include "engine_internal.hpp"
 
+#include <cstring>
+#include <thread>
+
 using namespace msbfs;
 
 #define MSBFS_VERSION "0.1.0"
@@ -410,6 +413,15 @@ int ensure_plain(DevBuf &b, size_t bytes) {
 
 namespace {
 
+// Output element kind for the kernels (0 u32, 1 u16, 2 u8) and bytes per element.
+uint32_t dist_kind(int dtype) {
+    return dtype == MSBFS_DIST_U16 ? 1u : dtype == MSBFS_DIST_U8 ? 2u : 0u;
+}
+size_t dist_bytes(int dtype) {
+    return dtype == MSBFS_DIST_U16 ? 2 : dtype == MSBFS_DIST_U8 ? 1 : 4;
+}
+
+
 Plan plan_for(const msbfs_engine *e, const msbfs_graph *g, uint64_t S) {
     Plan p;
     p.method = e->method;
@@ -539,7 +551,7 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
         p.reach = reach;
         p.reach_wpv = reach_wpv;
         p.tick_base = e->tick_base;
-        p.out_u16 = dtype == MSBFS_DIST_U16;
+        p.out_kind = dist_kind(dtype);
         p.policy = (uint32_t)plan.policy;
         p.alpha = e->tune.alpha ? e->tune.alpha : kDefaultAlpha;
         p.beta = e->tune.beta ? e->tune.beta : kDefaultBeta;
@@ -644,7 +656,7 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
         p.reach = reach;
         p.reach_wpv = reach_wpv;
         p.ntasks = (uint32_t)ntasks;
-        p.out_u16 = dtype == MSBFS_DIST_U16;
+        p.out_kind = dist_kind(dtype);
         HIP_TRY(hipEventRecord(e->ev[1], st));
         HIP_TRY(launch_deep(B, plan.ell, p, blocks, threads, plan.lds, st));
         HIP_TRY(hipEventRecord(e->ev[2], st));
@@ -686,6 +698,81 @@ int finish_run(msbfs_engine *e) {
         return fail(MSBFS_ERR_DIAMETER_U16,
                     "a distance >= 65535 cannot be stored as u16 (0xFFFF is the unreachable "
                     "sentinel); use u32 output");
+    if (f[FlagU16Overflow] && e->pending_dtype == MSBFS_DIST_U8)
+        return fail(MSBFS_ERR_DIAMETER_U8,
+                    "a distance >= 255 cannot be stored as u8 (0xFF is the unreachable "
+                    "sentinel); use u16 or u32 output");
+    return MSBFS_OK;
+}
+
+// Widens n wire elements (type `from`) into n host elements (type `to`),
+// mapping the sentinel to the sentinel, on up to `threads` host threads.
+template <typename F, typename T>
+void widen_span(const F *src, T *dst, uint64_t n, int threads) {
+    constexpr F fs = (F)~(F)0;
+    constexpr T ts = (T)~(T)0;
+    auto body = [&](uint64_t b, uint64_t e) {
+        if constexpr (sizeof(F) == sizeof(T)) {
+            std::memcpy(dst + b, src + b, (e - b) * sizeof(T));
+        } else {
+            for (uint64_t i = b; i < e; ++i) dst[i] = src[i] == fs ? ts : (T)src[i];
+        }
+    };
+    if (threads <= 1 || n < (1u << 20)) {
+        body(0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    const uint64_t per = (n + threads - 1) / threads;
+    for (int t = 0; t < threads; ++t) {
+        const uint64_t b = t * per, e = std::min(n, b + per);
+        if (b < e) th.emplace_back(body, b, e);
+    }
+    for (auto &x : th) x.join();
+}
+
+void widen_any(const void *src, int from, void *dst, int to, uint64_t n, int threads) {
+    if (from == MSBFS_DIST_U8) {
+        if (to == MSBFS_DIST_U8) widen_span((const uint8_t *)src, (uint8_t *)dst, n, threads);
+        else if (to == MSBFS_DIST_U16) widen_span((const uint8_t *)src, (uint16_t *)dst, n, threads);
+        else widen_span((const uint8_t *)src, (uint32_t *)dst, n, threads);
+    } else if (from == MSBFS_DIST_U16) {
+        if (to == MSBFS_DIST_U16) widen_span((const uint16_t *)src, (uint16_t *)dst, n, threads);
+        else widen_span((const uint16_t *)src, (uint32_t *)dst, n, threads);
+    } else {
+        widen_span((const uint32_t *)src, (uint32_t *)dst, n, threads);
+    }
+}
+
+// Device -> caller's host buffer through three pinned staging chunks: the
+// copy of chunk k+1.. runs on `st` while host threads widen chunk k.
+int transfer_widen(msbfs_engine *e, const void *src, int from, void *dst, int to, uint64_t n,
+                   hipStream_t st) {
+    constexpr int NS = 3;
+    const size_t fb = dist_bytes(from), tb = dist_bytes(to);
+    const uint64_t chunk = (32ull << 20) / fb;  // elements per chunk: 32 MiB of wire data
+    int rc = e->h_stage.ensure(NS * chunk * fb);
+    if (rc < 0) return -rc;
+    for (auto &ev : e->stage_ev)
+        if (!ev) HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    const uint64_t nch = (n + chunk - 1) / chunk;
+    const int threads = std::min(16, host_threads());
+    auto issue = [&](uint64_t k) -> int {
+        const uint64_t off = k * chunk, cnt = std::min(chunk, n - off);
+        HIP_TRY(hipMemcpyAsync((char *)e->h_stage.p + (k % NS) * chunk * fb,
+                               (const char *)src + off * fb, cnt * fb, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipEventRecord(e->stage_ev[k % NS], st));
+        return MSBFS_OK;
+    };
+    for (uint64_t k = 0; k < std::min<uint64_t>(NS, nch); ++k)
+        if ((rc = issue(k))) return rc;
+    for (uint64_t k = 0; k < nch; ++k) {
+        HIP_TRY(hipEventSynchronize(e->stage_ev[k % NS]));
+        const uint64_t off = k * chunk, cnt = std::min(chunk, n - off);
+        widen_any((const char *)e->h_stage.p + (k % NS) * chunk * fb, from,
+                  (char *)dst + off * tb, to, cnt, threads);
+        if (k + NS < nch && (rc = issue(k + NS))) return rc;
+    }
     return MSBFS_OK;
 }
 
@@ -725,7 +812,7 @@ extern "C" int msbfs_run_ms_bfs_device(msbfs_engine *e, const msbfs_graph *g,
                                        int sync) {
     clear_error();
     if (!e || !g) return fail(MSBFS_ERR_INVALID_ARG, "null engine or graph");
-    if (dtype != MSBFS_DIST_U32 && dtype != MSBFS_DIST_U16)
+    if (dtype != MSBFS_DIST_U32 && dtype != MSBFS_DIST_U16 && dtype != MSBFS_DIST_U8)
         return fail(MSBFS_ERR_INVALID_ARG, "unknown dist type %d", dtype);
     if (S == 0) return MSBFS_OK;
     if (!sources) return fail(MSBFS_ERR_INVALID_ARG, "null sources");
@@ -790,7 +877,7 @@ extern "C" int msbfs_run_ms_bfs(msbfs_engine *e, const msbfs_graph *g, const uin
                                 uint64_t S, int dtype, void *out, uint64_t out_len) {
     clear_error();
     if (!e || !g) return fail(MSBFS_ERR_INVALID_ARG, "null engine or graph");
-    if (dtype != MSBFS_DIST_U32 && dtype != MSBFS_DIST_U16)
+    if (dtype != MSBFS_DIST_U32 && dtype != MSBFS_DIST_U16 && dtype != MSBFS_DIST_U8)
         return fail(MSBFS_ERR_INVALID_ARG, "unknown dist type %d", dtype);
     if (S == 0) return MSBFS_OK;
     if (!sources) return fail(MSBFS_ERR_INVALID_ARG, "null sources");
@@ -803,13 +890,27 @@ extern "C" int msbfs_run_ms_bfs(msbfs_engine *e, const msbfs_graph *g, const uin
                     (unsigned long long)out_len, (unsigned long long)n);
     if (!out) return fail(MSBFS_ERR_INVALID_ARG, "null output");
     HIP_TRY(hipSetDevice(e->ctx->device));
-    const size_t esz = dtype == MSBFS_DIST_U16 ? 2 : 4;
-    if ((rc = ensure_plain(e->out_tmp, n * esz))) return rc;
     hipStream_t st = e->ctx->stream;
-    rc = run_common(e, g, sources, S, dtype, e->out_tmp.p, nullptr, 0, st, true);
+    // Device run in the most compact type that surely holds the result.
+    const int dev = dtype == MSBFS_DIST_U8 ? MSBFS_DIST_U8 : MSBFS_DIST_U16;
+    if ((rc = ensure_plain(e->out_tmp, n * dist_bytes(dev)))) return rc;
+    rc = run_common(e, g, sources, S, dev, e->out_tmp.p, nullptr, 0, st, true);
+    if (rc == MSBFS_ERR_DIAMETER_U16 && dtype == MSBFS_DIST_U32) {
+        // a distance of 65,535 or more: the run is repeated with u32 on the device
+        clear_error();
+        if ((rc = ensure_plain(e->out_tmp, n * 4))) return rc;
+        rc = run_common(e, g, sources, S, MSBFS_DIST_U32, e->out_tmp.p, nullptr, 0, st, true);
+        if (rc) return rc;
+        return transfer_widen(e, e->out_tmp.p, MSBFS_DIST_U32, out, dtype, n, st);
+    }
     if (rc) return rc;
-    HIP_TRY(hipMemcpy(out, e->out_tmp.p, n * esz, hipMemcpyDeviceToHost));
-    return MSBFS_OK;
+    if (dev == MSBFS_DIST_U16 && e->last.levels <= 255) {
+        // every distance is below 255: u8 on the wire
+        if ((rc = ensure_plain(e->out_pack, n))) return rc;
+        HIP_TRY(launch_pack_u16_u8(e->out_tmp.as<uint16_t>(), e->out_pack.as<uint8_t>(), n, st));
+        return transfer_widen(e, e->out_pack.p, MSBFS_DIST_U8, out, dtype, n, st);
+    }
+    return transfer_widen(e, e->out_tmp.p, dev, out, dtype, n, st);
 }
 
 extern "C" int msbfs_run_ms_reachability(msbfs_engine *e, const msbfs_graph *g,

@@ -142,7 +142,10 @@ struct msbfs_engine {
     // deep workspace
     msbfs::DevBuf queues, tasks, graphs, work;
     // shared
-    msbfs::DevBuf flags, sources, out_tmp, reach_tmp, trace;
+    msbfs::DevBuf flags, sources, out_tmp, out_pack, reach_tmp, trace;
+    // host-output path: pinned staging chunks and their copy-done events
+    msbfs::PinnedBuf h_stage;
+    hipEvent_t stage_ev[3] = {nullptr, nullptr, nullptr};
     uint32_t trace_cap = 0;
     bool trace_valid = false;
     msbfs::PinnedBuf h_sources, h_flags, h_tasks;

@@ -33,6 +33,7 @@
 
 #include "kernels.hpp"
 
+#include <algorithm>
 #include <type_traits>
 
 namespace cg = cooperative_groups;
@@ -87,39 +88,45 @@ __device__ __forceinline__ u64 wave_sum64(u64 x) {
 // for the wide family's row-coalesced stores, which fill whole lines and are
 // never read back.  Scattered single-entry stores stay cached so that L2 can
 // merge the partial lines they write.
+// Output element kinds (OutKind): 0 u32, 1 u16, 2 u8.  A distance that the
+// kind cannot hold below its sentinel raises FlagU16Overflow (the flag means
+// "distance does not fit the output type") and is stored clamped.
+template <bool NT, typename T>
+__device__ __forceinline__ void store_elem(void *out, uint64_t idx, T v) {
+    if (NT)
+        __builtin_nontemporal_store(v, (T *)out + idx);
+    else
+        ((T *)out)[idx] = v;
+}
+
 template <bool NT = false>
-__device__ __forceinline__ void store_dist(void *out, bool u16, uint64_t idx, uint32_t d,
+__device__ __forceinline__ void store_dist(void *out, uint32_t kind, uint64_t idx, uint32_t d,
                                            uint32_t *flags) {
-    if (u16) {
+    if (kind == 1) {
         if (d >= 0xFFFFu) {
             atomicOr(&flags[FlagU16Overflow], 1u);
             d = 0xFFFEu;
         }
-        if (NT)
-            __builtin_nontemporal_store((uint16_t)d, (uint16_t *)out + idx);
-        else
-            ((uint16_t *)out)[idx] = (uint16_t)d;
+        store_elem<NT>(out, idx, (uint16_t)d);
+    } else if (kind == 2) {
+        if (d >= 0xFFu) {
+            atomicOr(&flags[FlagU16Overflow], 1u);
+            d = 0xFEu;
+        }
+        store_elem<NT>(out, idx, (uint8_t)d);
     } else {
-        if (NT)
-            __builtin_nontemporal_store(d, (uint32_t *)out + idx);
-        else
-            ((uint32_t *)out)[idx] = d;
+        store_elem<NT>(out, idx, d);
     }
 }
 
 template <bool NT = false>
-__device__ __forceinline__ void store_sentinel(void *out, bool u16, uint64_t idx) {
-    if (u16) {
-        if (NT)
-            __builtin_nontemporal_store((uint16_t)0xFFFFu, (uint16_t *)out + idx);
-        else
-            ((uint16_t *)out)[idx] = 0xFFFFu;
-    } else {
-        if (NT)
-            __builtin_nontemporal_store(0xFFFFFFFFu, (uint32_t *)out + idx);
-        else
-            ((uint32_t *)out)[idx] = 0xFFFFFFFFu;
-    }
+__device__ __forceinline__ void store_sentinel(void *out, uint32_t kind, uint64_t idx) {
+    if (kind == 1)
+        store_elem<NT>(out, idx, (uint16_t)0xFFFFu);
+    else if (kind == 2)
+        store_elem<NT>(out, idx, (uint8_t)0xFFu);
+    else
+        store_elem<NT>(out, idx, 0xFFFFFFFFu);
 }
 
 __global__ void zero_u32_kernel(uint32_t *p, uint64_t n) {
@@ -133,6 +140,37 @@ hipError_t launch_zero_u32(uint32_t *ptr, uint64_t n, hipStream_t st) {
     uint64_t blocks = (n + 255) / 256;
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(zero_u32_kernel, dim3((unsigned)blocks), dim3(256), 0, st, ptr, n);
+    return hipGetLastError();
+}
+
+// Eight entries per thread: one 16-byte load, one 8-byte store.
+__global__ void pack_u16_u8_kernel(const uint16_t *in, uint8_t *out, uint64_t n) {
+    const uint64_t n8 = n / 8;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n8;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint4 x = ((const uint4 *)in)[i];
+        const uint32_t w[4] = {x.x, x.y, x.z, x.w};
+        u64 r = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t lo = w[k] & 0xFFFFu, hi = w[k] >> 16;
+            r |= (u64)(lo == 0xFFFFu ? 0xFFu : lo) << (16 * k);
+            r |= (u64)(hi == 0xFFFFu ? 0xFFu : hi) << (16 * k + 8);
+        }
+        ((u64 *)out)[i] = r;
+    }
+    if (blockIdx.x == 0 && threadIdx.x < n % 8) {
+        const uint64_t j = n8 * 8 + threadIdx.x;
+        const uint32_t v = in[j];
+        out[j] = (uint8_t)(v == 0xFFFFu ? 0xFFu : v);
+    }
+}
+
+hipError_t launch_pack_u16_u8(const uint16_t *in, uint8_t *out, uint64_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    uint64_t blocks = (n / 8 + 255) / 256;
+    blocks = std::max<uint64_t>(1, std::min<uint64_t>(blocks, 8192));
+    hipLaunchKernelGGL(pack_u16_u8_kernel, dim3((unsigned)blocks), dim3(256), 0, st, in, out, n);
     return hipGetLastError();
 }
 
@@ -202,13 +240,15 @@ struct DeepQueue {
 // vertex v.
 // Rows of one task (< 2^29 entries: |V| < 2^24, <= 32 sources) are indexed
 // with 32-bit offsets from the task's first row.
-template <int B, bool U16>
+template <int B, uint32_t KIND>
 __device__ __forceinline__ void deep_write_bits(void *out_task, uint32_t V, uint32_t v,
                                                 uint32_t bits, uint32_t d) {
     while (bits) {
         const uint32_t off = (uint32_t)__builtin_ctz(bits) * V + v;
-        if constexpr (U16)
+        if constexpr (KIND == 1)
             ((__attribute__((address_space(1))) uint16_t *)out_task)[off] = (uint16_t)d;
+        else if constexpr (KIND == 2)
+            ((__attribute__((address_space(1))) uint8_t *)out_task)[off] = (uint8_t)d;
         else
             ((__attribute__((address_space(1))) uint32_t *)out_task)[off] = d;
         bits &= bits - 1;
@@ -222,8 +262,11 @@ __device__ __forceinline__ void deep_write_bits(void *out_task, uint32_t V, uint
 // the wave appends all its new entries with one LDS atomic.
 template <int B, int ELL, int DT>
 __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
-    // DT: 0 = no distances (reachability only), 1 = u16, 2 = u32.
+    // DT: 0 = no distances (reachability only), 1 = u16, 2 = u32, 3 = u8.
     constexpr bool OUT = DT != 0;
+    constexpr uint32_t KIND = DT == 1 ? 1u : DT == 3 ? 2u : 0u;  // output element kind
+    constexpr uint32_t ESZ = DT == 1 ? 2u : DT == 3 ? 1u : 4u;
+    constexpr uint32_t DMAX = DT == 1 ? 0xFFFEu : DT == 3 ? 0xFEu : 0xFFFFFFFFu;
     typedef typename DeepEntry<B>::T Entry;
     constexpr uint32_t VPW = 32 / B;  // vertices per LDS word
     constexpr uint32_t FULL = (B == 32) ? 0xFFFFFFFFu : ((1u << B) - 1u);
@@ -241,7 +284,6 @@ __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
     const uint64_t gcap = p.cap - p.lds_qcap;  // spill entries per queue
     Entry *const gq0 = (Entry *)p.queues + (uint64_t)blockIdx.x * 2 * gcap;
     Entry *const gq1 = gq0 + gcap;
-    constexpr bool u16 = DT == 1;
 
     for (;;) {
         if (tid == 0) s_task = atomicAdd(p.work, 1u);
@@ -272,7 +314,7 @@ __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
         __syncthreads();
 
         void *const out_task =
-            OUT ? (void *)((char *)p.out + task.out_off * (u16 ? 2 : 4)) : nullptr;
+            OUT ? (void *)((char *)p.out + task.out_off * ESZ) : nullptr;
         // Distances are written when an entry is expanded: the entries of
         // level L are the (vertex, source bits) pairs first reached at
         // distance L.  The store of an entry's lowest bit is issued right
@@ -293,9 +335,9 @@ __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
             uint32_t *const cnt_next = &s_cnt[(level + 1) & 3];
             if (tid == 0) {
                 s_cnt[(level + 2) & 3] = 0;
-                if (u16 && level >= 0xFFFFu) atomicOr(&p.flags[FlagU16Overflow], 1u);
+                if (OUT && level > DMAX) atomicOr(&p.flags[FlagU16Overflow], 1u);
             }
-            const uint32_t dval = u16 ? min(level, 0xFFFEu) : level;
+            const uint32_t dval = min(level, DMAX);
             for (uint32_t base = 0; base < n; base += nth) {
                 const uint32_t i = base + tid;
                 uint32_t u = 0, f = 0;
@@ -338,8 +380,10 @@ __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
                             const uint32_t u_l0 = (uint32_t)__builtin_amdgcn_readlane((int)u, 0);
                             const uint32_t b_l0 = (uint32_t)__builtin_amdgcn_readlane((int)b0, 0);
                             const uint32_t off = (act ? b0 : b_l0) * V + (act ? u : u_l0);
-                            if constexpr (u16)
+                            if constexpr (KIND == 1)
                                 ((__attribute__((address_space(1))) uint16_t *)out_task)[off] = (uint16_t)dval;
+                            else if constexpr (KIND == 2)
+                                ((__attribute__((address_space(1))) uint8_t *)out_task)[off] = (uint8_t)dval;
                             else
                                 ((__attribute__((address_space(1))) uint32_t *)out_task)[off] = dval;
                         }
@@ -408,7 +452,7 @@ __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
                         // The entry's other source bits (rare: sources whose
                         // searches reach the vertex at the same level).
                         if constexpr (OUT && B > 1)
-                            if (act) deep_write_bits<B, u16>(out_task, V, u, f & (f - 1), dval);
+                            if (act) deep_write_bits<B, KIND>(out_task, V, u, f & (f - 1), dval);
                     };
                     if (base + nth <= qa.lcap)
                         pass(std::false_type{});
@@ -417,7 +461,7 @@ __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
                 } else {
                     if (i < n) {
                         unpack_entry(qa.get(i), u, f);
-                        if (OUT) deep_write_bits<B, u16>(out_task, V, u, f, dval);
+                        if (OUT) deep_write_bits<B, KIND>(out_task, V, u, f, dval);
                     }
                     uint32_t e = 0, eend = 0;
                     if (i < n) {
@@ -459,7 +503,7 @@ __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
                 const uint64_t row = task.out_off + (uint64_t)b * V;
                 for (uint32_t v = tid; v < V; v += nth)
                     if (!((vis[v / VPW] >> ((v % VPW) * B + b)) & 1u))
-                        store_sentinel(p.out, u16, row + v);
+                        store_sentinel(p.out, KIND, row + v);
             }
         }
         if (p.reach) {
@@ -481,8 +525,10 @@ static hipError_t deep_launch_t(const DeepParams &p, int blocks, int threads, si
                                 hipStream_t st) {
     if (!p.out)
         hipLaunchKernelGGL((deep_bfs_kernel<B, ELL, 0>), dim3(blocks), dim3(threads), lds, st, p);
-    else if (p.out_u16)
+    else if (p.out_kind == 1)
         hipLaunchKernelGGL((deep_bfs_kernel<B, ELL, 1>), dim3(blocks), dim3(threads), lds, st, p);
+    else if (p.out_kind == 2)
+        hipLaunchKernelGGL((deep_bfs_kernel<B, ELL, 3>), dim3(blocks), dim3(threads), lds, st, p);
     else
         hipLaunchKernelGGL((deep_bfs_kernel<B, ELL, 2>), dim3(blocks), dim3(threads), lds, st, p);
     return hipGetLastError();
@@ -503,13 +549,14 @@ static hipError_t deep_prepare_t(size_t lds) {
     hipError_t e = deep_prepare_dt<B, ELL, 0>(lds);
     if (e == hipSuccess) e = deep_prepare_dt<B, ELL, 1>(lds);
     if (e == hipSuccess) e = deep_prepare_dt<B, ELL, 2>(lds);
+    if (e == hipSuccess) e = deep_prepare_dt<B, ELL, 3>(lds);
     return e;
 }
 
 // Resident workgroups per CU: the least over the output variants.
 template <int B, int ELL>
 static hipError_t deep_occ_t(int threads, size_t lds, int *out) {
-    int o[3] = {0, 0, 0};
+    int o[4] = {0, 0, 0, 0};
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&o[0], deep_bfs_kernel<B, ELL, 0>,
                                                                 threads, lds);
     if (e == hipSuccess)
@@ -518,7 +565,10 @@ static hipError_t deep_occ_t(int threads, size_t lds, int *out) {
     if (e == hipSuccess)
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&o[2], deep_bfs_kernel<B, ELL, 2>,
                                                          threads, lds);
-    *out = std::min(o[0], std::min(o[1], o[2]));
+    if (e == hipSuccess)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&o[3], deep_bfs_kernel<B, ELL, 3>,
+                                                         threads, lds);
+    *out = std::min(std::min(o[0], o[1]), std::min(o[2], o[3]));
     return e;
 }
 
@@ -594,7 +644,7 @@ __device__ __forceinline__ u64 group_or64(u64 x) {
 // vertices): one store per set bit.
 __device__ __forceinline__ void wide_write_lane(const WideParams &p, uint32_t w, uint32_t v,
                                                 u64 n, uint32_t d) {
-    const bool u16 = p.out_u16 != 0;
+    const uint32_t u16 = p.out_kind;
     while (n) {
         const uint32_t b = (uint32_t)__builtin_ctzll(n);
         n &= n - 1;
@@ -612,7 +662,7 @@ __global__ void wide_seed_kernel(WideParams p) {
     atomicOr(&p.seen[idx], bit);
     atomicOr(&p.frA[idx], bit);
     atomicOr(&p.fbA[s >> 5], 1u << (s & 31));
-    if (p.out) store_dist(p.out, p.out_u16 != 0, i * p.V + s, 0, p.flags);
+    if (p.out) store_dist(p.out, p.out_kind, i * p.V + s, 0, p.flags);
     if (i < p.W) {  // active sources: all of them at level 0
         p.act[i] = word_valid(p.S, (uint32_t)i);
         p.act[p.W + i] = 0;
@@ -853,7 +903,7 @@ __device__ __forceinline__ void wide_pull(const WideParams &p, const WaveCtx &x,
     const uint32_t NB = p.pull_nb, LW = W + 1;  // LDS row stride (padded)
     const uint32_t wib = threadIdx.x / 64, nwb = blockDim.x / 64;
     const uint64_t nblk = (V + NB - 1) / NB;
-    const bool u16 = p.out_u16 != 0;
+    const uint32_t u16 = p.out_kind;
     for (uint64_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
         const uint32_t v0 = (uint32_t)blk * NB;
         for (uint32_t st = wib; st < NB / TV; st += nwb) {
@@ -1056,7 +1106,7 @@ __device__ __forceinline__ void wide_pull_resolve(const WideParams &p, const Wav
     const uint32_t nh = p.n_heavy_in;
     const uint64_t hg = (nh + 63) / 64;
     const uint64_t tiles = hg * W;
-    const bool u16 = p.out_u16 != 0;
+    const uint32_t u16 = p.out_kind;
     for (uint64_t t = x.gwave; t < tiles; t += x.nwaves) {
         const uint32_t w = (uint32_t)(t / hg);
         const uint32_t k = (uint32_t)(t % hg) * 64 + x.lane;
@@ -1242,7 +1292,7 @@ __global__ void __launch_bounds__(kWideThreads) wide_bfs_kernel(WideParams p) {
             }
             if (p.out) {
                 u64 un = wave_or64(miss);
-                const bool u16 = p.out_u16 != 0;
+                const uint32_t u16 = p.out_kind;
                 while (un) {
                     const uint32_t b = (uint32_t)__builtin_ctzll(un);
                     un &= un - 1;

@@ -41,7 +41,7 @@ struct DeepParams {
     uint32_t *reach;       // reachability words or null
     uint32_t reach_wpv;
     uint32_t ntasks;
-    uint32_t out_u16;
+    uint32_t out_kind;      // output elements: 0 u32, 1 u16, 2 u8
     uint32_t pad;
 };
 
@@ -68,7 +68,7 @@ struct WideParams {
     uint32_t *reach;
     uint32_t reach_wpv;
     uint32_t tick_base;
-    uint32_t out_u16;
+    uint32_t out_kind;      // output elements: 0 u32, 1 u16, 2 u8
     uint32_t policy;        // msbfs_direction_policy
     uint32_t alpha, beta;
     uint32_t rule;          // 0 measured frontier size / degree sum, 1 reference estimate
@@ -118,6 +118,8 @@ hipError_t wide_max_blocks_per_cu(int tile_words, int threads, size_t lds, int *
 hipError_t launch_wide_seed(const WideParams &p, hipStream_t st);
 
 hipError_t launch_zero_u32(uint32_t *ptr, uint64_t n, hipStream_t st);
+// u16 distances (all < 255 or 0xFFFF) -> u8 with 0xFF for unreached.
+hipError_t launch_pack_u16_u8(const uint16_t *in, uint8_t *out, uint64_t n, hipStream_t st);
 
 constexpr int kWideThreads = 512;
 

@@ -7,6 +7,8 @@ every entry point raises.
 from __future__ import annotations
 
 import ctypes
+
+import numpy as np
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -25,9 +27,27 @@ ERR_UNSUPPORTED = 9
 ERR_NO_DEVICE = 10
 ERR_BUSY = 11
 ERR_PARSE = 12
+ERR_DIAMETER_U8 = 13
 
 DIST_U32 = 0
 DIST_U16 = 1
+DIST_U8 = 2
+
+
+def dist_code(dtype) -> int:
+    """numpy dtype (uint32 / uint16 / uint8) -> msbfs_dist_type."""
+    d = np.dtype(dtype)
+    if d == np.uint16:
+        return DIST_U16
+    if d == np.uint8:
+        return DIST_U8
+    if d == np.uint32:
+        return DIST_U32
+    raise ValueError(f"distances are uint32, uint16 or uint8, not {d}")
+
+
+def dist_numpy(code: int):
+    return {DIST_U32: np.uint32, DIST_U16: np.uint16, DIST_U8: np.uint8}[code]
 
 FAMILY_NONE, FAMILY_DEEP, FAMILY_WIDE = 0, 1, 2
 

@@ -245,8 +245,8 @@ class GpuBfsEngine:
         src = _src_array(sources)
         S = len(src)
         nv = graph.num_vertices
-        dt = L.DIST_U16 if np.dtype(dist_dtype) == np.uint16 else L.DIST_U32
-        out = np.empty(S * nv, dtype=np.uint16 if dt == L.DIST_U16 else np.uint32)
+        dt = L.dist_code(dist_dtype)
+        out = np.empty(S * nv, dtype=L.dist_numpy(dt))
         if S == 0:
             return out
         if nv == 0:
@@ -265,8 +265,8 @@ class GpuBfsEngine:
 
     def run_ms_bfs_into(self, graph, sources, out: np.ndarray) -> np.ndarray:
         """engine.rs:219-297: fills (and resizes if needed) a caller buffer."""
-        res = self.run_ms_bfs(graph, sources, dist_dtype=out.dtype if out.dtype == np.uint16
-                              else np.uint32)
+        res = self.run_ms_bfs(graph, sources, dist_dtype=out.dtype if out.dtype in
+                              (np.uint16, np.uint8) else np.uint32)
         if out.size != res.size:
             out.resize(res.size, refcheck=False)
         out[...] = res
@@ -278,7 +278,7 @@ class GpuBfsEngine:
         """Output left in device memory at `out_ptr` (e.g. a torch tensor's
         data_ptr()), work enqueued on `stream` (a hipStream_t as int)."""
         src = _src_array(sources)
-        dt = L.DIST_U16 if np.dtype(dist_dtype) == np.uint16 else L.DIST_U32
+        dt = L.dist_code(dist_dtype)
         if len(src) and graph.num_vertices == 0:
             raise L.SourceOutOfBoundsError(
                 L.ERR_SOURCE_OOB, f"Source vertex {int(src[0])} at index 0 exceeds graph vertex count 0")
