@@ -117,7 +117,7 @@ typedef struct msbfs_tuning {
     uint32_t alpha;          /* push->pull when m_f > |E|/alpha (push_pull.rs:22-36), default 14 */
     uint32_t beta;           /* pull->push when n_f < |V|/beta, default 24 */
     uint32_t tile_words;     /* wide family wave tile: 64/tile_words vertices x tile_words words (1,2,4,8) */
-    uint32_t words_per_lane; /* wide family: words each lane walks per parent (1..4) */
+    uint32_t words_per_lane; /* reserved: ignored by this build */
     uint32_t deep_bits;      /* deep family: sources per instance (1,2,4,8,16,32); 0 = auto */
     uint32_t deep_threads;   /* deep family workgroup size (256, 512, 1024) */
     uint32_t wide_blocks_per_cu; /* wide family persistent blocks per CU */
@@ -192,6 +192,24 @@ int msbfs_engine_last_stats(msbfs_engine *e, msbfs_run_stats *st);
  * timing is wall clock around whole calls, crates/cli/src/bench.rs:338-370). */
 int msbfs_engine_level_trace(msbfs_engine *e, msbfs_level_record *out, uint32_t max_levels,
                              uint32_t *n_levels);
+
+/* Measured routing (the topology-aware replacement of the reference's fixed
+ * Auto table, engine.rs:157-198): runs candidate configurations on this graph
+ * and these sources and keeps the fastest in the engine.  With method Auto both
+ * families compete and the engine's method becomes the winner's
+ * (WorkgroupPersistentBfs or PushPullAdaptive); an explicit method is tuned
+ * within its family.  Deep candidates: every sources-per-task width whose bits
+ * fit LDS x workgroup sizes 256/512/1024.  Wide candidates, one knob at a time
+ * from the defaults: tile width (auto, half, double), blocks per CU (1, 2),
+ * direction rule (measured, reference), alpha (7, 14, 28), beta (12, 24, 48).
+ * Each candidate runs once untimed and twice timed (kernel time).  Output goes to
+ * out_device (S*|V| elements of dist_type) or, when it is NULL, to an internal
+ * buffer; all candidates produce the same distances.  Writes the winner's
+ * tuning, method and mean kernel ms. */
+int msbfs_engine_autotune(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources,
+                          uint64_t num_sources, int dist_type, void *out_device,
+                          uint64_t out_len, msbfs_tuning *best, int *best_method,
+                          float *best_ms);
 
 /* GpuBfsEngine::select_auto_method (crates/gpu_bfs/src/engine.rs:157-198):
  * the reference's decision table, returned as an msbfs_method. */

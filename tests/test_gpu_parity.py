@@ -358,3 +358,26 @@ def test_compact_host_transfer(engines, ctx):
     rs = gb.sample_sources(road, 300, seed=8)
     say("compact transfer road S=300")
     assert np.array_equal(e.run_ms_bfs(road, rs), oracle(road, rs))
+
+
+def test_engine_autotune_keeps_results(ctx):
+    """Measured routing: whichever family and knobs win, the distances are the
+    oracle's; an explicit method stays itself."""
+    road = gb.generate_road_like(20_000, 160, 900, seed=4)
+    rmat = gb.generate_rmat(14, 16)
+    for g, S in ((road, 300), (rmat, 200)):
+        src = gb.sample_sources(g, S, seed=1, non_isolated=True)
+        expect = oracle(g, src)
+        e = gb.GpuBfsEngine(ctx, M.Auto)
+        r = e.autotune(g, src)
+        say("autotune", g.num_vertices, r["method"].name, round(r["ms"], 3), r["tuning"])
+        assert r["ms"] > 0
+        assert r["method"] in (M.WorkgroupPersistentBfs, M.PushPullAdaptive)
+        assert e.method() == r["method"]
+        assert np.array_equal(e.run_ms_bfs(g, src), expect)
+    e = gb.GpuBfsEngine(ctx, M.BitParallel64Pull)
+    src = gb.sample_sources(rmat, 64, seed=2, non_isolated=True)
+    r = e.autotune(rmat, src, dist_dtype=np.uint16)
+    assert r["method"] == M.BitParallel64Pull
+    assert np.array_equal(e.run_ms_bfs(rmat, src, dist_dtype=np.uint16),
+                          to_u16_expect(oracle(rmat, src)))

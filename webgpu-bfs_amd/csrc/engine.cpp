@@ -940,3 +940,143 @@ extern "C" int msbfs_run_ms_reachability(msbfs_engine *e, const msbfs_graph *g,
     HIP_TRY(hipMemcpy(words, e->reach_tmp.p, n * 4, hipMemcpyDeviceToHost));
     return MSBFS_OK;
 }
+
+// ---------------------------------------------------------------- autotuner
+
+namespace {
+
+// Mean kernel time of two timed runs after one untimed run, or a negative
+// value when the configuration cannot run.
+float time_config(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, uint64_t S,
+                  int dtype, void *out, hipStream_t st) {
+    float total = 0;
+    for (int it = 0; it < 3; ++it) {
+        if (run_common(e, g, sources, S, dtype, out, nullptr, 0, st, true) != MSBFS_OK)
+            return -1.0f;
+        if (it) total += e->last.kernel_ms;
+    }
+    return total / 2;
+}
+
+}  // namespace
+
+extern "C" int msbfs_engine_autotune(msbfs_engine *e, const msbfs_graph *g,
+                                     const uint32_t *sources, uint64_t S, int dtype,
+                                     void *out_device, uint64_t out_len, msbfs_tuning *best,
+                                     int *best_method, float *best_ms) {
+    clear_error();
+    if (!e || !g) return fail(MSBFS_ERR_INVALID_ARG, "null engine or graph");
+    if (dtype != MSBFS_DIST_U32 && dtype != MSBFS_DIST_U16 && dtype != MSBFS_DIST_U8)
+        return fail(MSBFS_ERR_INVALID_ARG, "unknown dist type %d", dtype);
+    if (S == 0 || !sources) return fail(MSBFS_ERR_INVALID_ARG, "autotune needs sources");
+    std::lock_guard<std::mutex> lk(e->mu);
+    int rc = check_sources(g, sources, S);
+    if (rc) return rc;
+    const uint64_t n = S * g->V;
+    HIP_TRY(hipSetDevice(e->ctx->device));
+    void *out = out_device;
+    if (out) {
+        if (out_len < n)
+            return fail(MSBFS_ERR_OUT_TOO_SMALL, "output holds %llu elements, %llu needed",
+                        (unsigned long long)out_len, (unsigned long long)n);
+    } else {
+        if ((rc = ensure_plain(e->out_tmp, n * dist_bytes(dtype)))) return rc;
+        out = e->out_tmp.p;
+    }
+    if ((rc = finish_run(e))) return rc;
+    hipStream_t st = e->ctx->stream;
+    const int method0 = e->method;
+    const msbfs_tuning base = e->tune;
+    msbfs_tuning win_t = base;
+    int win_m = method0;
+    float win = 1e30f;
+    auto consider = [&](int method, const msbfs_tuning &t) {
+        e->method = method;
+        e->tune = t;
+        const float ms = time_config(e, g, sources, S, dtype, out, st);
+        if (ms >= 0 && ms < win) {
+            win = ms;
+            win_t = t;
+            win_m = method;
+        }
+    };
+    const bool try_deep = method0 == MSBFS_METHOD_AUTO ||
+                          method0 == MSBFS_METHOD_WORKGROUP_PERSISTENT_BFS;
+    const bool try_wide = method0 != MSBFS_METHOD_WORKGROUP_PERSISTENT_BFS;
+    if (try_deep) {
+        const int dm = MSBFS_METHOD_WORKGROUP_PERSISTENT_BFS;
+        for (uint32_t bits : {1u, 2u, 4u, 8u, 16u, 32u}) {
+            msbfs_tuning t = base;
+            t.deep_bits = bits;
+            if (deep_bits_for(e->ctx, t, g->V, S) != (int)bits) continue;  // does not fit LDS
+            if (bits > 1 && (S + bits / 2 - 1) / (bits / 2) <= 1) break;   // wider than S
+            for (uint32_t threads : {256u, 512u, 1024u}) {
+                t.deep_threads = threads;
+                consider(dm, t);
+            }
+        }
+    }
+    if (try_wide) {
+        const int wm = method0 == MSBFS_METHOD_AUTO ? MSBFS_METHOD_PUSH_PULL_ADAPTIVE : method0;
+        // one knob at a time from the defaults, keeping each knob's best
+        msbfs_tuning cur = base;
+        cur.tile_words = 0;
+        cur.wide_blocks_per_cu = 0;
+        cur.direction_rule = 0;
+        const float w0 = [&] {
+            e->method = wm;
+            e->tune = cur;
+            return time_config(e, g, sources, S, dtype, out, st);
+        }();
+        float cur_ms = w0 < 0 ? 1e30f : w0;
+        if (w0 >= 0 && w0 < win) {
+            win = w0;
+            win_t = cur;
+            win_m = wm;
+        }
+        auto knob = [&](auto set, const std::vector<uint32_t> &vals) {
+            msbfs_tuning keep = cur;
+            for (uint32_t v : vals) {
+                msbfs_tuning t = cur;
+                set(t, v);
+                e->method = wm;
+                e->tune = t;
+                const float ms = time_config(e, g, sources, S, dtype, out, st);
+                if (ms >= 0 && ms < cur_ms) {
+                    cur_ms = ms;
+                    keep = t;
+                }
+                if (ms >= 0 && ms < win) {
+                    win = ms;
+                    win_t = t;
+                    win_m = wm;
+                }
+            }
+            cur = keep;
+        };
+        const uint64_t W = (S + 63) / 64;
+        uint32_t tw = 1;
+        while (tw < 64 && tw < W) tw *= 2;
+        std::vector<uint32_t> tws;
+        if (tw > 1) tws.push_back(tw / 2);
+        if (tw < 64) tws.push_back(tw * 2);
+        knob([](msbfs_tuning &t, uint32_t v) { t.tile_words = v; }, tws);
+        knob([](msbfs_tuning &t, uint32_t v) { t.wide_blocks_per_cu = v; }, {1u});
+        knob([](msbfs_tuning &t, uint32_t v) { t.direction_rule = v; }, {1u});
+        knob([](msbfs_tuning &t, uint32_t v) { t.alpha = v; }, {7u, 28u});
+        knob([](msbfs_tuning &t, uint32_t v) { t.beta = v; }, {12u, 48u});
+    }
+    e->method = win_m;
+    e->tune = win_t;
+    e->tune.trace_levels = base.trace_levels;
+    if (win >= 1e30f) {
+        e->method = method0;
+        e->tune = base;
+        return fail(MSBFS_ERR_HIP, "autotune: no candidate ran");
+    }
+    clear_error();
+    if (best) *best = e->tune;
+    if (best_method) *best_method = win_m;
+    if (best_ms) *best_ms = win;
+    return MSBFS_OK;
+}

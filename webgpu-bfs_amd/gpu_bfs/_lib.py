@@ -216,6 +216,8 @@ SIGNATURES = {
                              ctypes.c_double, U64, I32, PP]),
     "msbfs_gen_erdos_renyi_np": (I32, [U64, ctypes.c_double, U64, PP]),
     "msbfs_sample_sources": (I32, [P, U64, U64, U64, I32, P]),
+    "msbfs_engine_autotune": (I32, [P, P, P, U64, I32, P, U64, P, ctypes.POINTER(I32),
+                                    ctypes.POINTER(ctypes.c_float)]),
     "msbfs_mtx_parse": (I32, [ctypes.c_char_p, U64, PP]),
     "msbfs_mtx_read": (I32, [ctypes.c_char_p, PP]),
     "msbfs_mtx_last_error": (I32, [P]),

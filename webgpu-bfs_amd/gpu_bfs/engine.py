@@ -257,6 +257,25 @@ class GpuBfsEngine:
                                          dt, out.ctypes.data_as(ctypes.c_void_p), out.size))
         return out
 
+    def autotune(self, graph, sources, dist_dtype=np.uint32, out_ptr: int = 0,
+                 out_len: int = 0) -> dict:
+        """Measured routing (msbfs_engine_autotune): times both families' and
+        their knobs' candidates on this graph and these sources, keeps the
+        fastest in the engine.  Returns the winner's method, tuning and mean
+        kernel ms.  Output goes to `out_ptr` (device) or an internal buffer."""
+        src = _src_array(sources)
+        dg = self._device_graph(graph)
+        t = L.Tuning()
+        meth = ctypes.c_int32()
+        ms = ctypes.c_float()
+        L.check(L.lib().msbfs_engine_autotune(
+            self._h, dg.handle, src.ctypes.data_as(ctypes.c_void_p), len(src),
+            L.dist_code(dist_dtype), ctypes.c_void_p(out_ptr) if out_ptr else None, out_len,
+            ctypes.byref(t), ctypes.byref(meth), ctypes.byref(ms)))
+        self._method = GpuBfsMethod(meth.value)
+        return {"method": GpuBfsMethod(meth.value), "ms": ms.value,
+                "tuning": {k: getattr(t, k) for k, _ in L.Tuning._fields_}}
+
     def run_ms_bfs_with_method(self, method: GpuBfsMethod, graph, sources) -> np.ndarray:
         """engine.rs:300-404: runs `method` regardless of the engine's own."""
         other = GpuBfsEngine(self._ctx, method)
