@@ -505,8 +505,19 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
         const uint64_t words = W * V;
         const bool force = e->dirty;
         if ((rc = ensure_zeroed(e->seen, words * 8, force, st))) return rc;
-        if ((rc = ensure_zeroed(e->frA, words * 8, force, st))) return rc;
-        if ((rc = ensure_zeroed(e->frB, words * 8, force, st))) return rc;
+        // Frontier slots: with an output, the first `defer` distances keep
+        // their words for the final pass (up to 15: the final pass encodes a
+        // distance in a nibble with 15 for unreached; within a quarter of the
+        // free device memory and 16 GiB), the rest alternate in two slots.
+        uint32_t defer = 0;
+        if (out) {
+            size_t free_b = 0, total_b = 0;
+            HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+            const uint64_t budget = std::min<uint64_t>(free_b / 4 + e->fr.bytes / 4, 16ull << 30);
+            const uint64_t fit = budget / std::max<uint64_t>(words * 8, 1);
+            defer = fit >= 4 ? (uint32_t)std::min<uint64_t>(fit - 2, 15) : 0;
+        }
+        if ((rc = ensure_zeroed(e->fr, (defer + 2) * words * 8, force, st))) return rc;
         if ((rc = ensure_zeroed(e->nx, words * 8, force, st))) return rc;
         if ((rc = ensure_zeroed(e->fbA, (V / 32 + 1) * 4, force, st))) return rc;
         if ((rc = ensure_zeroed(e->fbB, (V / 32 + 1) * 4, force, st))) return rc;
@@ -537,8 +548,8 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
         p.co = g->co;
         p.ri = g->ri;
         p.seen = e->seen.as<unsigned long long>();
-        p.frA = e->frA.as<unsigned long long>();
-        p.frB = e->frB.as<unsigned long long>();
+        p.fr = e->fr.as<unsigned long long>();
+        p.defer = defer;
         p.nx = e->nx.as<unsigned long long>();
         p.qf0 = e->qf0.as<uint32_t>();
         p.qf1 = e->qf1.as<uint32_t>();
@@ -603,7 +614,7 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
         HIP_TRY(launch_wide(tw, p, blocks, kWideThreads, lds, st));
         HIP_TRY(hipEventRecord(e->ev[2], st));
         e->last.grid_blocks = (uint32_t)blocks;
-        e->last.workspace_bytes = e->seen.bytes * 4 + e->stamp.bytes * 4;
+        e->last.workspace_bytes = e->seen.bytes * 2 + e->fr.bytes + e->stamp.bytes * 4;
     } else {
         const int B = plan.bits;
         const uint64_t ntasks = (S + B - 1) / B;

@@ -660,9 +660,9 @@ __global__ void wide_seed_kernel(WideParams p) {
     const u64 bit = 1ull << (i % 64);
     const uint64_t idx = (uint64_t)s * p.W + w;
     atomicOr(&p.seen[idx], bit);
-    atomicOr(&p.frA[idx], bit);
+    atomicOr(&p.fr[idx], bit);  // slot of distance 0
     atomicOr(&p.fbA[s >> 5], 1u << (s & 31));
-    if (p.out) store_dist(p.out, p.out_kind, i * p.V + s, 0, p.flags);
+    if (p.out && p.defer == 0) store_dist(p.out, p.out_kind, i * p.V + s, 0, p.flags);
     if (i < p.W) {  // active sources: all of them at level 0
         p.act[i] = word_valid(p.S, (uint32_t)i);
         p.act[p.W + i] = 0;
@@ -689,6 +689,8 @@ struct WaveCtx {
     const u64 *act_cur;  // sources whose frontier is not empty at this level (W words)
     uint32_t *blk;       // LDS: pull block's new frontier vertices: [0] count, [1..3]
                          // bitmap words from the block's first, [4..68) the vertices
+    bool keep;           // this level's frontier words stay for the final pass
+    bool direct;         // this level's new distances are stored now (not deferred)
 };
 
 __device__ __forceinline__ bool in_frontier(const uint32_t *fb, uint32_t u) {
@@ -724,7 +726,7 @@ __device__ __forceinline__ void wide_push_expand(const WideParams &p, const Wave
                 const uint64_t idx = (uint64_t)u * W + w;
                 f = cur[idx];
                 if (f) {
-                    cur[idx] = 0;
+                    if (!x.keep) cur[idx] = 0;
                     e = e0;
                     eend = e1;
                 }
@@ -814,7 +816,7 @@ __device__ __forceinline__ void wide_push_filter(const WideParams &p, const Wave
                         atomicOr(&x.act_lds[w], n);
                         p.seen[idx] = s | n;
                         nxt[idx] = n;
-                        if (p.out) wide_write_lane(p, w, v, n, nd);
+                        if (x.direct) wide_write_lane(p, w, v, n, nd);
                     }
                 }
             }
@@ -837,7 +839,8 @@ __device__ __forceinline__ void wide_push_filter(const WideParams &p, const Wave
         const uint32_t u = qcur[k];
         fb_cur[u >> 5] = 0;
         if (p.ro[u + 1] - p.ro[u] > p.heavy_t)
-            for (uint32_t w = 0; w < W; ++w) cur[(uint64_t)u * W + w] = 0;
+            if (!x.keep)
+                for (uint32_t w = 0; w < W; ++w) cur[(uint64_t)u * W + w] = 0;
     }
 }
 
@@ -1017,7 +1020,7 @@ __device__ __forceinline__ void wide_pull(const WideParams &p, const WaveCtx &x,
                 if (m) atomicOr(&fb_next[(v0 >> 5) + x.lane], m);
             }
         }
-        if (p.out) {
+        if (x.direct) {
             for (uint32_t w = wib; w < W; w += nwb) {
                 const bool okl = x.lane < NB && v0 + x.lane < V;
                 const u64 n = okl ? lds_n[x.lane * LW + w] : 0ull;
@@ -1097,7 +1100,7 @@ __device__ __forceinline__ void wide_pull_resolve(const WideParams &p, const Wav
     const uint64_t nclear = (uint64_t)n_front * W;
     for (uint64_t k = gthread; k < nclear; k += nthreads) {
         const uint32_t u = qcur[k / W];
-        cur[(uint64_t)u * W + (k % W)] = 0;
+        if (!x.keep) cur[(uint64_t)u * W + (k % W)] = 0;
         if (k % W == 0) fb_cur[u >> 5] = 0;
     }
     // Tiles of (word, 64 heavy vertices), lane = vertex: a source bit's
@@ -1130,7 +1133,7 @@ __device__ __forceinline__ void wide_pull_resolve(const WideParams &p, const Wav
                 }
             }
         }
-        if (p.out) {
+        if (x.direct) {
             u64 un = wave_or64(n);
             while (un) {
                 const uint32_t b = (uint32_t)__builtin_ctzll(un);
@@ -1144,6 +1147,49 @@ __device__ __forceinline__ void wide_pull_resolve(const WideParams &p, const Wav
             qnext[pos] = h;
             mark_frontier(fb_next, h);
             my_deg += p.ro[h + 1] - p.ro[h];
+        }
+    }
+}
+
+// Bits 0..7 of x to bit positions 0, 4, ..., 28.
+__device__ __forceinline__ uint32_t spread8x4(uint32_t x) {
+    x &= 0xFFu;
+    x = (x | (x << 12)) & 0x000F000Fu;
+    x = (x | (x << 6)) & 0x03030303u;
+    x = (x | (x << 3)) & 0x11111111u;
+    return x;
+}
+
+// Final-pass row stores of one tile, T = output element: rows r = wave,
+// wave+8, ... of the tile's kFinalWords x 64 source rows; lane l writes the
+// tile's vertices 2l and 2l+1 (one pair store when both entries are stored
+// and the pair is aligned, single stores otherwise).  nib[w][k][v]: nibble
+// i of byte k = distance of source bit 8k+i (15 = unreached); put[w][v]: bits
+// to store (a pair reached at a later, directly stored level is left alone).
+template <typename T, typename T2>
+__device__ __forceinline__ void final_tile_rows(T *out, uint64_t V, uint32_t v0, uint32_t w0,
+                                                uint32_t W, uint64_t S, const uint32_t *nib,
+                                                const u64 *put, uint32_t wave, uint32_t lane) {
+    const uint32_t va = 2 * lane, vb = va + 1;
+    const bool oka = v0 + va < V, okb = v0 + vb < V;
+    for (uint32_t r = wave; r < kFinalWords * 64; r += 8) {
+        const uint32_t wl = r / 64, b = r % 64, w = w0 + wl;
+        if (w >= W || (uint64_t)w * 64 + b >= S) continue;  // uniform
+        const uint32_t k = b / 8, i = b % 8;
+        const uint32_t *nw = nib + (wl * 8 + k) * kFinalVerts;
+        const u64 *pw = put + wl * kFinalVerts;
+        const uint32_t da = __builtin_amdgcn_ubfe(nw[va], 4 * i, 4);
+        const uint32_t db = __builtin_amdgcn_ubfe(nw[vb], 4 * i, 4);
+        const bool pa = oka && ((pw[va] >> b) & 1ull), pb = okb && ((pw[vb] >> b) & 1ull);
+        const T ta = da == 15u ? (T)~(T)0 : (T)da, tb = db == 15u ? (T)~(T)0 : (T)db;
+        const uint64_t row = ((uint64_t)w * 64 + b) * V + v0;
+        T *dst = out + row + va;
+        if (pa && pb && ((uintptr_t)dst & (sizeof(T2) - 1)) == 0) {
+            T2 two = (T2)ta | ((T2)tb << (8 * sizeof(T)));
+            __builtin_nontemporal_store(two, (T2 *)dst);
+        } else {
+            if (pa) __builtin_nontemporal_store(ta, dst);
+            if (pb) __builtin_nontemporal_store(tb, dst + 1);
         }
     }
 }
@@ -1170,7 +1216,10 @@ __global__ void __launch_bounds__(kWideThreads) wide_bfs_kernel(WideParams p) {
     if (threadIdx.x < 4) s_blk[threadIdx.x] = 0;
     __syncthreads();
 
-    u64 *cur = p.frA, *nxt = p.frB;
+    const uint64_t stride = (uint64_t)V * W;
+    const uint32_t D = p.defer;
+    auto slot = [&](uint32_t d) { return p.fr + (d < D ? d : D + ((d - D) & 1u)) * stride; };
+    u64 *cur = slot(0), *nxt = slot(1);
     uint32_t *fb_cur = p.fbA, *fb_next = p.fbB;
     uint32_t *qcur = p.qf0, *qnext = p.qf1;
     uint32_t n_front = ld_dev_u32(&p.ctr[3].next_count);
@@ -1204,6 +1253,8 @@ __global__ void __launch_bounds__(kWideThreads) wide_bfs_kernel(WideParams p) {
         const uint32_t nd = L + 1;
         const uint32_t tick = p.tick_base + L + 1;
         x.act_cur = p.act + (size_t)(L % 3) * W;
+        x.keep = L < D;
+        x.direct = p.out != nullptr && nd >= D;
         u64 my_bits = 0, my_deg = 0;
         if (p.trace && first && L < p.trace_cap) {
             p.trace[4 * L] = wall_clock64();
@@ -1257,9 +1308,8 @@ __global__ void __launch_bounds__(kWideThreads) wide_bfs_kernel(WideParams p) {
         prev_bits = ld_dev_u64(&c->active_bits);
         prev_deg = ld_dev_u64(&c->deg_sum);
         if (p.trace && first && L < p.trace_cap) p.trace[4 * L + 3] = prev_bits;
-        u64 *tf = cur;
         cur = nxt;
-        nxt = tf;
+        nxt = slot(L + 2);
         uint32_t *tb = fb_cur;
         fb_cur = fb_next;
         fb_next = tb;
@@ -1275,38 +1325,84 @@ __global__ void __launch_bounds__(kWideThreads) wide_bfs_kernel(WideParams p) {
         if (p.trace && L < p.trace_cap) p.trace[4 * L] = wall_clock64();
     }
 
-    // Final sweep (lane = vertex, so sentinel stores of one source row
-    // coalesce): sentinels for unreached pairs, reachability words, and the
-    // visited words cleared so the workspace starts clean next time.
-    const uint64_t nvt = (V + 63) / 64;
-    for (uint64_t t = x.gwave; t < nvt; t += x.nwaves) {
-        const uint32_t v = (uint32_t)t * 64 + x.lane;
-        const bool ok = v < V;
-        for (uint32_t w = 0; w < W; ++w) {
-            u64 s = 0, miss = 0;
-            if (ok) {
-                const uint64_t idx = (uint64_t)v * W + w;
-                s = p.seen[idx];
-                if (s) p.seen[idx] = 0;
-                miss = ~s & word_valid(p.S, w);
+    // Final sweep (lane = vertex, so a source row's stores coalesce): the
+    // distances of the kept levels are read from their slots as bit planes
+    // (plane j = OR of the slots whose distance has bit j) and every source
+    // row of the tile is written once, with the sentinel for unreached pairs;
+    // pairs reached at a later, directly stored level are left alone.  The
+    // kept slots and the visited words are cleared for the next run, and the
+    // reachability words are written when asked.
+    const uint32_t kept = min(L, D);
+    // Work items (tile, word group) in contiguous runs per workgroup, so the
+    // word groups of one tile (sharing the vertices' cache lines) run on one CU.
+    const uint64_t ntile = (V + kFinalVerts - 1) / kFinalVerts;
+    const uint64_t ngrp = (W + kFinalWords - 1) / kFinalWords;
+    const uint64_t items = ntile * ngrp;
+    const uint64_t per = (items + gridDim.x - 1) / gridDim.x;
+    const uint64_t i0 = blockIdx.x * per, i1 = min(items, i0 + per);
+    uint32_t *const L_nib = (uint32_t *)lds_n;  // [kFinalWords][8][kFinalVerts]
+    u64 *const L_put = (u64 *)(L_nib + kFinalWords * 8 * kFinalVerts);  // [kFinalWords][kFinalVerts]
+    const uint32_t wave = threadIdx.x / 64;
+    for (uint64_t item = i0; item < i1; ++item) {
+        const uint32_t v0 = (uint32_t)(item / ngrp) * kFinalVerts;
+        const uint32_t w0 = (uint32_t)(item % ngrp) * kFinalWords;
+        // thread -> (vertex, word): kFinalWords neighbouring threads read one
+        // vertex's words of the group (contiguous)
+        for (uint32_t th = threadIdx.x; th < kFinalVerts * kFinalWords; th += blockDim.x) {
+            const uint32_t vl = th / kFinalWords, wl = th % kFinalWords;
+            const uint32_t v = v0 + vl, w = w0 + wl;
+            const bool ok = v < V && w < W;
+            const uint64_t idx = (uint64_t)v * W + w;
+            u64 sw = 0, f[16];
+            if (ok) sw = p.seen[idx];
+#pragma unroll
+            for (uint32_t d = 0; d < 16; ++d)
+                f[d] = (ok && d < kept) ? p.fr[(uint64_t)d * stride + idx] : 0ull;
+            u64 found = 0, pl0 = 0, pl1 = 0, pl2 = 0, pl3 = 0;
+#pragma unroll
+            for (uint32_t d = 0; d < 16; ++d) {
+                found |= f[d];
+                if (d & 1u) pl0 |= f[d];
+                if (d & 2u) pl1 |= f[d];
+                if (d & 4u) pl2 |= f[d];
+                if (d & 8u) pl3 |= f[d];
             }
-            if (p.out) {
-                u64 un = wave_or64(miss);
-                const uint32_t u16 = p.out_kind;
-                while (un) {
-                    const uint32_t b = (uint32_t)__builtin_ctzll(un);
-                    un &= un - 1;
-                    if ((miss >> b) & 1ull)
-                        store_sentinel<true>(p.out, u16, ((uint64_t)w * 64 + b) * V + v);
+            if (ok) {
+                if (sw) p.seen[idx] = 0;
+#pragma unroll
+                for (uint32_t d = 0; d < 16; ++d)
+                    if (f[d]) p.fr[(uint64_t)d * stride + idx] = 0;
+                if (p.reach) {
+                    const u64 r = sw & word_valid(p.S, w);
+                    const uint64_t base = (uint64_t)v * p.reach_wpv + 2 * w;
+                    p.reach[base] = (uint32_t)r;
+                    if (2 * w + 1 < p.reach_wpv) p.reach[base + 1] = (uint32_t)(r >> 32);
                 }
             }
-            if (p.reach && ok) {
-                const u64 r = s & word_valid(p.S, w);
-                const uint64_t base = (uint64_t)v * p.reach_wpv + 2 * w;
-                p.reach[base] = (uint32_t)r;
-                if (2 * w + 1 < p.reach_wpv) p.reach[base + 1] = (uint32_t)(r >> 32);
+            const u64 miss = ok ? ~sw & word_valid(p.S, w) : 0ull;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const uint32_t sh = 8 * k;
+                L_nib[(wl * 8 + k) * kFinalVerts + vl] =
+                    spread8x4((uint32_t)(pl0 >> sh)) | spread8x4((uint32_t)(pl1 >> sh)) << 1 |
+                    spread8x4((uint32_t)(pl2 >> sh)) << 2 | spread8x4((uint32_t)(pl3 >> sh)) << 3 |
+                    spread8x4((uint32_t)(miss >> sh)) * 15u;
             }
+            L_put[wl * kFinalVerts + vl] = found | miss;
         }
+        __syncthreads();
+        if (p.out) {
+            if (p.out_kind == 1)
+                final_tile_rows<uint16_t, uint32_t>((uint16_t *)p.out, V, v0, w0, W, p.S, L_nib,
+                                                    L_put, wave, x.lane);
+            else if (p.out_kind == 2)
+                final_tile_rows<uint8_t, uint16_t>((uint8_t *)p.out, V, v0, w0, W, p.S, L_nib,
+                                                   L_put, wave, x.lane);
+            else
+                final_tile_rows<uint32_t, u64>((uint32_t *)p.out, V, v0, w0, W, p.S, L_nib,
+                                               L_put, wave, x.lane);
+        }
+        __syncthreads();
     }
 }
 

@@ -59,7 +59,12 @@ struct WideParams {
     uint64_t S, E;
     const uint32_t *ro, *ci;  // CSR (push)
     const uint32_t *co, *ri;  // CSC (pull); alias CSR when symmetric
-    unsigned long long *seen, *frA, *frB, *nx;  // vertex-major [V][W]
+    unsigned long long *seen, *nx;  // vertex-major [V][W]
+    // Frontier words by distance, `defer + 2` slots of V*W words (defer <= 15): distance d < defer
+    // keeps slot d until the final pass writes the output from them; later
+    // distances alternate between slots defer and defer+1.
+    unsigned long long *fr;
+    uint32_t defer;
     uint32_t *qf0, *qf1, *qc, *stamp;           // [V]
     const uint32_t *sources;
     WideCounters *ctr;      // ring of 4
@@ -76,7 +81,7 @@ struct WideParams {
     const uint32_t *heavy_out;  // vertices with out-degree > heavy_t (CSR)
     const uint32_t *heavy_in;   // vertices with in-degree > heavy_t (CSC)
     uint32_t n_heavy_out, n_heavy_in;
-    uint32_t *fbA, *fbB;    // frontier bitmaps (1 bit per vertex), ping-pong with frA/frB
+    uint32_t *fbA, *fbB;    // frontier bitmaps (1 bit per vertex), ping-pong
     const uint4 *hchunk_out;  // (vertex, first edge, end edge, 0): heavy out-lists in chunks
     const uint4 *hchunk_in;   // same for heavy in-lists (CSC)
     uint32_t n_hchunk_out, n_hchunk_in;
@@ -126,6 +131,14 @@ constexpr int kWideThreads = 512;
 // LDS bytes of the pull block buffer: nb vertices x (W + 1) words.
 // Dynamic LDS of the wide kernel: a pull block's new words (nb rows of W+1)
 // and the workgroup's active-source words (W).
-inline size_t wide_pull_lds(uint32_t nb, uint32_t W) { return ((size_t)nb * (W + 1) + W) * 8; }
+// Final pass of the wide kernel: tiles of kFinalVerts vertices x kFinalWords
+// source words; per (vertex, word) eight nibble words (u32) and the mask of
+// source bits it stores (u64) are staged in LDS.
+constexpr uint32_t kFinalVerts = 128, kFinalWords = 4;
+constexpr size_t kFinalLds = (size_t)kFinalWords * kFinalVerts * (8 * 4 + 8);
+inline size_t wide_pull_lds(uint32_t nb, uint32_t W) {
+    const size_t pull = ((size_t)nb * (W + 1) + W) * 8;
+    return pull > kFinalLds ? pull : kFinalLds;
+}
 
 }  // namespace msbfs
