@@ -1343,53 +1343,62 @@ __global__ void __launch_bounds__(kWideThreads) wide_bfs_kernel(WideParams p) {
     uint32_t *const L_nib = (uint32_t *)lds_n;  // [kFinalWords][8][kFinalVerts]
     u64 *const L_put = (u64 *)(L_nib + kFinalWords * 8 * kFinalVerts);  // [kFinalWords][kFinalVerts]
     const uint32_t wave = threadIdx.x / 64;
+    // thread -> (vertex, word) of an item: kFinalWords neighbouring threads
+    // read one vertex's words of the group (contiguous)
+    const uint32_t vl = threadIdx.x / kFinalWords, wl = threadIdx.x % kFinalWords;
+    static_assert(kWideThreads == kFinalVerts * kFinalWords, "one (vertex, word) per thread");
+    // Software pipeline: an item's loads are issued before the previous
+    // item's row stores.
+    u64 sw = 0, f[16];
+    auto load = [&](uint64_t item) {
+        const uint32_t v = (uint32_t)(item / ngrp) * kFinalVerts + vl;
+        const uint32_t w = (uint32_t)(item % ngrp) * kFinalWords + wl;
+        const bool ok = item < i1 && v < V && w < W;
+        const uint64_t idx = (uint64_t)v * W + w;
+        sw = ok ? p.seen[idx] : 0ull;
+#pragma unroll
+        for (uint32_t d = 0; d < 16; ++d)
+            f[d] = (ok && d < kept) ? p.fr[(uint64_t)d * stride + idx] : 0ull;
+    };
+    load(i0);
     for (uint64_t item = i0; item < i1; ++item) {
         const uint32_t v0 = (uint32_t)(item / ngrp) * kFinalVerts;
         const uint32_t w0 = (uint32_t)(item % ngrp) * kFinalWords;
-        // thread -> (vertex, word): kFinalWords neighbouring threads read one
-        // vertex's words of the group (contiguous)
-        for (uint32_t th = threadIdx.x; th < kFinalVerts * kFinalWords; th += blockDim.x) {
-            const uint32_t vl = th / kFinalWords, wl = th % kFinalWords;
-            const uint32_t v = v0 + vl, w = w0 + wl;
-            const bool ok = v < V && w < W;
-            const uint64_t idx = (uint64_t)v * W + w;
-            u64 sw = 0, f[16];
-            if (ok) sw = p.seen[idx];
+        const uint32_t v = v0 + vl, w = w0 + wl;
+        const bool ok = v < V && w < W;
+        const uint64_t idx = (uint64_t)v * W + w;
+        u64 found = 0, pl0 = 0, pl1 = 0, pl2 = 0, pl3 = 0;
+#pragma unroll
+        for (uint32_t d = 0; d < 16; ++d) {
+            found |= f[d];
+            if (d & 1u) pl0 |= f[d];
+            if (d & 2u) pl1 |= f[d];
+            if (d & 4u) pl2 |= f[d];
+            if (d & 8u) pl3 |= f[d];
+        }
+        if (ok) {
+            if (sw) p.seen[idx] = 0;
 #pragma unroll
             for (uint32_t d = 0; d < 16; ++d)
-                f[d] = (ok && d < kept) ? p.fr[(uint64_t)d * stride + idx] : 0ull;
-            u64 found = 0, pl0 = 0, pl1 = 0, pl2 = 0, pl3 = 0;
-#pragma unroll
-            for (uint32_t d = 0; d < 16; ++d) {
-                found |= f[d];
-                if (d & 1u) pl0 |= f[d];
-                if (d & 2u) pl1 |= f[d];
-                if (d & 4u) pl2 |= f[d];
-                if (d & 8u) pl3 |= f[d];
+                if (f[d]) p.fr[(uint64_t)d * stride + idx] = 0;
+            if (p.reach) {
+                const u64 r = sw & word_valid(p.S, w);
+                const uint64_t base = (uint64_t)v * p.reach_wpv + 2 * w;
+                p.reach[base] = (uint32_t)r;
+                if (2 * w + 1 < p.reach_wpv) p.reach[base + 1] = (uint32_t)(r >> 32);
             }
-            if (ok) {
-                if (sw) p.seen[idx] = 0;
-#pragma unroll
-                for (uint32_t d = 0; d < 16; ++d)
-                    if (f[d]) p.fr[(uint64_t)d * stride + idx] = 0;
-                if (p.reach) {
-                    const u64 r = sw & word_valid(p.S, w);
-                    const uint64_t base = (uint64_t)v * p.reach_wpv + 2 * w;
-                    p.reach[base] = (uint32_t)r;
-                    if (2 * w + 1 < p.reach_wpv) p.reach[base + 1] = (uint32_t)(r >> 32);
-                }
-            }
-            const u64 miss = ok ? ~sw & word_valid(p.S, w) : 0ull;
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const uint32_t sh = 8 * k;
-                L_nib[(wl * 8 + k) * kFinalVerts + vl] =
-                    spread8x4((uint32_t)(pl0 >> sh)) | spread8x4((uint32_t)(pl1 >> sh)) << 1 |
-                    spread8x4((uint32_t)(pl2 >> sh)) << 2 | spread8x4((uint32_t)(pl3 >> sh)) << 3 |
-                    spread8x4((uint32_t)(miss >> sh)) * 15u;
-            }
-            L_put[wl * kFinalVerts + vl] = found | miss;
         }
+        const u64 miss = ok ? ~sw & word_valid(p.S, w) : 0ull;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const uint32_t sh = 8 * k;
+            L_nib[(wl * 8 + k) * kFinalVerts + vl] =
+                spread8x4((uint32_t)(pl0 >> sh)) | spread8x4((uint32_t)(pl1 >> sh)) << 1 |
+                spread8x4((uint32_t)(pl2 >> sh)) << 2 | spread8x4((uint32_t)(pl3 >> sh)) << 3 |
+                spread8x4((uint32_t)(miss >> sh)) * 15u;
+        }
+        L_put[wl * kFinalVerts + vl] = found | miss;
+        load(item + 1);
         __syncthreads();
         if (p.out) {
             if (p.out_kind == 1)
