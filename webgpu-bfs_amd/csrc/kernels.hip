@@ -459,33 +459,94 @@ __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
                     else
                         pass(std::true_type{});
                 } else {
-                    if (i < n) {
-                        unpack_entry(qa.get(i), u, f);
-                        if (OUT) deep_write_bits<B, KIND>(out_task, V, u, f, dval);
-                    }
-                    uint32_t e = 0, eend = 0;
-                    if (i < n) {
-                        e = g.ro[u];
-                        eend = g.ro[u + 1];
-                    }
-                    // Wave-uniform edge loop so that every lane takes part
-                    // in the queue reservation.
-                    while (__any(e < eend)) {
-                        uint32_t v = 0, newly = 0;
-                        if (e < eend) {
-                            v = g.ci[e];
-                            const uint32_t wi = v / VPW, sh = (v % VPW) * B;
-                            const uint32_t m = f & ~((vis[wi] >> sh) & FULL);
-                            if (m) {
-                                const uint32_t old = atomicOr(&vis[wi], m << sh);
-                                newly = m & ~((old >> sh) & FULL);
-                            }
-                            ++e;
+                    if (base + (tid & ~63u) >= n) continue;  // wave-uniform: no entries
+                    // CSR neighbour lists: four edges per lane per step, the
+                    // indices clamped to the list (a repeated neighbour claims
+                    // nothing new), all claims issued before one wait, one
+                    // queue reservation per step, branch-free LDS appends.
+                    auto pass = [&](auto spill) {
+                        typedef const __attribute__((address_space(1))) uint32_t *g32;
+                        const bool act = i < n;
+                        if (act) {
+                            Entry x;
+                            if constexpr (decltype(spill)::value)
+                                x = qa.get(i);
+                            else
+                                x = ((__attribute__((address_space(3))) Entry *)qa.lds)[i];
+                            unpack_entry(x, u, f);
+                            if (OUT) deep_write_bits<B, KIND>(out_task, V, u, f, dval);
                         }
-                        const uint32_t pos = wave_reserve(cnt_next, newly != 0);
-                        if (newly && !qb.put(pos, pack_entry(v, newly, (Entry *)nullptr)))
-                            atomicOr(&p.flags[FlagQueueOverflow], 1u);
-                    }
+                        uint32_t e = 0, eend = 0;
+                        if (act) {
+                            e = ((g32)g.ro)[u];
+                            eend = ((g32)g.ro)[u + 1];
+                        }
+                        __attribute__((address_space(3))) uint32_t *const lv =
+                            (__attribute__((address_space(3))) uint32_t *)vis;
+                        constexpr int U = 4;
+                        while (__any(e < eend)) {
+                            uint32_t nb[U], newly[U];
+                            uint32_t c = 0;
+#pragma unroll
+                            for (int k = 0; k < U; ++k) newly[k] = 0;
+                            if (e < eend) {
+#pragma unroll
+                                for (int k = 0; k < U; ++k)
+                                    nb[k] = ((g32)g.ci)[min(e + (uint32_t)k, eend - 1)];
+                                uint32_t old[U];
+#pragma unroll
+                                for (int k = 0; k < U; ++k)
+                                    old[k] = __hip_atomic_fetch_or(lv + nb[k] / VPW,
+                                                                   f << ((nb[k] % VPW) * B),
+                                                                   __ATOMIC_RELAXED,
+                                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+#pragma unroll
+                                for (int k = 0; k < U; ++k) {
+                                    newly[k] = f & ~(old[k] >> ((nb[k] % VPW) * B));
+                                    c += newly[k] != 0;
+                                }
+                            }
+                            uint32_t pre = 0, tot = 0;
+#pragma unroll
+                            for (int j = 0; j < U; ++j) {
+                                const u64 bj = __ballot(c > (uint32_t)j);
+                                pre += (uint32_t)__popcll(bj & lt_mask);
+                                tot += (uint32_t)__popcll(bj);
+                            }
+                            if (tot) {
+                                uint32_t wbase = 0;
+                                if (lane == 0) wbase = atomicAdd(cnt_next, tot);
+                                wbase = (uint32_t)__builtin_amdgcn_readfirstlane((int)wbase);
+                                uint32_t pos = wbase + pre;
+                                if (wbase + tot <= qb.lcap) {
+                                    __attribute__((address_space(3))) Entry *const q =
+                                        (__attribute__((address_space(3))) Entry *)qb.lds;
+                                    __attribute__((address_space(3))) Entry *const sk =
+                                        (__attribute__((address_space(3))) Entry *)s_sink + lane;
+#pragma unroll
+                                    for (int k = 0; k < U; ++k) {
+                                        *(newly[k] ? q + pos : sk) =
+                                            pack_entry(newly[k] ? nb[k] : 0u, newly[k], (Entry *)nullptr);
+                                        pos += newly[k] != 0;
+                                    }
+                                } else {
+#pragma unroll
+                                    for (int k = 0; k < U; ++k) {
+                                        if (newly[k]) {
+                                            if (!qb.put(pos, pack_entry(nb[k], newly[k], (Entry *)nullptr)))
+                                                atomicOr(&p.flags[FlagQueueOverflow], 1u);
+                                            ++pos;
+                                        }
+                                    }
+                                }
+                            }
+                            e += U;
+                        }
+                    };
+                    if (base + nth <= qa.lcap)
+                        pass(std::false_type{});
+                    else
+                        pass(std::true_type{});
                 }
             }
             DeepQueue<Entry> tq = qa;
