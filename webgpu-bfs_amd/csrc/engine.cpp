@@ -531,7 +531,9 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
             HIP_TRY(hipMemsetAsync(e->stamp.p, 0, e->stamp.bytes, st));
             e->tick_base = 1;
         }
-        if ((rc = ensure_plain(e->ctr, 4 * sizeof(WideCounters)))) return rc;
+        // level counters (4 x 32 B), then the grid barrier words from byte 256
+        static_assert(4 * sizeof(WideCounters) <= 256, "level counters overlap the barrier");
+        if ((rc = ensure_plain(e->ctr, 256 + kGridBarrierWords * 4))) return rc;
         // zero row (one word per source word), read in place of frontier
         // words of parents outside the frontier; never written
         if ((rc = ensure_zeroed(e->zrow, std::max<uint64_t>(W, 64) * 8, false, st))) return rc;
@@ -593,6 +595,7 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
         p.pull_nb = nb;
         p.zrow = e->zrow.as<unsigned long long>();
         p.act = e->act.as<unsigned long long>();
+        p.gbar = (uint32_t *)((char *)e->ctr.p + 256);
         const size_t lds = wide_pull_lds(nb, (uint32_t)W);
         if (e->tune.trace_levels) {
             e->trace_cap = e->tune.trace_levels;
@@ -608,7 +611,7 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
         per_cu = std::min(per_cu, occ);
         const int blocks = c->cus * per_cu;
 
-        HIP_TRY(launch_zero_u32((uint32_t *)e->ctr.p, 4 * sizeof(WideCounters) / 4, st));
+        HIP_TRY(launch_zero_u32((uint32_t *)e->ctr.p, (256 + kGridBarrierWords * 4) / 4, st));
         HIP_TRY(launch_wide_seed(p, st));
         HIP_TRY(hipEventRecord(e->ev[1], st));
         HIP_TRY(launch_wide(tw, p, blocks, kWideThreads, lds, st));
@@ -704,6 +707,10 @@ int finish_run(msbfs_engine *e) {
     if (f[FlagQueueOverflow]) {
         e->dirty = true;
         return fail(MSBFS_ERR_HIP, "internal frontier queue overflow");
+    }
+    if (f[FlagAbort]) {
+        e->dirty = true;
+        return fail(MSBFS_ERR_HIP, "grid barrier wait timed out: the traversal was abandoned");
     }
     if (f[FlagU16Overflow] && e->pending_dtype == MSBFS_DIST_U16)
         return fail(MSBFS_ERR_DIAMETER_U16,

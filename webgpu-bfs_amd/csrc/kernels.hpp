@@ -88,6 +88,7 @@ struct WideParams {
     uint32_t pull_nb;       // vertices per workgroup block in pull levels
     const unsigned long long *zrow;  // >= W zero words: gather target of parents not in the frontier
     unsigned long long *act;  // ring of 3 x W words: sources whose frontier is not empty
+    uint32_t *gbar;         // grid barrier words (kGridBarrierWords, zero at launch)
     uint32_t trace_cap;     // levels the trace buffer holds
     unsigned long long *trace;  // per level: start clock, dir<<32|frontier, mid clock, new bits
 };
@@ -98,6 +99,10 @@ constexpr uint32_t kHeavyDegree = 128;
 // Edges per work item when a heavy vertex's list is split across waves.
 constexpr uint32_t kHeavyChunk = 2048;
 
+// Grid barrier of the wide kernel: top counter, 8 group counters, the
+// generation and the abort flag, each on its own 128-byte line.
+constexpr uint32_t kGridBarrierWords = 32 * 11;
+
 // flags[] layout (device, u32)
 enum : int {
     FlagU16Overflow = 0,
@@ -105,6 +110,7 @@ enum : int {
     FlagLevels = 2,
     FlagPushLevels = 3,
     FlagPullLevels = 4,
+    FlagAbort = 5,          // a grid barrier wait timed out
     FlagCount = 8
 };
 
