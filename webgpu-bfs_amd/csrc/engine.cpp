@@ -341,10 +341,10 @@ int deep_bits_for(const msbfs_context *c, const msbfs_tuning &t, uint64_t V, uin
     if (t.deep_bits) return fits((int)t.deep_bits) ? (int)t.deep_bits : 0;
     // Several tasks per CU hide each other's per-level latency, so the
     // width is the largest that still gives every CU k tasks at once, for
-    // k = 4, then 2, with k sets of visited bits and queues fitting its LDS
-    // (road-like |V| = 114,599, S = 2048, ms: k=4 3.73, k=2 3.91, k=1 4.22).
+    // k = 8, 4, then 2, with k sets of visited bits and queues fitting its LDS
+    // (road-like |V| = 114,599, S = 2048, ms: k=8 3.46, k=4 3.55, k=2 3.91).
     const uint64_t cus = (uint64_t)std::max(1, c->cus);
-    for (uint64_t k : {4u, 2u}) {
+    for (uint64_t k : {8u, 4u, 2u}) {
         for (int b : {32, 16, 8, 4, 2, 1}) {
             const size_t need = deep_lds_bytes(V, b) + kDeepStaticLds + 2 * 512 * (b <= 8 ? 4 : 8);
             if ((S + b - 1) / b >= k * cus && k * need <= c->lds_max) return b;

@@ -993,7 +993,10 @@ __device__ __forceinline__ void wide_pull(const WideParams &p, const WaveCtx &x,
                     u64 acc = 0;
                     uint32_t eb = e0;
                     while (__any(gon)) {
-                        acc |= scan_gather<TW>(p, x, cur, fb_cur, gon ? eb : e1, e1, w, wok);
+                        // a lane whose word already has every needed bit
+                        // reads the zero row instead of parent words
+                        acc |= scan_gather<TW>(p, x, cur, fb_cur, gon ? eb : e1, e1, w,
+                                               wok && (need & ~acc) != 0);
                         eb += TW;
                         const bool sat = (acc & need) == need;
                         gon = gon && eb < e1 && (__ballot(!sat) & gmask) != 0;
@@ -1117,11 +1120,14 @@ __device__ __forceinline__ void wide_pull(const WideParams &p, const WaveCtx &x,
         const bool wok = w < W;
         u64 acc = 0;
         if constexpr (TW >= 8) {
-            // each lane group scans its own TW-edge slice of every 64 edges
+            // each lane group scans its own TW-edge slice of every 64 edges;
+            // words whose needed bits are all found stop reading parents
+            u64 got = 0;
             for (uint32_t base = e0; base < e1; base += 64) {
-                acc |= scan_gather<TW>(p, x, cur, fb_cur, base + x.vi * TW, e1, w, wok);
-                const u64 g = group_or64<TV>(acc);
-                if (__all((g & need) == need)) break;
+                acc |= scan_gather<TW>(p, x, cur, fb_cur, base + x.vi * TW, e1, w,
+                                       wok && (need & ~got) != 0);
+                got = group_or64<TV>(acc);
+                if (__all((got & need) == need)) break;
             }
         } else
         for (uint32_t base = e0; base < e1; base += TV * U) {
@@ -1219,8 +1225,9 @@ __device__ __forceinline__ uint32_t spread8x4(uint32_t x) {
     return x;
 }
 
-// Final-pass row stores of one tile, T = output element: rows r = wave,
-// wave+8, ... of the tile's kFinalWords x 64 source rows; lane l writes the
+// Final-pass row stores of one tile, T = output element: wave i (of 8)
+// writes the rows b = 8k + i of each of the tile's kFinalWords words, so the
+// distance it needs is always nibble i of nibble word k; lane l writes the
 // tile's vertices 2l and 2l+1 (one pair store when both entries are stored
 // and the pair is aligned, single stores otherwise).  nib[w][k][v]: nibble
 // i of byte k = distance of source bit 8k+i (15 = unreached); put[w][v]: bits
@@ -1229,26 +1236,32 @@ template <typename T, typename T2>
 __device__ __forceinline__ void final_tile_rows(T *out, uint64_t V, uint32_t v0, uint32_t w0,
                                                 uint32_t W, uint64_t S, const uint32_t *nib,
                                                 const u64 *put, uint32_t wave, uint32_t lane) {
-    const uint32_t va = 2 * lane, vb = va + 1;
-    const bool oka = v0 + va < V, okb = v0 + vb < V;
-    for (uint32_t r = wave; r < kFinalWords * 64; r += 8) {
-        const uint32_t wl = r / 64, b = r % 64, w = w0 + wl;
-        if (w >= W || (uint64_t)w * 64 + b >= S) continue;  // uniform
-        const uint32_t k = b / 8, i = b % 8;
-        const uint32_t *nw = nib + (wl * 8 + k) * kFinalVerts;
-        const u64 *pw = put + wl * kFinalVerts;
-        const uint32_t da = __builtin_amdgcn_ubfe(nw[va], 4 * i, 4);
-        const uint32_t db = __builtin_amdgcn_ubfe(nw[vb], 4 * i, 4);
-        const bool pa = oka && ((pw[va] >> b) & 1ull), pb = okb && ((pw[vb] >> b) & 1ull);
-        const T ta = da == 15u ? (T)~(T)0 : (T)da, tb = db == 15u ? (T)~(T)0 : (T)db;
-        const uint64_t row = ((uint64_t)w * 64 + b) * V + v0;
-        T *dst = out + row + va;
-        if (pa && pb && ((uintptr_t)dst & (sizeof(T2) - 1)) == 0) {
-            T2 two = (T2)ta | ((T2)tb << (8 * sizeof(T)));
-            __builtin_nontemporal_store(two, (T2 *)dst);
-        } else {
-            if (pa) __builtin_nontemporal_store(ta, dst);
-            if (pb) __builtin_nontemporal_store(tb, dst + 1);
+    const uint32_t va = 2 * lane;
+    const bool oka = v0 + va < V, okb = v0 + va + 1 < V;
+    for (uint32_t wl = 0; wl < kFinalWords; ++wl) {
+        const uint32_t w = w0 + wl;
+        if (w >= W) break;  // uniform
+        // both vertices' put words, read once per word (adjacent: one ds_read2)
+        const u64 *pw = put + wl * kFinalVerts + va;
+        const u64 pwa = oka ? pw[0] : 0ull, pwb = okb ? pw[1] : 0ull;
+#pragma unroll
+        for (uint32_t k = 0; k < 8; ++k) {
+            const uint32_t b = 8 * k + wave;
+            if ((uint64_t)w * 64 + b >= S) break;  // uniform, b grows with k
+            const uint32_t *nw = nib + (wl * 8 + k) * kFinalVerts + va;
+            const uint32_t da = __builtin_amdgcn_ubfe(nw[0], 4 * wave, 4);
+            const uint32_t db = __builtin_amdgcn_ubfe(nw[1], 4 * wave, 4);
+            const bool pa = (pwa >> b) & 1ull, pb = (pwb >> b) & 1ull;
+            const T ta = da == 15u ? (T)~(T)0 : (T)da, tb = db == 15u ? (T)~(T)0 : (T)db;
+            const uint64_t row = ((uint64_t)w * 64 + b) * V + v0;
+            T *dst = out + row + va;
+            if (pa && pb && ((uintptr_t)dst & (sizeof(T2) - 1)) == 0) {
+                T2 two = (T2)ta | ((T2)tb << (8 * sizeof(T)));
+                __builtin_nontemporal_store(two, (T2 *)dst);
+            } else {
+                if (pa) __builtin_nontemporal_store(ta, dst);
+                if (pb) __builtin_nontemporal_store(tb, dst + 1);
+            }
         }
     }
 }
