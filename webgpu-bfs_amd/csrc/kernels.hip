@@ -267,7 +267,6 @@ __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
     constexpr uint32_t DMAX = DT == 1 ? 0xFFFEu : DT == 3 ? 0xFEu : 0xFFFFFFFFu;
     typedef typename DeepEntry<B>::T Entry;
     constexpr uint32_t VPW = 32 / B;  // vertices per LDS word
-    constexpr uint32_t FULL = (B == 32) ? 0xFFFFFFFFu : ((1u << B) - 1u);
     extern __shared__ uint32_t lds[];
     __shared__ uint32_t s_task;
     __shared__ uint32_t s_cnt[4];
@@ -907,8 +906,10 @@ __device__ __forceinline__ void wide_push_filter(const WideParams &p, const Wave
 // groups, group vi = lanes wi*TV + vi): each lane of a group loads one
 // in-edge of [ebase, eend) (TW edges per group), tests its frontier bit, and
 // the group's ballot lists the parents in the frontier; their frontier rows
-// are then gathered four at a time, parent index broadcast from the lane that
-// loaded it.  Returns the OR of this lane's word over those parents.
+// are then gathered eight at a time (the gathers are latency-bound: on
+// R-MAT20 S=2048 eight per round beat four by 7%, twelve and sixteen spill
+// registers and lose), parent index broadcast from the lane that loaded it.
+// Returns the OR of this lane's word over those parents.
 template <int TW>
 __device__ __forceinline__ u64 scan_gather(const WideParams &p, const WaveCtx &x, const u64 *cur,
                                            const uint32_t *fb, uint32_t ebase, uint32_t eend,
@@ -929,19 +930,19 @@ __device__ __forceinline__ u64 scan_gather(const WideParams &p, const WaveCtx &x
     u64 acc = 0;
     const uint64_t W = p.W;
     while (__any(m != 0)) {
-        uint32_t j[4];
-        bool ok[4];
+        uint32_t j[8];
+        bool ok[8];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < 8; ++k) {
             ok[k] = m != 0;
             j[k] = ok[k] ? (uint32_t)__builtin_ctzll(m) : x.lane;
             m &= m - 1;
         }
-        uint32_t u[4];
+        uint32_t u[8];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) u[k] = (uint32_t)__shfl((int)uu, (int)j[k]);
+        for (int k = 0; k < 8; ++k) u[k] = (uint32_t)__shfl((int)uu, (int)j[k]);
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
+        for (int k = 0; k < 8; ++k)
             acc |= *(g64)((ok[k] && wok) ? cur + (uint64_t)u[k] * W + w : p.zrow + (w & 63));
     }
     return acc;
