@@ -521,6 +521,7 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
         if ((rc = ensure_zeroed(e->nx, words * 8, force, st))) return rc;
         if ((rc = ensure_zeroed(e->fbA, (V / 32 + 1) * 4, force, st))) return rc;
         if ((rc = ensure_zeroed(e->fbB, (V / 32 + 1) * 4, force, st))) return rc;
+        if ((rc = ensure_zeroed(e->lv, (V + 1) * 4, force, st))) return rc;
         if ((rc = ensure_plain(e->qf0, V * 4))) return rc;
         if ((rc = ensure_plain(e->qf1, V * 4))) return rc;
         if ((rc = ensure_plain(e->qc, V * 4))) return rc;
@@ -576,6 +577,7 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
         p.n_heavy_in = g->n_heavy_in;
         p.fbA = e->fbA.as<uint32_t>();
         p.fbB = e->fbB.as<uint32_t>();
+        p.lv = e->lv.as<uint32_t>();
         p.hchunk_out = (const uint4 *)g->hchunk_out;
         p.hchunk_in = (const uint4 *)g->hchunk_in;
         p.n_hchunk_out = g->n_hchunk_out;

@@ -135,7 +135,7 @@ struct msbfs_engine {
     msbfs_tuning tune{};
     std::mutex mu;
     // wide workspace (kept all-zero between runs by the kernels)
-    msbfs::DevBuf seen, fr, nx, qf0, qf1, qc, stamp, ctr, fbA, fbB, zrow, act;
+    msbfs::DevBuf seen, fr, nx, qf0, qf1, qc, stamp, ctr, fbA, fbB, lv, zrow, act;
     uint64_t wide_words = 0;  // entries allocated per state array
     uint32_t tick_base = 1;
     bool dirty = true;

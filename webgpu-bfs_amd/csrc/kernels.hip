@@ -720,6 +720,7 @@ __global__ void wide_seed_kernel(WideParams p) {
     atomicOr(&p.seen[idx], bit);
     atomicOr(&p.fr[idx], bit);  // slot of distance 0
     atomicOr(&p.fbA[s >> 5], 1u << (s & 31));
+    if (p.defer) atomicOr(&p.lv[s], 1u);  // in the frontier of level 0
     if (p.out && p.defer == 0) store_dist(p.out, p.out_kind, i * p.V + s, 0, p.flags);
     if (i < p.W) {  // active sources: all of them at level 0
         p.act[i] = word_valid(p.S, (uint32_t)i);
@@ -890,6 +891,7 @@ __device__ __forceinline__ void wide_push_filter(const WideParams &p, const Wave
         if (app) {
             qnext[pos] = v;
             mark_frontier(fb_next, v);
+            if (nd < p.defer) p.lv[v] |= 1u << nd;  // one writer per vertex and level
             my_deg += p.ro[v + 1] - p.ro[v];
         }
     }
@@ -1063,6 +1065,7 @@ __device__ __forceinline__ void wide_pull(const WideParams &p, const WaveCtx &x,
                 if ((int)x.lane == leader) lb = atomicAdd(&x.blk[0], (uint32_t)__popcll(ab));
                 lb = (uint32_t)__builtin_amdgcn_readlane((int)lb, leader);
                 if (app) {
+                    if (nd < p.defer) p.lv[v] |= 1u << nd;
                     x.blk[4 + lb + (uint32_t)__popcll(ab & ((1ull << x.lane) - 1ull))] = v;
                     atomicOr(&x.blk[1 + ((v >> 5) - (v0 >> 5))], 1u << (v & 31));
                     my_deg += p.ro[v + 1] - p.ro[v];
@@ -1212,6 +1215,7 @@ __device__ __forceinline__ void wide_pull_resolve(const WideParams &p, const Wav
         if (app) {
             qnext[pos] = h;
             mark_frontier(fb_next, h);
+            if (nd < p.defer) p.lv[h] |= 1u << nd;
             my_deg += p.ro[h + 1] - p.ro[h];
         }
     }
@@ -1467,13 +1471,14 @@ __global__ void __launch_bounds__(kWideThreads) __attribute__((amdgpu_waves_per_
     // kept slots and the visited words are cleared for the next run, and the
     // reachability words are written when asked.
     const uint32_t kept = min(L, D);
-    // Work items (tile, word group) in contiguous runs per workgroup, so the
-    // word groups of one tile (sharing the vertices' cache lines) run on one CU.
+    // Work items (tile, word group) in runs of whole tiles per workgroup, so
+    // the word groups of one tile (sharing the vertices' cache lines) run on
+    // one CU, in order: the last one clears the tile's level masks.
     const uint64_t ntile = (V + kFinalVerts - 1) / kFinalVerts;
     const uint64_t ngrp = (W + kFinalWords - 1) / kFinalWords;
     const uint64_t items = ntile * ngrp;
-    const uint64_t per = (items + gridDim.x - 1) / gridDim.x;
-    const uint64_t i0 = blockIdx.x * per, i1 = min(items, i0 + per);
+    const uint64_t per = ((ntile + gridDim.x - 1) / gridDim.x) * ngrp;
+    const uint64_t i0 = min(items, blockIdx.x * per), i1 = min(items, i0 + per);
     uint32_t *const L_nib = (uint32_t *)lds_n;  // [kFinalWords][8][kFinalVerts]
     u64 *const L_put = (u64 *)(L_nib + kFinalWords * 8 * kFinalVerts);  // [kFinalWords][kFinalVerts]
     const uint32_t wave = threadIdx.x / 64;
@@ -1483,6 +1488,7 @@ __global__ void __launch_bounds__(kWideThreads) __attribute__((amdgpu_waves_per_
     static_assert(kWideThreads == kFinalVerts * kFinalWords, "one (vertex, word) per thread");
     // Software pipeline: an item's loads are issued before the previous
     // item's row stores.
+    // Only the slots of levels whose frontier held the vertex are read.
     u64 sw = 0, f[16];
     auto load = [&](uint64_t item) {
         const uint32_t v = (uint32_t)(item / ngrp) * kFinalVerts + vl;
@@ -1490,9 +1496,10 @@ __global__ void __launch_bounds__(kWideThreads) __attribute__((amdgpu_waves_per_
         const bool ok = item < i1 && v < V && w < W;
         const uint64_t idx = (uint64_t)v * W + w;
         sw = ok ? p.seen[idx] : 0ull;
+        const uint32_t lvm = (ok && kept) ? p.lv[v] : 0u;
 #pragma unroll
         for (uint32_t d = 0; d < 16; ++d)
-            f[d] = (ok && d < kept) ? p.fr[(uint64_t)d * stride + idx] : 0ull;
+            f[d] = (d < kept && ((lvm >> d) & 1u)) ? p.fr[(uint64_t)d * stride + idx] : 0ull;
     };
     load(i0);
     for (uint64_t item = i0; item < i1; ++item) {
@@ -1512,6 +1519,9 @@ __global__ void __launch_bounds__(kWideThreads) __attribute__((amdgpu_waves_per_
         }
         if (ok) {
             if (sw) p.seen[idx] = 0;
+            // the tile's last word group clears the level mask (this thread's
+            // load of it is done; the tile's other items ran before on this block)
+            if (wl == 0 && kept && item % ngrp == ngrp - 1) p.lv[v] = 0;
 #pragma unroll
             for (uint32_t d = 0; d < 16; ++d)
                 if (f[d]) p.fr[(uint64_t)d * stride + idx] = 0;

@@ -82,6 +82,7 @@ struct WideParams {
     const uint32_t *heavy_in;   // vertices with in-degree > heavy_t (CSC)
     uint32_t n_heavy_out, n_heavy_in;
     uint32_t *fbA, *fbB;    // frontier bitmaps (1 bit per vertex), ping-pong
+    uint32_t *lv;           // per vertex: bit d set when it was in the frontier of kept level d
     const uint4 *hchunk_out;  // (vertex, first edge, end edge, 0): heavy out-lists in chunks
     const uint4 *hchunk_in;   // same for heavy in-lists (CSC)
     uint32_t n_hchunk_out, n_hchunk_in;
