@@ -1166,11 +1166,16 @@ __device__ __forceinline__ void wide_pull_resolve(const WideParams &p, const Wav
                                                   uint64_t nthreads, u64 &my_bits,
                                                   u64 &my_deg) {
     const uint32_t W = p.W;
-    const uint64_t nclear = (uint64_t)n_front * W;
-    for (uint64_t k = gthread; k < nclear; k += nthreads) {
-        const uint32_t u = qcur[k / W];
-        if (!x.keep) cur[(uint64_t)u * W + (k % W)] = 0;
-        if (k % W == 0) fb_cur[u >> 5] = 0;
+    if (x.keep) {
+        // the frontier words stay for the final pass: only the bitmap goes
+        for (uint64_t k = gthread; k < n_front; k += nthreads) fb_cur[qcur[k] >> 5] = 0;
+    } else {
+        const uint64_t nclear = (uint64_t)n_front * W;
+        for (uint64_t k = gthread; k < nclear; k += nthreads) {
+            const uint32_t u = qcur[k / W];
+            cur[(uint64_t)u * W + (k % W)] = 0;
+            if (k % W == 0) fb_cur[u >> 5] = 0;
+        }
     }
     // Tiles of (word, 64 heavy vertices), lane = vertex: a source bit's
     // distances for the tile's vertices go out in one store instruction
