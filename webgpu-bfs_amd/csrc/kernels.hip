@@ -44,21 +44,6 @@ typedef unsigned long long u64;
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
-// Reserves one slot per lane with `pred` set in the queue counted by *ctr,
-// using one atomic per wave.  Every active lane of the wave must call it.
-template <typename Ctr>
-__device__ __forceinline__ uint32_t wave_reserve(Ctr *ctr, bool pred) {
-    const u64 bal = __ballot(pred);
-    uint32_t base = 0;
-    if (bal) {
-        const int leader = __ffsll((long long)bal) - 1;
-        if ((int)lane_id() == leader) base = atomicAdd(ctr, (uint32_t)__popcll(bal));
-        base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
-    }
-    const u64 below = lane_id() ? (bal & ((~0ull) >> (64 - lane_id()))) : 0ull;
-    return base + (uint32_t)__popcll(below);
-}
-
 __device__ __forceinline__ u64 wave_or64(u64 x) {
     uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
 #pragma unroll
@@ -750,6 +735,34 @@ struct WaveCtx {
                          // bitmap words from the block's first, [4..68) the vertices
     bool keep;           // this level's frontier words stay for the final pass
     bool direct;         // this level's new distances are stored now (not deferred)
+    uint32_t *wq;        // this wave's 64 LDS slots for WaveAppender
+};
+
+// Appends to a global queue through the wave's LDS slots: one atomic on the
+// queue counter per 64 entries instead of one per call (the counters are
+// single words that every wave of the grid hits).  Every lane of the wave
+// calls push and flush; flush before the queue is read.
+struct WaveAppender {
+    uint32_t *buf;
+    uint32_t n = 0;  // entries held, wave-uniform
+    __device__ __forceinline__ void flush(uint32_t *ctr, uint32_t *queue) {
+        if (n == 0) return;
+        const int leader = __ffsll((long long)__ballot(1)) - 1;
+        uint32_t base = 0;
+        if ((int)lane_id() == leader) base = atomicAdd(ctr, n);
+        base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
+        if (lane_id() < n) queue[base + lane_id()] = buf[lane_id()];
+        n = 0;
+    }
+    __device__ __forceinline__ void push(bool pred, uint32_t v, uint32_t *ctr, uint32_t *queue) {
+        const u64 bal = __ballot(pred);
+        if (!bal) return;
+        const uint32_t c = (uint32_t)__popcll(bal);
+        if (n + c > 64) flush(ctr, queue);
+        const u64 below = lane_id() ? (bal & ((~0ull) >> (64 - lane_id()))) : 0ull;
+        if (pred) buf[n + (uint32_t)__popcll(below)] = v;
+        n += c;
+    }
 };
 
 __device__ __forceinline__ bool in_frontier(const uint32_t *fb, uint32_t u) {
@@ -773,6 +786,7 @@ __device__ __forceinline__ void wide_push_expand(const WideParams &p, const Wave
     constexpr uint32_t TV = 64 / TW;
     const uint32_t W = p.W, WG = (W + TW - 1) / TW;
     const uint64_t ntiles = (uint64_t)((n_front + TV - 1) / TV) * WG;
+    WaveAppender qa{x.wq};
     for (uint64_t t = x.gwave; t < ntiles; t += x.nwaves) {
         const uint32_t i = (uint32_t)(t / WG) * TV + x.vi;
         const uint32_t w = (uint32_t)(t % WG) * TW + x.wi;
@@ -805,8 +819,7 @@ __device__ __forceinline__ void wide_push_expand(const WideParams &p, const Wave
                     app = true;
                 ++e;
             }
-            const uint32_t pos = wave_reserve(&c->cand_count, app);
-            if (app) p.qc[pos] = v;
+            qa.push(app, v, &c->cand_count, p.qc);
         }
     }
     for (uint64_t t = x.gwave; t < p.n_hchunk_out; t += x.nwaves) {
@@ -834,12 +847,12 @@ __device__ __forceinline__ void wide_push_expand(const WideParams &p, const Wave
                             atomicExch(&p.stamp[v], tick) != tick)
                             app = true;
                     }
-                    const uint32_t pos = wave_reserve(&c->cand_count, app);
-                    if (app) p.qc[pos] = v;
+                    qa.push(app, v, &c->cand_count, p.qc);
                 }
             }
         }
     }
+    qa.flush(&c->cand_count, p.qc);
 }
 
 // Push, part 2: new bits of each candidate become its next-frontier words
@@ -856,6 +869,7 @@ __device__ __forceinline__ void wide_push_filter(const WideParams &p, const Wave
     const uint32_t W = p.W, WG = (W + TW - 1) / TW;
     const uint32_t ncand = ld_dev_u32(&c->cand_count);
     const uint64_t ftiles = (ncand + TV - 1) / TV;
+    WaveAppender qa{x.wq};
     for (uint64_t t = x.gwave; t < ftiles; t += x.nwaves) {
         const uint32_t i = (uint32_t)t * TV + x.vi;
         const bool okv = i < ncand;
@@ -887,14 +901,14 @@ __device__ __forceinline__ void wide_push_filter(const WideParams &p, const Wave
 #pragma unroll
         for (uint32_t k = 0; k < TW; ++k) vany |= ((bal >> (k * TV + x.vi)) & 1ull) != 0;
         const bool app = x.wi == 0 && okv && vany;
-        const uint32_t pos = wave_reserve(&c->next_count, app);
+        qa.push(app, v, &c->next_count, qnext);
         if (app) {
-            qnext[pos] = v;
             mark_frontier(fb_next, v);
             if (nd < p.defer) p.lv[v] |= 1u << nd;  // one writer per vertex and level
             my_deg += p.ro[v + 1] - p.ro[v];
         }
     }
+    qa.flush(&c->next_count, qnext);
     for (uint64_t k = gthread; k < n_front; k += nthreads) {
         const uint32_t u = qcur[k];
         fb_cur[u >> 5] = 0;
@@ -1184,6 +1198,7 @@ __device__ __forceinline__ void wide_pull_resolve(const WideParams &p, const Wav
     const uint64_t hg = (nh + 63) / 64;
     const uint64_t tiles = hg * W;
     const uint32_t u16 = p.out_kind;
+    WaveAppender qa{x.wq};
     for (uint64_t t = x.gwave; t < tiles; t += x.nwaves) {
         const uint32_t w = (uint32_t)(t / hg);
         const uint32_t k = (uint32_t)(t % hg) * 64 + x.lane;
@@ -1216,14 +1231,14 @@ __device__ __forceinline__ void wide_pull_resolve(const WideParams &p, const Wav
                     store_dist(p.out, u16, ((uint64_t)w * 64 + b) * p.V + h, nd, p.flags);
             }
         }
-        const uint32_t pos = wave_reserve(&c->next_count, app);
+        qa.push(app, h, &c->next_count, qnext);
         if (app) {
-            qnext[pos] = h;
             mark_frontier(fb_next, h);
             if (nd < p.defer) p.lv[h] |= 1u << nd;
             my_deg += p.ro[h + 1] - p.ro[h];
         }
     }
+    qa.flush(&c->next_count, qnext);
 }
 
 // Bits 0..7 of x to bit positions 0, 4, ..., 28.
@@ -1341,7 +1356,9 @@ __global__ void __launch_bounds__(kWideThreads) __attribute__((amdgpu_waves_per_
     x.act_lds = lds_n + (size_t)p.pull_nb * (W + 1);
     for (uint32_t w = threadIdx.x; w < W; w += blockDim.x) x.act_lds[w] = 0;
     __shared__ uint32_t s_blk[4 + 64];
+    __shared__ uint32_t s_wq[kWideThreads];
     x.blk = s_blk;
+    x.wq = s_wq + (threadIdx.x / 64) * 64;
     if (threadIdx.x < 4) s_blk[threadIdx.x] = 0;
     __syncthreads();
 
