@@ -964,14 +964,105 @@ __device__ __forceinline__ u64 scan_gather(const WideParams &p, const WaveCtx &x
     return acc;
 }
 
-// Pull: each workgroup takes blocks of NB consecutive vertices.  Every
-// (vertex, word) with unseen bits ORs the frontier words of its parents
-// (only those set in the frontier bitmap are read) until all still-needed
-// bits are found (early exit); the new bits go to LDS, and the block's
-// distances are then stored one source row at a time with lane = vertex,
-// so each store covers NB consecutive entries of a row.  Heavy in-lists
-// come in chunks, one wave per chunk; a chunk's bits are ORed into nx and
-// resolved by wide_pull_resolve after the grid barrier.
+// One light vertex of a pull level on its TW lanes (all lanes of the wave
+// call it; lanes with okv false take part in the shuffles only): every word
+// with unseen bits ORs the frontier words of its parents (only those set in
+// the frontier bitmap are read) until all still-needed bits are found (early
+// exit); new bits go to seen, the next frontier slot and, when lds_row is
+// given, to lds_row[w].  Returns whether the vertex got any new bit (the
+// same on all its lanes).
+template <int TW>
+__device__ __forceinline__ bool pull_vertex(const WideParams &p, const WaveCtx &x, const u64 *cur,
+                                            const uint32_t *fb_cur, u64 *nxt, uint32_t v,
+                                            bool okv, uint32_t e0, uint32_t e1, u64 *lds_row,
+                                            u64 &my_bits) {
+    constexpr uint32_t TV = 64 / TW;
+    const uint32_t W = p.W, WG = (W + TW - 1) / TW;
+    bool any = false;
+    for (uint32_t g = 0; g < WG; ++g) {
+        const uint32_t w = g * TW + x.wi;
+        u64 n = 0;
+        if constexpr (TW >= 8) {
+            // The vertex's TW lanes scan its in-list together.
+            const bool wok = okv && w < W;
+            const uint64_t idx = (uint64_t)v * W + w;
+            const u64 s = wok ? p.seen[idx] : ~0ull;
+            const u64 need = wok ? ~s & word_valid(p.S, w) & x.act_cur[w] : 0ull;
+            const u64 gmask = (~0ull / ((1ull << TV) - 1ull)) << x.vi;
+            bool gon = (__ballot(need != 0) & gmask) != 0 && e0 < e1;
+            u64 acc = 0;
+            uint32_t eb = e0;
+            while (__any(gon)) {
+                // a lane whose word already has every needed bit
+                // reads the zero row instead of parent words
+                acc |= scan_gather<TW>(p, x, cur, fb_cur, gon ? eb : e1, e1, w,
+                                       wok && (need & ~acc) != 0);
+                eb += TW;
+                const bool sat = (acc & need) == need;
+                gon = gon && eb < e1 && (__ballot(!sat) & gmask) != 0;
+            }
+            n = acc & need;
+            if (n) {
+                atomicOr(&x.act_lds[w], n);
+                p.seen[idx] = s | n;
+                nxt[idx] = n;
+            }
+        } else if (okv && w < W) {
+            const uint64_t idx = (uint64_t)v * W + w;
+            const u64 s = p.seen[idx];
+            const u64 need = ~s & word_valid(p.S, w) & x.act_cur[w];
+            if (need) {
+                // Parents in steps of 8 with no branches inside a
+                // step: indices past the end repeat the last parent,
+                // parents outside the frontier read the zero row, so
+                // each step is three rounds of independent loads
+                // (indices, bitmap words, frontier words).
+                typedef const __attribute__((address_space(1))) uint32_t *g32;
+                typedef const __attribute__((address_space(1))) u64 *g64;
+                constexpr int U = 8;
+                u64 acc = 0;
+                for (uint32_t e = e0; e < e1; e += U) {
+                    uint32_t uu[U];
+#pragma unroll
+                    for (int k = 0; k < U; ++k) uu[k] = ((g32)p.ri)[min(e + (uint32_t)k, e1 - 1)];
+                    uint32_t fw[U];
+#pragma unroll
+                    for (int k = 0; k < U; ++k) fw[k] = ((g32)fb_cur)[uu[k] >> 5];
+#pragma unroll
+                    for (int k = 0; k < U; ++k) {
+                        const bool inf = (fw[k] >> (uu[k] & 31)) & 1u;
+                        acc |= *(g64)(inf ? cur + (uint64_t)uu[k] * W + w : p.zrow + w);
+                    }
+                    if ((acc & need) == need) break;
+                }
+                n = acc & need;
+                if (n) {
+                    atomicOr(&x.act_lds[w], n);
+                    p.seen[idx] = s | n;
+                    nxt[idx] = n;
+                }
+            }
+        }
+        if (lds_row && w < W) lds_row[w] = n;
+        any |= n != 0;
+        my_bits += __popcll(n);
+    }
+    const u64 bal = __ballot(any);
+    bool vany = false;
+#pragma unroll
+    for (uint32_t k = 0; k < TW; ++k) vany |= ((bal >> (k * TV + x.vi)) & 1ull) != 0;
+    return vany;
+}
+
+// Pull.  Levels whose distances are deferred to the final pass: each wave
+// takes its own tiles of TV consecutive vertices and appends new frontier
+// vertices through its LDS buffer, so no wave waits for another's in-lists.
+// Levels that store distances now: each workgroup takes blocks of NB
+// consecutive vertices, the new bits go to LDS, and the block's distances
+// are then stored one source row at a time with lane = vertex, so each
+// store covers NB consecutive entries of a row.  Heavy in-lists come in
+// chunks, one wave per chunk; a chunk's bits are ORed into nx and resolved
+// by wide_pull_resolve after the grid barrier.
 template <int TW>
 __device__ __forceinline__ void wide_pull(const WideParams &p, const WaveCtx &x, const u64 *cur,
                                           const uint32_t *fb_cur, u64 *nxt, uint32_t *qnext,
@@ -983,6 +1074,30 @@ __device__ __forceinline__ void wide_pull(const WideParams &p, const WaveCtx &x,
     const uint32_t wib = threadIdx.x / 64, nwb = blockDim.x / 64;
     const uint64_t nblk = (V + NB - 1) / NB;
     const uint32_t u16 = p.out_kind;
+    if (!x.direct) {
+        const uint64_t ntv = (V + TV - 1) / TV;
+        WaveAppender qa{x.wq};
+        for (uint64_t t = x.gwave; t < ntv; t += x.nwaves) {
+            const uint32_t v = (uint32_t)t * TV + x.vi;
+            bool okv = v < V;
+            uint32_t e0 = 0, e1 = 0;
+            if (okv) {
+                e0 = p.co[v];
+                e1 = p.co[v + 1];
+                if (e1 - e0 > p.heavy_t) okv = false;  // chunked below
+            }
+            const bool vany = pull_vertex<TW>(p, x, cur, fb_cur, nxt, v, okv, e0, e1, nullptr,
+                                              my_bits);
+            const bool app = x.wi == 0 && okv && vany;
+            qa.push(app, v, &c->next_count, qnext);
+            if (app) {
+                if (nd < p.defer) p.lv[v] |= 1u << nd;
+                mark_frontier(fb_next, v);
+                my_deg += p.ro[v + 1] - p.ro[v];
+            }
+        }
+        qa.flush(&c->next_count, qnext);
+    } else
     for (uint64_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
         const uint32_t v0 = (uint32_t)blk * NB;
         for (uint32_t st = wib; st < NB / TV; st += nwb) {
@@ -995,80 +1110,8 @@ __device__ __forceinline__ void wide_pull(const WideParams &p, const WaveCtx &x,
                 e1 = p.co[v + 1];
                 if (e1 - e0 > p.heavy_t) okv = false;  // chunked below
             }
-            bool any = false;
-            for (uint32_t g = 0; g < WG; ++g) {
-                const uint32_t w = g * TW + x.wi;
-                u64 n = 0;
-                if constexpr (TW >= 8) {
-                    // The vertex's TW lanes scan its in-list together.
-                    const bool wok = okv && w < W;
-                    const uint64_t idx = (uint64_t)v * W + w;
-                    const u64 s = wok ? p.seen[idx] : ~0ull;
-                    const u64 need = wok ? ~s & word_valid(p.S, w) & x.act_cur[w] : 0ull;
-                    const u64 gmask = (~0ull / ((1ull << TV) - 1ull)) << x.vi;
-                    bool gon = (__ballot(need != 0) & gmask) != 0 && e0 < e1;
-                    u64 acc = 0;
-                    uint32_t eb = e0;
-                    while (__any(gon)) {
-                        // a lane whose word already has every needed bit
-                        // reads the zero row instead of parent words
-                        acc |= scan_gather<TW>(p, x, cur, fb_cur, gon ? eb : e1, e1, w,
-                                               wok && (need & ~acc) != 0);
-                        eb += TW;
-                        const bool sat = (acc & need) == need;
-                        gon = gon && eb < e1 && (__ballot(!sat) & gmask) != 0;
-                    }
-                    n = acc & need;
-                    if (n) {
-                        atomicOr(&x.act_lds[w], n);
-                        p.seen[idx] = s | n;
-                        nxt[idx] = n;
-                    }
-                } else if (okv && w < W) {
-                    const uint64_t idx = (uint64_t)v * W + w;
-                    const u64 s = p.seen[idx];
-                    const u64 need = ~s & word_valid(p.S, w) & x.act_cur[w];
-                    if (need) {
-                        // Parents in steps of 8 with no branches inside a
-                        // step: indices past the end repeat the last parent,
-                        // parents outside the frontier read the zero row, so
-                        // each step is three rounds of independent loads
-                        // (indices, bitmap words, frontier words).
-                        typedef const __attribute__((address_space(1))) uint32_t *g32;
-                        typedef const __attribute__((address_space(1))) u64 *g64;
-                        constexpr int U = 8;
-                        u64 acc = 0;
-                        for (uint32_t e = e0; e < e1; e += U) {
-                            uint32_t uu[U];
-#pragma unroll
-                            for (int k = 0; k < U; ++k)
-                                uu[k] = ((g32)p.ri)[min(e + (uint32_t)k, e1 - 1)];
-                            uint32_t fw[U];
-#pragma unroll
-                            for (int k = 0; k < U; ++k) fw[k] = ((g32)fb_cur)[uu[k] >> 5];
-#pragma unroll
-                            for (int k = 0; k < U; ++k) {
-                                const bool inf = (fw[k] >> (uu[k] & 31)) & 1u;
-                                acc |= *(g64)(inf ? cur + (uint64_t)uu[k] * W + w : p.zrow + w);
-                            }
-                            if ((acc & need) == need) break;
-                        }
-                        n = acc & need;
-                        if (n) {
-                            atomicOr(&x.act_lds[w], n);
-                            p.seen[idx] = s | n;
-                            nxt[idx] = n;
-                        }
-                    }
-                }
-                if (w < W) lds_n[vl * LW + w] = n;
-                any |= n != 0;
-                my_bits += __popcll(n);
-            }
-            const u64 bal = __ballot(any);
-            bool vany = false;
-#pragma unroll
-            for (uint32_t k = 0; k < TW; ++k) vany |= ((bal >> (k * TV + x.vi)) & 1ull) != 0;
+            const bool vany = pull_vertex<TW>(p, x, cur, fb_cur, nxt, v, okv, e0, e1,
+                                              lds_n + vl * LW, my_bits);
             // New frontier vertices are gathered in LDS and published once
             // per block (one global atomic instead of one per wave).
             const bool app = x.wi == 0 && okv && vany;
@@ -1357,6 +1400,8 @@ __global__ void __launch_bounds__(kWideThreads) __attribute__((amdgpu_waves_per_
     for (uint32_t w = threadIdx.x; w < W; w += blockDim.x) x.act_lds[w] = 0;
     __shared__ uint32_t s_blk[4 + 64];
     __shared__ uint32_t s_wq[kWideThreads];
+    __shared__ u64 s_sum[2];
+    if (threadIdx.x < 2) s_sum[threadIdx.x] = 0;
     x.blk = s_blk;
     x.wq = s_wq + (threadIdx.x / 64) * 64;
     if (threadIdx.x < 4) s_blk[threadIdx.x] = 0;
@@ -1429,12 +1474,13 @@ __global__ void __launch_bounds__(kWideThreads) __attribute__((amdgpu_waves_per_
             wide_pull_resolve<TW>(p, x, cur, qcur, n_front, fb_cur, nxt, qnext, fb_next, c, nd,
                                   tick, gthread, nthreads, my_bits, my_deg);
         }
-        // Level counters for the direction controller: one atomic per wave.
+        // Level counters for the direction controller: summed per workgroup
+        // in LDS, one global atomic per workgroup (after the barrier below).
         {
             const u64 sb = wave_sum64(my_bits), sd = wave_sum64(my_deg);
             if (x.lane == 0) {
-                if (sb) atomicAdd(&c->active_bits, sb);
-                if (sd) atomicAdd(&c->deg_sum, sd);
+                if (sb) atomicAdd(&s_sum[0], sb);
+                if (sd) atomicAdd(&s_sum[1], sd);
             }
         }
         if (first) {
@@ -1448,6 +1494,12 @@ __global__ void __launch_bounds__(kWideThreads) __attribute__((amdgpu_waves_per_
         // frontier at the next: one atomic per word per workgroup.  The ring
         // slot of level L+2 was last read at level L-1 and is cleared here.
         __syncthreads();
+        if (threadIdx.x == 0) {
+            if (s_sum[0]) atomicAdd(&c->active_bits, s_sum[0]);
+            if (s_sum[1]) atomicAdd(&c->deg_sum, s_sum[1]);
+            s_sum[0] = 0;
+            s_sum[1] = 0;
+        }
         for (uint32_t w = threadIdx.x; w < W; w += blockDim.x) {
             const u64 a = x.act_lds[w];
             if (a) {
