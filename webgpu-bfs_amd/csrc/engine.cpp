@@ -617,6 +617,12 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
         HIP_TRY(launch_wide_seed(p, st));
         HIP_TRY(hipEventRecord(e->ev[1], st));
         HIP_TRY(launch_wide(tw, p, blocks, kWideThreads, lds, st));
+        {
+            // final pass: whole tiles per workgroup, up to four workgroups per CU
+            const uint64_t ntile = (V + kFinalVerts - 1) / kFinalVerts;
+            const int fblocks = (int)std::max<uint64_t>(1, std::min<uint64_t>(ntile, (uint64_t)c->cus * 4));
+            HIP_TRY(launch_wide_final(p, fblocks, st));
+        }
         HIP_TRY(hipEventRecord(e->ev[2], st));
         e->last.grid_blocks = (uint32_t)blocks;
         e->last.workspace_bytes = e->seen.bytes * 2 + e->fr.bytes + e->stamp.bytes * 4;

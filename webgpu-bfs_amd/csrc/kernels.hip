@@ -1536,15 +1536,25 @@ __global__ void __launch_bounds__(kWideThreads) __attribute__((amdgpu_waves_per_
         p.flags[FlagPullLevels] = pull_levels;
         if (p.trace && L < p.trace_cap) p.trace[4 * L] = wall_clock64();
     }
+}
 
-    // Final sweep (lane = vertex, so a source row's stores coalesce): the
+// Final pass of a wide traversal, launched after wide_bfs_kernel on the same
+// stream (its own launch, so its registers and LDS are sized for it alone).
+template <typename T, typename T2>
+__global__ void __launch_bounds__(kWideThreads) wide_final_kernel(WideParams p) {
+    __shared__ uint32_t L_nib[kFinalWords * 8 * kFinalVerts];  // [kFinalWords][8][kFinalVerts]
+    __shared__ u64 L_put[kFinalWords * kFinalVerts];            // [kFinalWords][kFinalVerts]
+    if (p.flags[FlagAbort]) return;  // the traversal did not finish
+    const uint32_t V = p.V, W = p.W;
+    const uint64_t stride = (uint64_t)V * W;
+    const uint32_t kept = min(p.flags[FlagLevels], p.defer);
+    // Lane = vertex in the row stores, so a source row's stores coalesce: the
     // distances of the kept levels are read from their slots as bit planes
     // (plane j = OR of the slots whose distance has bit j) and every source
     // row of the tile is written once, with the sentinel for unreached pairs;
     // pairs reached at a later, directly stored level are left alone.  The
     // kept slots and the visited words are cleared for the next run, and the
     // reachability words are written when asked.
-    const uint32_t kept = min(L, D);
     // Work items (tile, word group) in runs of whole tiles per workgroup, so
     // the word groups of one tile (sharing the vertices' cache lines) run on
     // one CU, in order: the last one clears the tile's level masks.
@@ -1553,8 +1563,6 @@ __global__ void __launch_bounds__(kWideThreads) __attribute__((amdgpu_waves_per_
     const uint64_t items = ntile * ngrp;
     const uint64_t per = ((ntile + gridDim.x - 1) / gridDim.x) * ngrp;
     const uint64_t i0 = min(items, blockIdx.x * per), i1 = min(items, i0 + per);
-    uint32_t *const L_nib = (uint32_t *)lds_n;  // [kFinalWords][8][kFinalVerts]
-    u64 *const L_put = (u64 *)(L_nib + kFinalWords * 8 * kFinalVerts);  // [kFinalWords][kFinalVerts]
     const uint32_t wave = threadIdx.x / 64;
     // thread -> (vertex, word) of an item: kFinalWords neighbouring threads
     // read one vertex's words of the group (contiguous)
@@ -1618,19 +1626,23 @@ __global__ void __launch_bounds__(kWideThreads) __attribute__((amdgpu_waves_per_
         L_put[wl * kFinalVerts + vl] = found | miss;
         load(item + 1);
         __syncthreads();
-        if (p.out) {
-            if (p.out_kind == 1)
-                final_tile_rows<uint16_t, uint32_t>((uint16_t *)p.out, V, v0, w0, W, p.S, L_nib,
-                                                    L_put, wave, x.lane);
-            else if (p.out_kind == 2)
-                final_tile_rows<uint8_t, uint16_t>((uint8_t *)p.out, V, v0, w0, W, p.S, L_nib,
-                                                   L_put, wave, x.lane);
-            else
-                final_tile_rows<uint32_t, u64>((uint32_t *)p.out, V, v0, w0, W, p.S, L_nib,
-                                               L_put, wave, x.lane);
-        }
+        if (p.out)
+            final_tile_rows<T, T2>((T *)p.out, V, v0, w0, W, p.S, L_nib, L_put, wave, lane_id());
         __syncthreads();
     }
+}
+
+hipError_t launch_wide_final(const WideParams &p, int blocks, hipStream_t st) {
+    if (p.out_kind == 1)
+        hipLaunchKernelGGL((wide_final_kernel<uint16_t, uint32_t>), dim3(blocks), dim3(kWideThreads),
+                           0, st, p);
+    else if (p.out_kind == 2)
+        hipLaunchKernelGGL((wide_final_kernel<uint8_t, uint16_t>), dim3(blocks), dim3(kWideThreads),
+                           0, st, p);
+    else
+        hipLaunchKernelGGL((wide_final_kernel<uint32_t, u64>), dim3(blocks), dim3(kWideThreads), 0,
+                           st, p);
+    return hipGetLastError();
 }
 
 template <int TW>

@@ -128,6 +128,9 @@ hipError_t launch_wide(int tile_words, const WideParams &p, int blocks, int thre
                        hipStream_t st);
 hipError_t wide_max_blocks_per_cu(int tile_words, int threads, size_t lds, int *out);
 hipError_t launch_wide_seed(const WideParams &p, hipStream_t st);
+// Final pass of a wide traversal (distances of the kept levels, sentinels,
+// reachability words, workspace clean-up); launched after launch_wide.
+hipError_t launch_wide_final(const WideParams &p, int blocks, hipStream_t st);
 
 hipError_t launch_zero_u32(uint32_t *ptr, uint64_t n, hipStream_t st);
 // u16 distances (all < 255 or 0xFFFF) -> u8 with 0xFF for unreached.
@@ -135,17 +138,12 @@ hipError_t launch_pack_u16_u8(const uint16_t *in, uint8_t *out, uint64_t n, hipS
 
 constexpr int kWideThreads = 512;
 
-// LDS bytes of the pull block buffer: nb vertices x (W + 1) words.
+// Final pass of a wide traversal: tiles of kFinalVerts vertices x kFinalWords
+// source words; per (vertex, word) eight nibble words (u32) and the mask of
+// source bits it stores (u64) are staged in LDS (static, 20 KiB).
+constexpr uint32_t kFinalVerts = 128, kFinalWords = 4;
 // Dynamic LDS of the wide kernel: a pull block's new words (nb rows of W+1)
 // and the workgroup's active-source words (W).
-// Final pass of the wide kernel: tiles of kFinalVerts vertices x kFinalWords
-// source words; per (vertex, word) eight nibble words (u32) and the mask of
-// source bits it stores (u64) are staged in LDS.
-constexpr uint32_t kFinalVerts = 128, kFinalWords = 4;
-constexpr size_t kFinalLds = (size_t)kFinalWords * kFinalVerts * (8 * 4 + 8);
-inline size_t wide_pull_lds(uint32_t nb, uint32_t W) {
-    const size_t pull = ((size_t)nb * (W + 1) + W) * 8;
-    return pull > kFinalLds ? pull : kFinalLds;
-}
+inline size_t wide_pull_lds(uint32_t nb, uint32_t W) { return ((size_t)nb * (W + 1) + W) * 8; }
 
 }  // namespace msbfs
