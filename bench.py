@@ -457,7 +457,7 @@ def main():
                      "traffic": traffic, "algorithmic_bytes": alg_bytes,
                      "byte_model": "compulsory" if deep else "level",
                      "compulsory_bytes": compulsory, "level_model_bytes": model_bytes,
-                     "kernel": "deep_bfs_kernel" if deep else "wide_bfs_kernel"},
+                     "kernel": "deep_bfs_kernel" if deep else "wide_bfs_kernel + wide_final_kernel"},
     }
 
     if not args.no_cpu_baseline:
