@@ -366,7 +366,7 @@ void deep_layout_v(const msbfs_context *c, const msbfs_tuning &t, uint64_t max_v
     const int B = p.bits;
     p.ell = ell;
     // Workgroup size: the 2048 threads of a CU shared by the tasks it holds
-    // at once (capped by LDS), between 256 and 1024.
+    // at once (capped by LDS), between 128 and 1024.
     if (t.deep_threads) {
         p.threads = (int)t.deep_threads;
     } else {
@@ -374,10 +374,12 @@ void deep_layout_v(const msbfs_context *c, const msbfs_tuning &t, uint64_t max_v
         const uint64_t fit = std::max<uint64_t>(
             1, c->lds_max / (vis + 2 * 512 * (B <= 8 ? 4 : 8) + kDeepStaticLds));
         const uint64_t k = std::min<uint64_t>((ntasks + c->cus - 1) / c->cus, fit);
+        // (road-like |V| = 114,599, S = 2048, B = 1, eight tasks per CU, ms:
+        // 64 threads 3.57, 128 3.39, 256 3.48; one task per CU: 256 best)
         if (k <= 1)
             p.threads = B >= 4 ? 1024 : 256;
         else
-            p.threads = k < 4 ? 512 : 256;
+            p.threads = k < 4 ? 512 : (k < 8 ? 256 : 128);
     }
     p.vis_words = (uint32_t)((max_v * B + 31) / 32);
     const size_t esz = B <= 8 ? 4 : 8;
@@ -1036,7 +1038,7 @@ extern "C" int msbfs_engine_autotune(msbfs_engine *e, const msbfs_graph *g,
             t.deep_bits = bits;
             if (deep_bits_for(e->ctx, t, g->V, S) != (int)bits) continue;  // does not fit LDS
             if (bits > 1 && (S + bits / 2 - 1) / (bits / 2) <= 1) break;   // wider than S
-            for (uint32_t threads : {256u, 512u, 1024u}) {
+            for (uint32_t threads : {128u, 256u, 512u, 1024u}) {
                 t.deep_threads = threads;
                 consider(dm, t);
             }
