@@ -1234,10 +1234,58 @@ __device__ __forceinline__ void wide_pull_resolve(const WideParams &p, const Wav
             if (k % W == 0) fb_cur[u >> 5] = 0;
         }
     }
-    // Tiles of (word, 64 heavy vertices), lane = vertex: a source bit's
-    // distances for the tile's vertices go out in one store instruction
-    // (heavy vertices are listed in increasing order).
     const uint32_t nh = p.n_heavy_in;
+    if (!x.direct) {
+        // Distances deferred: tiles of TV heavy vertices x TW words as in the
+        // push filter, so a vertex's words are read and written as one
+        // contiguous run and its lanes own it (no level stamp needed).
+        constexpr uint32_t TV = 64 / TW;
+        const uint32_t WG = (W + TW - 1) / TW;
+        const uint64_t htiles = (nh + TV - 1) / TV;
+        WaveAppender qa{x.wq};
+        for (uint64_t t = x.gwave; t < htiles; t += x.nwaves) {
+            const uint32_t i = (uint32_t)t * TV + x.vi;
+            const bool okv = i < nh;
+            const uint32_t h = okv ? p.heavy_in[i] : 0;
+            bool any = false;
+            for (uint32_t g = 0; g < WG; ++g) {
+                const uint32_t w = g * TW + x.wi;
+                u64 n = 0;
+                if (okv && w < W) {
+                    const uint64_t idx = (uint64_t)h * W + w;
+                    const u64 xw = p.nx[idx];
+                    if (xw) {
+                        p.nx[idx] = 0;
+                        const u64 s = p.seen[idx];
+                        n = xw & ~s & word_valid(p.S, w);
+                        if (n) {
+                            atomicOr(&x.act_lds[w], n);
+                            p.seen[idx] = s | n;
+                            nxt[idx] = n;
+                        }
+                    }
+                }
+                any |= n != 0;
+                my_bits += __popcll(n);
+            }
+            const u64 bal = __ballot(any);
+            bool vany = false;
+#pragma unroll
+            for (uint32_t k = 0; k < TW; ++k) vany |= ((bal >> (k * TV + x.vi)) & 1ull) != 0;
+            const bool app = x.wi == 0 && okv && vany;
+            qa.push(app, h, &c->next_count, qnext);
+            if (app) {
+                mark_frontier(fb_next, h);
+                if (nd < p.defer) p.lv[h] |= 1u << nd;
+                my_deg += p.ro[h + 1] - p.ro[h];
+            }
+        }
+        qa.flush(&c->next_count, qnext);
+        return;
+    }
+    // Distances stored now: tiles of (word, 64 heavy vertices), lane =
+    // vertex, so a source bit's distances for the tile's vertices go out in
+    // one store instruction (heavy vertices are listed in increasing order).
     const uint64_t hg = (nh + 63) / 64;
     const uint64_t tiles = hg * W;
     const uint32_t u16 = p.out_kind;
