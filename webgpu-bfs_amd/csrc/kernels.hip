@@ -1054,6 +1054,61 @@ __device__ __forceinline__ bool pull_vertex(const WideParams &p, const WaveCtx &
     return vany;
 }
 
+// Heavy in-list chunk [e0, e1) for tiles of TW >= 8 words: steps of 64
+// in-edges, lane group vi taking edges vi*TW .. vi*TW+TW-1 of each step, as
+// scan_gather, but software-pipelined: the next step's bitmap word and the
+// index after it are loaded before this step's gathers, so only the gathers
+// stay on the chain.  Stops once every word's needed bits are found.
+// Returns the OR of this lane's word over the chunk's frontier parents.
+template <int TW>
+__device__ __forceinline__ u64 heavy_gather(const WideParams &p, const WaveCtx &x, const u64 *cur,
+                                            const uint32_t *fb, uint32_t e0, uint32_t e1,
+                                            uint32_t w, bool wok, u64 need) {
+    constexpr uint32_t TV = 64 / TW;
+    static_assert(TV <= 8, "heavy_gather needs tiles of 8 or more words");
+    typedef const __attribute__((address_space(1))) uint32_t *g32;
+    typedef const __attribute__((address_space(1))) u64 *g64;
+    const u64 gmask = (~0ull / ((1ull << TV) - 1ull)) << x.vi;
+    const uint64_t W = p.W;
+    uint32_t e = e0 + x.vi * TW + x.wi;
+    uint32_t uu = e < e1 ? ((g32)p.ri)[e] : 0u;
+    uint32_t fw = e < e1 ? ((g32)fb)[uu >> 5] : 0u;
+    uint32_t un = e + 64 < e1 ? ((g32)p.ri)[e + 64] : 0u;
+    u64 acc = 0, got = 0;
+    for (uint32_t base = e0; base < e1; base += 64) {
+        const bool inf = e < e1 && ((fw >> (uu & 31)) & 1u);
+        u64 m = __ballot(inf) & gmask;
+        const uint32_t en = e + 64;
+        const uint32_t fwn = en < e1 ? ((g32)fb)[un >> 5] : 0u;
+        const uint32_t unn = en + 64 < e1 ? ((g32)p.ri)[en + 64] : 0u;
+        // a lane whose word already has every needed bit reads the zero row
+        const bool lok = wok && (need & ~got) != 0;
+        while (__any(m != 0)) {
+            uint32_t j[8];
+            bool ok[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                ok[k] = m != 0;
+                j[k] = ok[k] ? (uint32_t)__builtin_ctzll(m) : x.lane;
+                m &= m - 1;
+            }
+            uint32_t u[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) u[k] = (uint32_t)__shfl((int)uu, (int)j[k]);
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                acc |= *(g64)((ok[k] && lok) ? cur + (uint64_t)u[k] * W + w : p.zrow + (w & 63));
+        }
+        got = group_or64<TV>(acc);
+        if (__all((got & need) == need)) break;
+        e = en;
+        uu = un;
+        fw = fwn;
+        un = unn;
+    }
+    return acc;
+}
+
 // Pull.  Levels whose distances are deferred to the final pass: each wave
 // takes its own tiles of TV consecutive vertices and appends new frontier
 // vertices through its LDS buffer, so no wave waits for another's in-lists.
@@ -1181,15 +1236,7 @@ __device__ __forceinline__ void wide_pull(const WideParams &p, const WaveCtx &x,
         const bool wok = w < W;
         u64 acc = 0;
         if constexpr (TW >= 8) {
-            // each lane group scans its own TW-edge slice of every 64 edges;
-            // words whose needed bits are all found stop reading parents
-            u64 got = 0;
-            for (uint32_t base = e0; base < e1; base += 64) {
-                acc |= scan_gather<TW>(p, x, cur, fb_cur, base + x.vi * TW, e1, w,
-                                       wok && (need & ~got) != 0);
-                got = group_or64<TV>(acc);
-                if (__all((got & need) == need)) break;
-            }
+            acc = heavy_gather<TW>(p, x, cur, fb_cur, e0, e1, w, wok, need);
         } else
         for (uint32_t base = e0; base < e1; base += TV * U) {
             uint32_t uu[U];
