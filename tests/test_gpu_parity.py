@@ -381,3 +381,52 @@ def test_engine_autotune_keeps_results(ctx):
     assert r["method"] == M.BitParallel64Pull
     assert np.array_equal(e.run_ms_bfs(rmat, src, dist_dtype=np.uint16),
                           to_u16_expect(oracle(rmat, src)))
+
+
+def hub_and_path(n_leaves=3000, path=60):
+    """A hub of degree n_leaves (a heavy vertex on both lists) with a path of
+    `path` vertices hanging from one leaf: the wide family passes its deferred
+    levels and stores later distances at once (block pull, heavy resolve with
+    lane = vertex), and light vertices sit beside the heavy one."""
+    n = 1 + n_leaves + path
+    pairs = [(0, i) for i in range(1, n_leaves + 1)]
+    pairs += [(1, n_leaves + 1)] + [(n_leaves + 1 + i, n_leaves + 2 + i) for i in range(path - 1)]
+    return gb.csr_from_undirected_pairs(n, pairs)
+
+
+def test_random_graphs_every_path(ctx):
+    """Seeded sweep over the wide family's code paths and the deep family:
+    hub-heavy R-MAT (undirected and directed: true transpose, heavy chunks in
+    pull and push, heavy resolve), a hub with a long path (more levels than
+    the final pass defers, so both pull layouts and both heavy resolves run),
+    G(n, p) with many components; S at word boundaries; push-only,
+    pull-only, adaptive and deep methods; default and narrow wave tiles
+    (several word groups per vertex); u32, u16 and u8 outputs and the
+    reachability words.  Bar: bit-exact against the oracle."""
+    graphs = {
+        "rmat13": gb.generate_rmat(13, 16, seed=11),
+        "rmat12d": gb.generate_rmat(12, 16, seed=12, directed=True),
+        "hubpath": hub_and_path(),
+        "er6000": gb.generate_erdos_renyi_np(6000, 3.0 / 5999, seed=13),
+    }
+    cases = [(M.Auto, {}), (M.PushPullAdaptive, {}), (M.BitParallel64Pull, {}),
+             (M.BitParallel64, {}), (M.WorkgroupPersistentBfs, {}),
+             (M.PushPullAdaptive, {"tile_words": 4}), (M.PushPullAdaptive, {"tile_words": 8})]
+    for gname, g in graphs.items():
+        assert g.max_degree() > 128 or gname == "er6000", gname
+        for S in (1, 64, 65, 700):
+            src = gb.sample_sources(g, S, seed=S + 1)
+            d32 = oracle(g, src)
+            for m, tune in cases:
+                e = gb.GpuBfsEngine(ctx, m)
+                if tune:
+                    e.set_tuning(**tune)
+                say("sweep", gname, S, m.name, tune)
+                assert np.array_equal(e.run_ms_bfs(g, src), d32), (gname, S, m.name, tune)
+            e = gb.GpuBfsEngine(ctx, M.PushPullAdaptive)
+            say("sweep u16/u8/reach", gname, S)
+            assert np.array_equal(e.run_ms_bfs(g, src, dist_dtype=np.uint16), to_u16_expect(d32))
+            if int(d32[d32 != INF32].max(initial=0)) < 255:
+                assert np.array_equal(e.run_ms_bfs(g, src, dist_dtype=np.uint8), to_u8_expect(d32))
+            bm = e.run_ms_reachability(g, src)
+            assert np.array_equal(bm.words, orc.reachability_from_dist(g.num_vertices, S, d32))
