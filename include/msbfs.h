@@ -119,7 +119,8 @@ typedef struct msbfs_tuning {
     uint32_t tile_words;     /* wide family wave tile: 64/tile_words vertices x tile_words words (1,2,4,8) */
     uint32_t words_per_lane; /* reserved: ignored by this build */
     uint32_t deep_bits;      /* deep family: sources per instance (1,2,4,8,16,32); 0 = auto */
-    uint32_t deep_threads;   /* deep family workgroup size (a multiple of 64, <= 1024) */
+    uint32_t deep_threads;   /* deep family workgroup size (a multiple of 64, <= 1024; 64 with a
+                                neighbour table: the one-wave kernel) */
     uint32_t wide_blocks_per_cu; /* wide family persistent blocks per CU */
     uint32_t direction_rule; /* 0: measured frontier degree sum and size (default);
                                 1: the reference's estimate n_f = ceil(bits/S), m_f = n_f*|E|/|V| */
@@ -199,7 +200,7 @@ int msbfs_engine_level_trace(msbfs_engine *e, msbfs_level_record *out, uint32_t 
  * families compete and the engine's method becomes the winner's
  * (WorkgroupPersistentBfs or PushPullAdaptive); an explicit method is tuned
  * within its family.  Deep candidates: every sources-per-task width whose bits
- * fit LDS x workgroup sizes 128/256/512/1024.  Wide candidates, one knob at a time
+ * fit LDS x workgroup sizes 64 (one wave per task)/128/256/512/1024.  Wide candidates, one knob at a time
  * from the defaults: tile width (auto, half, double), blocks per CU (1, 2),
  * direction rule (measured, reference), alpha (7, 14, 28), beta (12, 24, 48).
  * Each candidate runs once untimed and twice timed (kernel time).  Output goes to

@@ -106,14 +106,17 @@ def test_road_like_full(engines, road, S):
 
 
 def test_road_like_deep_tile_variants(ctx, road):
-    """Every task width whose visited bits fit in LDS, at two workgroup
-    sizes; widths that do not fit fall back to the wide family."""
+    """Every task width whose visited bits fit in LDS, at three workgroup
+    sizes (64 threads: the one-wave kernel; neighbour tables of width 4 and
+    8); widths that do not fit fall back to the wide family."""
     small = gb.generate_road_like(20_000, 160, 900, seed=3)
-    for g, widths in ((road, (1, 2, 4, 8)), (small, (16, 32))):
+    sparse = gb.generate_erdos_renyi_np(20_000, 1.6 / 19_999, seed=21)
+    assert 4 < sparse.max_degree() <= 8  # a width-8 neighbour table
+    for g, widths in ((road, (1, 2, 4, 8)), (small, (16, 32)), (sparse, (1, 4, 32))):
         src = wl.sources(g, 300)
         expect = oracle(g, src)
         for bits in widths:
-            for threads in (256, 1024):
+            for threads in (64, 256, 1024):
                 e = gb.GpuBfsEngine(ctx, M.WorkgroupPersistentBfs)
                 e.set_tuning(deep_bits=bits, deep_threads=threads)
                 say("road V=%d S=300 deep bits" % g.num_vertices, bits, "threads", threads)
@@ -411,6 +414,7 @@ def test_random_graphs_every_path(ctx):
     }
     cases = [(M.Auto, {}), (M.PushPullAdaptive, {}), (M.BitParallel64Pull, {}),
              (M.BitParallel64, {}), (M.WorkgroupPersistentBfs, {}),
+             (M.WorkgroupPersistentBfs, {"deep_threads": 64}),
              (M.PushPullAdaptive, {"tile_words": 4}), (M.PushPullAdaptive, {"tile_words": 8})]
     for gname, g in graphs.items():
         assert g.max_degree() > 128 or gname == "er6000", gname

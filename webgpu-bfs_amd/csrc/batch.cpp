@@ -99,7 +99,7 @@ int batch_plan(msbfs_batch *b) {
         return b->graphs[x.graph]->E > b->graphs[y.graph]->E;
     });
     b->ntasks = tasks.size();
-    deep_layout_v(c, b->tune, b->max_v, b->ell, b->ntasks, p);
+    deep_layout_v(c, b->tune, b->max_v, b->ell, b->ntasks, p, 0);
     HIP_TRY(deep_prepare(B, p.ell, p.lds));
     int occ = 0;
     HIP_TRY(deep_max_blocks_per_cu(B, p.ell, p.threads, p.lds, &occ));

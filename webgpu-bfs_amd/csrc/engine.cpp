@@ -276,6 +276,7 @@ void msbfs::engine_release(msbfs_engine *e) {
     msbfs_context *ctx = e->ctx;
     (void)hipSetDevice(ctx->device);
     (void)hipDeviceSynchronize();
+    graph_release(e->pending_graph);
     for (auto &ev : e->ev)
         if (ev) (void)hipEventDestroy(ev);
     delete e;
@@ -362,13 +363,21 @@ int deep_bits_for(const msbfs_context *c, const msbfs_tuning &t, uint64_t V, uin
 // frontier queues.  Enough workgroups per CU are planned to hold every task
 // at once when LDS and threads allow.
 void deep_layout_v(const msbfs_context *c, const msbfs_tuning &t, uint64_t max_v, int ell,
-                   uint64_t ntasks, Plan &p) {
+                   uint64_t ntasks, Plan &p, uint32_t levels_seen) {
     const int B = p.bits;
     p.ell = ell;
     // Workgroup size: the 2048 threads of a CU shared by the tasks it holds
     // at once (capped by LDS), between 128 and 1024.
     if (t.deep_threads) {
         p.threads = (int)t.deep_threads;
+    } else if (B == 1 && ell > 0 && levels_seen > 0 && max_v <= 64ull * levels_seen) {
+        // One source per task and a frontier that fits one wave (|V| /
+        // levels, from an earlier traversal of this graph): the one-wave
+        // kernel (road-like |V| = 114,599, 3,080 levels, ms: S=2048 3.11 vs
+        // 3.44, S=256 2.07 vs 2.21; Grid 256x256 (128 per level) S=512 0.82
+        // vs 0.57 and sparse G(n,p) S=2048 11.5 vs 7.7 stay on the
+        // workgroup kernel).
+        p.threads = 64;
     } else {
         const uint64_t vis = ((max_v * B + 31) / 32) * 4;
         const uint64_t fit = std::max<uint64_t>(
@@ -457,7 +466,8 @@ Plan plan_for(const msbfs_engine *e, const msbfs_graph *g, uint64_t S) {
             p.family = MSBFS_FAMILY_WIDE;
             p.policy = MSBFS_DIR_ADAPTIVE;
         } else {
-            deep_layout_v(e->ctx, e->tune, g->V, g->ell_width, (S + p.bits - 1) / p.bits, p);
+            deep_layout_v(e->ctx, e->tune, g->V, g->ell_width, (S + p.bits - 1) / p.bits, p,
+                          g->deep_levels.load(std::memory_order_relaxed));
         }
     }
     return p;
@@ -695,6 +705,11 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
     e->pending = true;
     e->last_stream = st;
     e->pending_dtype = dtype;
+    if (plan.family == MSBFS_FAMILY_DEEP && plan.bits == 1 && g->ell_width > 0) {
+        msbfs_graph *const gm = const_cast<msbfs_graph *>(g);
+        gm->refs.fetch_add(1);
+        e->pending_graph = gm;
+    }
     return MSBFS_OK;
 }
 
@@ -702,12 +717,25 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
 int finish_run(msbfs_engine *e) {
     if (!e->pending) return MSBFS_OK;
     e->pending = false;
+    msbfs_graph *const pg = e->pending_graph;
+    e->pending_graph = nullptr;
     hipError_t er = hipStreamSynchronize(e->last_stream);
     if (er != hipSuccess) {
+        graph_release(pg);
         e->dirty = true;
         return hip_fail(er, "traversal");
     }
     const uint32_t *f = (const uint32_t *)e->h_flags.p;
+    if (pg) {
+        // record the level count for later plans on this graph
+        if (!f[FlagQueueOverflow] && !f[FlagAbort]) {
+            uint32_t cur = pg->deep_levels.load(std::memory_order_relaxed);
+            while (f[FlagLevels] > cur &&
+                   !pg->deep_levels.compare_exchange_weak(cur, f[FlagLevels], std::memory_order_relaxed)) {
+            }
+        }
+        graph_release(pg);
+    }
     e->last.levels = f[FlagLevels];
     e->last.push_levels = f[FlagPushLevels];
     e->last.pull_levels = f[FlagPullLevels];
@@ -1038,7 +1066,7 @@ extern "C" int msbfs_engine_autotune(msbfs_engine *e, const msbfs_graph *g,
             t.deep_bits = bits;
             if (deep_bits_for(e->ctx, t, g->V, S) != (int)bits) continue;  // does not fit LDS
             if (bits > 1 && (S + bits / 2 - 1) / (bits / 2) <= 1) break;   // wider than S
-            for (uint32_t threads : {128u, 256u, 512u, 1024u}) {
+            for (uint32_t threads : {64u, 128u, 256u, 512u, 1024u}) {
                 t.deep_threads = threads;
                 consider(dm, t);
             }

@@ -126,6 +126,9 @@ struct msbfs_graph {
     uint32_t *hchunk_out = nullptr, *hchunk_in = nullptr;  // uint4 (v, e0, e1, 0) items
     uint32_t n_hchunk_out = 0, n_hchunk_in = 0;
     uint64_t bytes = 0;
+    // Most BFS levels a deep traversal with one source per task has needed
+    // on this graph (0: none yet); |V| / levels estimates its frontier size.
+    std::atomic<uint32_t> deep_levels{0};
 };
 
 struct msbfs_engine {
@@ -154,6 +157,7 @@ struct msbfs_engine {
     hipStream_t last_stream = nullptr;
     bool pending = false;
     int pending_dtype = 0;
+    msbfs_graph *pending_graph = nullptr;  // held until its level count is recorded
 };
 
 
@@ -194,7 +198,8 @@ int ensure_zeroed(DevBuf &b, size_t bytes, bool force, hipStream_t st);
 int ensure_plain(DevBuf &b, size_t bytes);
 // LDS split of a deep-family workgroup for graphs of up to max_v vertices.
 void deep_layout_v(const msbfs_context *c, const msbfs_tuning &t, uint64_t max_v, int ell,
-                   uint64_t ntasks, Plan &p);
+                   uint64_t ntasks, Plan &p,
+                   uint32_t levels_seen);
 int deep_bits_for(const msbfs_context *c, const msbfs_tuning &t, uint64_t V, uint64_t S);
 
 }  // namespace msbfs

@@ -563,28 +563,217 @@ __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
     }
 }
 
+// One task per wave (workgroups of 64 threads) for graphs with a neighbour
+// table: the frontier counts live in scalar registers, queue slots are
+// reserved with ballot prefix sums instead of an LDS atomic, and the wave
+// needs no barrier (its LDS operations execute in order).  LDS layout,
+// queues with global spill, outputs and flags as in deep_bfs_kernel.
+template <int B, int ELL, int DT>
+__global__ void __launch_bounds__(64) deep_wave_kernel(DeepParams p) {
+    static_assert(ELL == 4 || ELL == 8, "the wave kernel reads neighbour tables");
+    constexpr bool OUT = DT != 0;
+    constexpr uint32_t KIND = DT == 1 ? 1u : DT == 3 ? 2u : 0u;
+    constexpr uint32_t ESZ = DT == 1 ? 2u : DT == 3 ? 1u : 4u;
+    constexpr uint32_t DMAX = DT == 1 ? 0xFFFEu : DT == 3 ? 0xFEu : 0xFFFFFFFFu;
+    typedef typename DeepEntry<B>::T Entry;
+    typedef __attribute__((address_space(3))) Entry *lq_t;
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    constexpr uint32_t VPW = 32 / B;
+    extern __shared__ uint32_t lds[];
+    __shared__ Entry s_sink[64];  // target of the appends a lane does not make
+
+    uint32_t *const vis = lds;
+    __attribute__((address_space(3))) uint32_t *const lvis =
+        (__attribute__((address_space(3))) uint32_t *)vis;
+    const uint32_t lane = lane_id();
+    const u64 lt_mask = lane ? ((~0ull) >> (64 - lane)) : 0ull;
+    Entry *const lq0 = (Entry *)(lds + p.lds_vis_words);
+    Entry *const lq1 = lq0 + p.lds_qcap;
+    const uint64_t gcap = p.cap - p.lds_qcap;
+    Entry *const gq0 = (Entry *)p.queues + (uint64_t)blockIdx.x * 2 * gcap;
+    Entry *const gq1 = gq0 + gcap;
+
+    for (;;) {
+        uint32_t t = 0;
+        if (lane == 0) t = atomicAdd(p.work, 1u);
+        t = (uint32_t)__builtin_amdgcn_readlane((int)t, 0);
+        if (t >= p.ntasks) break;
+        const DeepTask task = p.tasks[t];
+        const DeepGraph g = p.graphs[task.graph];
+        const uint32_t V = g.V;
+        const uint32_t nwords = (V + VPW - 1) / VPW;
+        for (uint32_t i = lane; i < nwords; i += 64) vis[i] = 0;
+        DeepQueue<Entry> qa{lq0, gq0, p.lds_qcap, gcap};
+        DeepQueue<Entry> qb{lq1, gq1, p.lds_qcap, gcap};
+        if (lane < task.nsrc) {
+            const uint32_t s = p.sources[task.src_off + lane];
+            const uint32_t bit = 1u << lane;
+            atomicOr(&vis[s / VPW], bit << ((s % VPW) * B));
+            qa.put(lane, pack_entry(s, bit, (Entry *)nullptr));
+        }
+        void *const out_task = OUT ? (void *)((char *)p.out + task.out_off * ESZ) : nullptr;
+        uint32_t n = task.nsrc;  // entries of this level (wave-uniform)
+        uint32_t level = 0;
+        while (n) {
+            if (OUT && level > DMAX && lane == 0) atomicOr(&p.flags[FlagU16Overflow], 1u);
+            const uint32_t dval = min(level, DMAX);
+            uint32_t nn = 0;  // entries appended for the next level (wave-uniform)
+            for (uint32_t base = 0; base < n; base += 64) {
+                const uint32_t i = base + lane;
+                const bool act = i < n;
+                auto pass = [&](auto spill) {
+                    uint32_t u = 0, f = 0;
+                    if (act) {
+                        Entry x;
+                        if constexpr (decltype(spill)::value)
+                            x = qa.get(i);
+                        else
+                            x = ((lq_t)qa.lds)[i];
+                        unpack_entry(x, u, f);
+                    }
+                    uint32_t nb[ELL];
+                    if (act) {
+                        const __attribute__((address_space(1))) u32x4 *row =
+                            (const __attribute__((address_space(1))) u32x4 *)(g.ell + (uint64_t)u * ELL);
+#pragma unroll
+                        for (int k = 0; k < ELL / 4; ++k) {
+                            const u32x4 r = row[k];
+                            nb[4 * k] = r.x;
+                            nb[4 * k + 1] = r.y;
+                            nb[4 * k + 2] = r.z;
+                            nb[4 * k + 3] = r.w;
+                        }
+                    }
+                    // the entry's lowest source bit, one straight-line store
+                    // per lane (a lane without an entry repeats lane 0's)
+                    if constexpr (OUT) {
+                        const uint32_t b0 = __builtin_ctz(f | 0x80000000u);
+                        const uint32_t u_l0 = (uint32_t)__builtin_amdgcn_readlane((int)u, 0);
+                        const uint32_t b_l0 = (uint32_t)__builtin_amdgcn_readlane((int)b0, 0);
+                        const uint32_t off = (act ? b0 : b_l0) * V + (act ? u : u_l0);
+                        if constexpr (KIND == 1)
+                            ((__attribute__((address_space(1))) uint16_t *)out_task)[off] = (uint16_t)dval;
+                        else if constexpr (KIND == 2)
+                            ((__attribute__((address_space(1))) uint8_t *)out_task)[off] = (uint8_t)dval;
+                        else
+                            ((__attribute__((address_space(1))) uint32_t *)out_task)[off] = dval;
+                    }
+                    // claims: all slots' LDS atomics, then one wait
+                    uint32_t newly[ELL];
+                    uint32_t c = 0;
+                    if (act) {
+                        uint32_t old[ELL];
+#pragma unroll
+                        for (int k = 0; k < ELL; ++k)
+                            old[k] = __hip_atomic_fetch_or(lvis + nb[k] / VPW, f << ((nb[k] % VPW) * B),
+                                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+#pragma unroll
+                        for (int k = 0; k < ELL; ++k) {
+                            newly[k] = f & ~(old[k] >> ((nb[k] % VPW) * B));
+                            c += newly[k] != 0;
+                        }
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < ELL; ++k) newly[k] = 0;
+                    }
+                    // slots: exclusive prefix of c over the wave from ELL ballots
+                    uint32_t pre = 0, tot = 0;
+#pragma unroll
+                    for (int j = 0; j < ELL; ++j) {
+                        const u64 bj = __ballot(c > (uint32_t)j);
+                        pre += (uint32_t)__popcll(bj & lt_mask);
+                        tot += (uint32_t)__popcll(bj);
+                    }
+                    uint32_t pos = nn + pre;
+                    if (nn + tot <= qb.lcap) {
+                        lq_t const q = (lq_t)qb.lds;
+                        lq_t const sk = (lq_t)s_sink + lane;
+#pragma unroll
+                        for (int k = 0; k < ELL; ++k) {
+                            *(newly[k] ? q + pos : sk) = pack_entry(nb[k], newly[k], (Entry *)nullptr);
+                            pos += newly[k] != 0;
+                        }
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < ELL; ++k) {
+                            if (newly[k]) {
+                                if (!qb.put(pos, pack_entry(nb[k], newly[k], (Entry *)nullptr)))
+                                    atomicOr(&p.flags[FlagQueueOverflow], 1u);
+                                ++pos;
+                            }
+                        }
+                    }
+                    nn += tot;
+                    // the entry's other source bits (sources whose searches
+                    // reach the vertex at the same level)
+                    if constexpr (OUT && B > 1)
+                        if (act) deep_write_bits<B, KIND>(out_task, V, u, f & (f - 1), dval);
+                };
+                if (base + 64 <= qa.lcap)
+                    pass(std::false_type{});
+                else
+                    pass(std::true_type{});
+            }
+            DeepQueue<Entry> tq = qa;
+            qa = qb;
+            qb = tq;
+            n = nn;
+            ++level;
+        }
+        if (lane == 0) atomicMax(&p.flags[FlagLevels], level);
+        if constexpr (OUT) {
+            for (uint32_t b = 0; b < task.nsrc; ++b) {
+                const uint64_t row = task.out_off + (uint64_t)b * V;
+                for (uint32_t v = lane; v < V; v += 64)
+                    if (!((vis[v / VPW] >> ((v % VPW) * B + b)) & 1u)) store_sentinel(p.out, KIND, row + v);
+            }
+        }
+        if (p.reach) {
+            const uint32_t word = (uint32_t)(task.reach_bit / 32), shift = task.reach_bit % 32;
+            const uint32_t valid = (task.nsrc == 32) ? 0xFFFFFFFFu : ((1u << task.nsrc) - 1u);
+            for (uint32_t v = lane; v < V; v += 64) {
+                const uint32_t bits = (vis[v / VPW] >> ((v % VPW) * B)) & valid;
+                if (bits) atomicOr(&p.reach[(uint64_t)v * p.reach_wpv + word], bits << shift);
+            }
+        }
+    }
+}
+
+// The deep kernel for a launch: the one-wave kernel for 64-thread
+// workgroups on a neighbour table, the workgroup kernel otherwise.
+template <int B, int ELL, int DT>
+static const void *deep_kernel_for(int threads) {
+    if constexpr (ELL > 0)
+        if (threads == 64) return (const void *)deep_wave_kernel<B, ELL, DT>;
+    return (const void *)deep_bfs_kernel<B, ELL, DT>;
+}
+
 template <int B, int ELL>
 static hipError_t deep_launch_t(const DeepParams &p, int blocks, int threads, size_t lds,
                                 hipStream_t st) {
-    if (!p.out)
-        hipLaunchKernelGGL((deep_bfs_kernel<B, ELL, 0>), dim3(blocks), dim3(threads), lds, st, p);
-    else if (p.out_kind == 1)
-        hipLaunchKernelGGL((deep_bfs_kernel<B, ELL, 1>), dim3(blocks), dim3(threads), lds, st, p);
-    else if (p.out_kind == 2)
-        hipLaunchKernelGGL((deep_bfs_kernel<B, ELL, 3>), dim3(blocks), dim3(threads), lds, st, p);
-    else
-        hipLaunchKernelGGL((deep_bfs_kernel<B, ELL, 2>), dim3(blocks), dim3(threads), lds, st, p);
-    return hipGetLastError();
+    const int dt = !p.out ? 0 : p.out_kind == 1 ? 1 : p.out_kind == 2 ? 3 : 2;
+    const void *k = dt == 0   ? deep_kernel_for<B, ELL, 0>(threads)
+                    : dt == 1 ? deep_kernel_for<B, ELL, 1>(threads)
+                    : dt == 3 ? deep_kernel_for<B, ELL, 3>(threads)
+                              : deep_kernel_for<B, ELL, 2>(threads);
+    void *args[] = {(void *)&p};
+    return hipLaunchKernel(k, dim3(blocks), dim3(threads), args, lds, st);
+}
+
+template <int B, int ELL, int DT>
+static hipError_t deep_prepare_one(const void *k, size_t lds) {
+    hipFuncAttributes a;
+    hipError_t e = hipFuncGetAttributes(&a, k);
+    if (e != hipSuccess) return e;
+    if (a.sharedSizeBytes > kDeepStaticLds) return hipErrorInvalidConfiguration;
+    return hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 }
 
 template <int B, int ELL, int DT>
 static hipError_t deep_prepare_dt(size_t lds) {
-    hipFuncAttributes a;
-    hipError_t e = hipFuncGetAttributes(&a, (const void *)deep_bfs_kernel<B, ELL, DT>);
-    if (e != hipSuccess) return e;
-    if (a.sharedSizeBytes > kDeepStaticLds) return hipErrorInvalidConfiguration;
-    return hipFuncSetAttribute((const void *)deep_bfs_kernel<B, ELL, DT>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipError_t e = deep_prepare_one<B, ELL, DT>(deep_kernel_for<B, ELL, DT>(256), lds);
+    if (e == hipSuccess && ELL > 0) e = deep_prepare_one<B, ELL, DT>(deep_kernel_for<B, ELL, DT>(64), lds);
+    return e;
 }
 
 template <int B, int ELL>
@@ -600,17 +789,11 @@ static hipError_t deep_prepare_t(size_t lds) {
 template <int B, int ELL>
 static hipError_t deep_occ_t(int threads, size_t lds, int *out) {
     int o[4] = {0, 0, 0, 0};
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&o[0], deep_bfs_kernel<B, ELL, 0>,
-                                                                threads, lds);
-    if (e == hipSuccess)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&o[1], deep_bfs_kernel<B, ELL, 1>,
-                                                         threads, lds);
-    if (e == hipSuccess)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&o[2], deep_bfs_kernel<B, ELL, 2>,
-                                                         threads, lds);
-    if (e == hipSuccess)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&o[3], deep_bfs_kernel<B, ELL, 3>,
-                                                         threads, lds);
+    const void *k[4] = {deep_kernel_for<B, ELL, 0>(threads), deep_kernel_for<B, ELL, 1>(threads),
+                        deep_kernel_for<B, ELL, 2>(threads), deep_kernel_for<B, ELL, 3>(threads)};
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < 4 && e == hipSuccess; ++i)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&o[i], k[i], threads, lds);
     *out = std::min(std::min(o[0], o[1]), std::min(o[2], o[3]));
     return e;
 }
