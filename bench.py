@@ -448,7 +448,8 @@ def main():
                    "family": ["none", "deep", "wide"][stats["family"]],
                    "levels": int(stats["levels"]), "max_distance": maxl,
                    "bits_per_instance": int(stats["bits_per_instance"]),
-                   "grid_blocks": int(stats["grid_blocks"]), "parallelism": "single GPU"},
+                   "grid_blocks": int(stats["grid_blocks"]),
+                   "block_threads": int(stats["block_threads"]), "parallelism": "single GPU"},
         "sources_per_s": round(S / (ms_per_step / 1e3), 1),
         "traversed_edges": te,
         "kernel_ms": round(kmean, 4),
@@ -457,7 +458,8 @@ def main():
                      "traffic": traffic, "algorithmic_bytes": alg_bytes,
                      "byte_model": "compulsory" if deep else "level",
                      "compulsory_bytes": compulsory, "level_model_bytes": model_bytes,
-                     "kernel": "deep_bfs_kernel" if deep else "wide_bfs_kernel + wide_final_kernel"},
+                     "kernel": ("deep_wave_kernel" if stats["block_threads"] == 64 else "deep_bfs_kernel")
+                     if deep else "wide_bfs_kernel + wide_final_kernel"},
     }
 
     if not args.no_cpu_baseline:

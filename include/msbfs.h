@@ -148,6 +148,8 @@ typedef struct msbfs_run_stats {
     uint32_t instances;      /* deep family instances */
     uint32_t bits_per_instance;
     uint32_t grid_blocks;
+    uint32_t block_threads;  /* workgroup size of the traversal kernel (deep family: 64 is the
+                                one-wave kernel) */
     float kernel_ms;         /* traversal kernel(s) device time, hipEvents */
     float total_ms;          /* whole call device time, hipEvents */
     uint64_t workspace_bytes;

@@ -129,6 +129,24 @@ def test_road_like_deep_tile_variants(ctx, road):
     assert e.last_stats()["family"] == int(gb.Family.WIDE)
 
 
+def test_deep_one_wave_routing(ctx, road):
+    """The first deep traversal of a graph records its level count; later
+    ones take the one-wave kernel when |V| / levels <= 64 (road-like: ~37
+    vertices per level) and keep the workgroup kernel otherwise (Grid
+    256x256: ~128).  Every run matches the oracle."""
+    for g, wave in ((road, True), (gb.generate_grid2d(256, 256), False)):
+        src = wl.sources(g, 512)
+        expect = oracle(g, src)
+        dg = gb.DeviceGraph(ctx, g)
+        e = gb.GpuBfsEngine(ctx, M.WorkgroupPersistentBfs)
+        for run in range(3):
+            say("one-wave routing V=%d run" % g.num_vertices, run)
+            assert np.array_equal(e.run_ms_bfs(dg, src), expect), (g.num_vertices, run)
+            st = e.last_stats()
+            assert st["bits_per_instance"] == 1
+            assert (st["block_threads"] == 64) == (wave and run > 0), (g.num_vertices, run, st)
+
+
 @pytest.fixture(scope="module")
 def rmat20():
     return wl.rmat(20)

@@ -121,6 +121,7 @@ int batch_plan(msbfs_batch *b) {
     b->last.instances = (uint32_t)b->ntasks;
     b->last.bits_per_instance = (uint32_t)B;
     b->last.grid_blocks = (uint32_t)b->blocks;
+    b->last.block_threads = (uint32_t)b->plan.threads;
     b->last.workspace_bytes = b->queues.bytes;
     b->planned = true;
     return MSBFS_OK;

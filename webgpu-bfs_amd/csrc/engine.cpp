@@ -637,6 +637,7 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
         }
         HIP_TRY(hipEventRecord(e->ev[2], st));
         e->last.grid_blocks = (uint32_t)blocks;
+        e->last.block_threads = (uint32_t)kWideThreads;
         e->last.workspace_bytes = e->seen.bytes * 2 + e->fr.bytes + e->stamp.bytes * 4;
     } else {
         const int B = plan.bits;
@@ -697,6 +698,7 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
         e->last.instances = (uint32_t)ntasks;
         e->last.bits_per_instance = (uint32_t)B;
         e->last.grid_blocks = (uint32_t)blocks;
+        e->last.block_threads = (uint32_t)threads;
         e->last.workspace_bytes = e->queues.bytes;
     }
     HIP_TRY(hipMemcpyAsync(e->h_flags.p, e->flags.p, FlagCount * 4, hipMemcpyDeviceToHost, st));

@@ -139,6 +139,7 @@ class RunStats(ctypes.Structure):
                 ("levels", ctypes.c_uint32), ("push_levels", ctypes.c_uint32),
                 ("pull_levels", ctypes.c_uint32), ("instances", ctypes.c_uint32),
                 ("bits_per_instance", ctypes.c_uint32), ("grid_blocks", ctypes.c_uint32),
+                ("block_threads", ctypes.c_uint32),
                 ("kernel_ms", ctypes.c_float), ("total_ms", ctypes.c_float),
                 ("workspace_bytes", ctypes.c_uint64)]
 
