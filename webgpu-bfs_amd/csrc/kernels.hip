@@ -601,6 +601,11 @@ __global__ void __launch_bounds__(64) deep_wave_kernel(DeepParams p) {
         const DeepTask task = p.tasks[t];
         const DeepGraph g = p.graphs[task.graph];
         const uint32_t V = g.V;
+        // the table base is uniform: scalar registers for the whole task
+        const uint64_t ell_a = (uint64_t)g.ell;
+        const uint32_t *const ell_s = (const uint32_t *)(
+            ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(ell_a >> 32)) << 32) |
+            (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)ell_a));
         const uint32_t nwords = (V + VPW - 1) / VPW;
         for (uint32_t i = lane; i < nwords; i += 64) vis[i] = 0;
         DeepQueue<Entry> qa{lq0, gq0, p.lds_qcap, gcap};
@@ -623,18 +628,21 @@ __global__ void __launch_bounds__(64) deep_wave_kernel(DeepParams p) {
                 const bool act = i < n;
                 auto pass = [&](auto spill) {
                     uint32_t u = 0, f = 0;
-                    if (act) {
-                        Entry x;
-                        if constexpr (decltype(spill)::value)
-                            x = qa.get(i);
-                        else
-                            x = ((lq_t)qa.lds)[i];
-                        unpack_entry(x, u, f);
+                    if constexpr (decltype(spill)::value) {
+                        if (act) unpack_entry(qa.get(i), u, f);
+                    } else {
+                        // all lanes read (i < lcap); a lane without an entry
+                        // takes vertex 0 and no source bits, so its row load
+                        // is valid and it appends nothing
+                        unpack_entry(((lq_t)qa.lds)[i], u, f);
+                        u = act ? u : 0u;
+                        f = act ? f : 0u;
                     }
+                    constexpr bool ALL = !decltype(spill)::value;
                     uint32_t nb[ELL];
-                    if (act) {
+                    if (ALL || act) {
                         const __attribute__((address_space(1))) u32x4 *row =
-                            (const __attribute__((address_space(1))) u32x4 *)(g.ell + (uint64_t)u * ELL);
+                            (const __attribute__((address_space(1))) u32x4 *)(ell_s + (uint64_t)u * ELL);
 #pragma unroll
                         for (int k = 0; k < ELL / 4; ++k) {
                             const u32x4 r = row[k];
@@ -647,7 +655,7 @@ __global__ void __launch_bounds__(64) deep_wave_kernel(DeepParams p) {
                     // the entry's lowest source bit, one straight-line store
                     // per lane (a lane without an entry repeats lane 0's)
                     if constexpr (OUT) {
-                        const uint32_t b0 = __builtin_ctz(f | 0x80000000u);
+                        const uint32_t b0 = B == 1 ? 0u : __builtin_ctz(f | 0x80000000u);
                         const uint32_t u_l0 = (uint32_t)__builtin_amdgcn_readlane((int)u, 0);
                         const uint32_t b_l0 = (uint32_t)__builtin_amdgcn_readlane((int)b0, 0);
                         const uint32_t off = (act ? b0 : b_l0) * V + (act ? u : u_l0);
@@ -661,7 +669,7 @@ __global__ void __launch_bounds__(64) deep_wave_kernel(DeepParams p) {
                     // claims: all slots' LDS atomics, then one wait
                     uint32_t newly[ELL];
                     uint32_t c = 0;
-                    if (act) {
+                    if (act) {  // (lanes without an entry would all hit vertex 0's words)
                         uint32_t old[ELL];
 #pragma unroll
                         for (int k = 0; k < ELL; ++k)
