@@ -174,38 +174,37 @@ extern "C" int msbfs_graph_create(msbfs_context *c, uint64_t V, uint64_t E, cons
         std::vector<uint32_t> hout, hin;
         for (uint64_t v = 0; v < V; ++v)
             if (ro[v + 1] - ro[v] > kHeavyDegree) hout.push_back((uint32_t)v);
-        // Heavy edge lists split into work items of at most kHeavyChunk edges.
-        auto chunks = [](const std::vector<uint32_t> &hv, const uint32_t *off) {
+        // Heavy edge lists split into work items of at most `len` edges.
+        auto chunks = [](const std::vector<uint32_t> &hv, const uint32_t *off, uint32_t len) {
             std::vector<uint32_t> out;
             for (uint32_t v : hv)
-                for (uint32_t b = off[v]; b < off[v + 1]; b += kHeavyChunk) {
+                for (uint32_t b = off[v]; b < off[v + 1]; b += len) {
                     out.push_back(v);
                     out.push_back(b);
-                    out.push_back(std::min(off[v + 1], b + kHeavyChunk));
+                    out.push_back(std::min(off[v + 1], b + len));
                     out.push_back(0);
                 }
             return out;
         };
         g->n_heavy_out = (uint32_t)hout.size();
         e = upload(&g->heavy_out, hout.data(), hout.size());
-        std::vector<uint32_t> cout = chunks(hout, ro);
+        std::vector<uint32_t> cout = chunks(hout, ro, kHeavyChunkOut);
         g->n_hchunk_out = (uint32_t)(cout.size() / 4);
         if (e == hipSuccess) e = upload(&g->hchunk_out, cout.data(), cout.size());
         if (e == hipSuccess) {
             if (g->symmetric) {
                 g->heavy_in = g->heavy_out;
                 g->n_heavy_in = g->n_heavy_out;
-                g->hchunk_in = g->hchunk_out;
-                g->n_hchunk_in = g->n_hchunk_out;
+                hin = hout;
             } else {
                 for (uint64_t v = 0; v < V; ++v)
                     if (co[v + 1] - co[v] > kHeavyDegree) hin.push_back((uint32_t)v);
                 g->n_heavy_in = (uint32_t)hin.size();
                 e = upload(&g->heavy_in, hin.data(), hin.size());
-                std::vector<uint32_t> cin = chunks(hin, co.data());
-                g->n_hchunk_in = (uint32_t)(cin.size() / 4);
-                if (e == hipSuccess) e = upload(&g->hchunk_in, cin.data(), cin.size());
             }
+            std::vector<uint32_t> cin = chunks(hin, g->symmetric ? ro : co.data(), kHeavyChunk);
+            g->n_hchunk_in = (uint32_t)(cin.size() / 4);
+            if (e == hipSuccess) e = upload(&g->hchunk_in, cin.data(), cin.size());
         }
     }
     if (e != hipSuccess) {
@@ -232,7 +231,7 @@ void msbfs::graph_release(msbfs_graph *g) {
     if (g->ell) (void)hipFree(g->ell);
     if (g->heavy_in && g->heavy_in != g->heavy_out) (void)hipFree(g->heavy_in);
     if (g->heavy_out) (void)hipFree(g->heavy_out);
-    if (g->hchunk_in && g->hchunk_in != g->hchunk_out) (void)hipFree(g->hchunk_in);
+    if (g->hchunk_in) (void)hipFree(g->hchunk_in);
     if (g->hchunk_out) (void)hipFree(g->hchunk_out);
     delete g;
     context_release(ctx);

@@ -976,8 +976,50 @@ __device__ __forceinline__ void wide_push_expand(const WideParams &p, const Wave
                                                  uint32_t tick) {
     constexpr uint32_t TV = 64 / TW;
     const uint32_t W = p.W, WG = (W + TW - 1) / TW;
-    const uint64_t ntiles = (uint64_t)((n_front + TV - 1) / TV) * WG;
     WaveAppender qa{x.wq};
+    // The whole wave ORs vertex h's active words into the children along
+    // out-edges [e0, e1), 64 edges per step, one word at a time; with
+    // `consume` the vertex's frontier words are cleared once read.
+    auto walk = [&](uint32_t h, uint32_t e0, uint32_t e1, bool consume) {
+        for (uint32_t w0 = 0; w0 < W; w0 += 64) {
+            const uint32_t wl = w0 + x.lane;
+            const u64 fl = wl < W ? cur[(uint64_t)h * W + wl] : 0ull;
+            if (consume && fl) cur[(uint64_t)h * W + wl] = 0;
+            u64 act = __ballot(fl != 0);
+            while (act) {
+                const uint32_t k = (uint32_t)__builtin_ctzll(act);
+                act &= act - 1;
+                const uint32_t w = w0 + k;
+                const u64 f = __shfl(fl, (int)k);
+                for (uint32_t base = e0; base < e1; base += 64) {
+                    const uint32_t e = base + x.lane;
+                    bool app = false;
+                    uint32_t v = 0;
+                    if (e < e1) {
+                        v = p.ci[e];
+                        const uint64_t idx = (uint64_t)v * W + w;
+                        const u64 m = f & ~p.seen[idx];
+                        // Only the first toucher of (v, w) can be the first
+                        // toucher of v, so only it consults the level stamp.
+                        if (m && atomicOr(&p.nx[idx], m) == 0 &&
+                            atomicExch(&p.stamp[v], tick) != tick)
+                            app = true;
+                    }
+                    qa.push(app, v, &c->cand_count, p.qc);
+                }
+            }
+        }
+    };
+    if (n_front <= x.nwaves) {
+        // Small frontier (sources, last levels): a wave per light vertex, so
+        // its edges are not walked one by one by a single lane.
+        for (uint64_t i = x.gwave; i < n_front; i += x.nwaves) {
+            const uint32_t u = qcur[i];
+            const uint32_t e0 = p.ro[u], e1 = p.ro[u + 1];
+            if (e1 - e0 <= p.heavy_t) walk(u, e0, e1, !x.keep);  // heavy: chunks below
+        }
+    } else {
+    const uint64_t ntiles = (uint64_t)((n_front + TV - 1) / TV) * WG;
     for (uint64_t t = x.gwave; t < ntiles; t += x.nwaves) {
         const uint32_t i = (uint32_t)(t / WG) * TV + x.vi;
         const uint32_t w = (uint32_t)(t % WG) * TW + x.wi;
@@ -1013,34 +1055,29 @@ __device__ __forceinline__ void wide_push_expand(const WideParams &p, const Wave
             qa.push(app, v, &c->cand_count, p.qc);
         }
     }
-    for (uint64_t t = x.gwave; t < p.n_hchunk_out; t += x.nwaves) {
-        const uint4 ch = p.hchunk_out[t];
-        const uint32_t h = ch.x, e0 = ch.y, e1 = ch.z;
-        if (!in_frontier(fb_cur, h)) continue;  // wave-uniform
-        for (uint32_t w0 = 0; w0 < W; w0 += 64) {
-            const uint32_t wl = w0 + x.lane;
-            const u64 fl = wl < W ? cur[(uint64_t)h * W + wl] : 0ull;
-            u64 act = __ballot(fl != 0);
-            while (act) {
-                const uint32_t k = (uint32_t)__builtin_ctzll(act);
-                act &= act - 1;
-                const uint32_t w = w0 + k;
-                const u64 f = __shfl(fl, (int)k);
-                for (uint32_t base = e0; base < e1; base += 64) {
-                    const uint32_t e = base + x.lane;
-                    bool app = false;
-                    uint32_t v = 0;
-                    if (e < e1) {
-                        v = p.ci[e];
-                        const uint64_t idx = (uint64_t)v * W + w;
-                        const u64 m = f & ~p.seen[idx];
-                        if (m && atomicOr(&p.nx[idx], m) == 0 &&
-                            atomicExch(&p.stamp[v], tick) != tick)
-                            app = true;
-                    }
-                    qa.push(app, v, &c->cand_count, p.qc);
-                }
-            }
+    }
+    // Heavy out-lists: chunk descriptors are scanned 64 at a time, lane =
+    // chunk, with a hub's consecutive chunks on different waves; the wave
+    // then walks each chunk whose vertex is in the frontier.
+    const uint64_t nhc = p.n_hchunk_out;
+    for (uint64_t it = 0; x.gwave + it * 64 * x.nwaves < nhc; ++it) {
+        const uint64_t tl = x.gwave + (it * 64 + x.lane) * x.nwaves;
+        uint32_t cx = 0, cy = 0, cz = 0;
+        bool cact = false;
+        if (tl < nhc) {
+            const uint4 ch = p.hchunk_out[tl];
+            cx = ch.x;
+            cy = ch.y;
+            cz = ch.z;
+            cact = in_frontier(fb_cur, cx);
+        }
+        u64 am = __ballot(cact);
+        while (am) {
+            const int kc = (int)__builtin_ctzll(am);
+            am &= am - 1;
+            walk((uint32_t)__builtin_amdgcn_readlane((int)cx, kc),
+                 (uint32_t)__builtin_amdgcn_readlane((int)cy, kc),
+                 (uint32_t)__builtin_amdgcn_readlane((int)cz, kc), false);
         }
     }
     qa.flush(&c->cand_count, p.qc);

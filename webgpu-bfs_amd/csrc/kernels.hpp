@@ -97,8 +97,11 @@ struct WideParams {
 // Degree above which the wide family gives a vertex's edge list to a whole
 // wave instead of one lane.
 constexpr uint32_t kHeavyDegree = 128;
-// Edges per work item when a heavy vertex's list is split across waves.
+// Edges per work item when a heavy vertex's list is split across waves:
+// in-lists (pull) and out-lists (push; shorter, so a hub source's edges
+// spread over many waves).
 constexpr uint32_t kHeavyChunk = 2048;
+constexpr uint32_t kHeavyChunkOut = 256;
 
 // Grid barrier of the wide kernel: top counter, 8 group counters, the
 // generation and the abort flag, each on its own 128-byte line.
