@@ -1154,10 +1154,12 @@ __device__ __forceinline__ void wide_push_filter(const WideParams &p, const Wave
 // R-MAT20 S=2048 eight per round beat four by 7%, twelve and sixteen spill
 // registers and lose), parent index broadcast from the lane that loaded it.
 // Returns the OR of this lane's word over those parents.
+// need / have: the lane's word's still-needed bits and those already found;
+// a group whose words all have their needed bits stops after the round.
 template <int TW>
 __device__ __forceinline__ u64 scan_gather(const WideParams &p, const WaveCtx &x, const u64 *cur,
                                            const uint32_t *fb, uint32_t ebase, uint32_t eend,
-                                           uint32_t w, bool wok) {
+                                           uint32_t w, bool wok, u64 need, u64 have) {
     constexpr uint32_t TV = 64 / TW;
     static_assert(TV <= 8, "scan_gather needs tiles of 8 or more words");
     typedef const __attribute__((address_space(1))) uint32_t *g32;
@@ -1188,6 +1190,7 @@ __device__ __forceinline__ u64 scan_gather(const WideParams &p, const WaveCtx &x
 #pragma unroll
         for (int k = 0; k < 8; ++k)
             acc |= *(g64)((ok[k] && wok) ? cur + (uint64_t)u[k] * W + w : p.zrow + (w & 63));
+        if (!(__ballot(((have | acc) & need) != need) & gmask)) m = 0;
     }
     return acc;
 }
@@ -1224,7 +1227,7 @@ __device__ __forceinline__ bool pull_vertex(const WideParams &p, const WaveCtx &
                 // a lane whose word already has every needed bit
                 // reads the zero row instead of parent words
                 acc |= scan_gather<TW>(p, x, cur, fb_cur, gon ? eb : e1, e1, w,
-                                       wok && (need & ~acc) != 0);
+                                       wok && (need & ~acc) != 0, need, acc);
                 eb += TW;
                 const bool sat = (acc & need) == need;
                 gon = gon && eb < e1 && (__ballot(!sat) & gmask) != 0;
