@@ -44,6 +44,13 @@ typedef unsigned long long u64;
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
+// acc + the number of bits of m set for lanes below this one (v_mbcnt pair:
+// two instructions in place of masking and counting each half).
+__device__ __forceinline__ uint32_t mbcnt64(u64 m, uint32_t acc) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)m, acc));
+}
+
 __device__ __forceinline__ u64 wave_or64(u64 x) {
     uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
 #pragma unroll
@@ -260,7 +267,6 @@ __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
     uint32_t *const vis = lds;
     const uint32_t tid = threadIdx.x, nth = blockDim.x;
     const uint32_t lane = lane_id();
-    const u64 lt_mask = lane ? ((~0ull) >> (64 - lane)) : 0ull;
     Entry *const lq0 = (Entry *)(lds + p.lds_vis_words);
     Entry *const lq1 = lq0 + p.lds_qcap;
     const uint64_t gcap = p.cap - p.lds_qcap;  // spill entries per queue
@@ -399,7 +405,7 @@ __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
 #pragma unroll
                         for (int j = 0; j < ELL; ++j) {
                             const u64 bj = __ballot(c > (uint32_t)j);
-                            pre += (uint32_t)__popcll(bj & lt_mask);
+                            pre = mbcnt64(bj, pre);
                             tot += (uint32_t)__popcll(bj);
                         }
                         uint32_t wbase = 0;
@@ -492,7 +498,7 @@ __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
 #pragma unroll
                             for (int j = 0; j < U; ++j) {
                                 const u64 bj = __ballot(c > (uint32_t)j);
-                                pre += (uint32_t)__popcll(bj & lt_mask);
+                                pre = mbcnt64(bj, pre);
                                 tot += (uint32_t)__popcll(bj);
                             }
                             if (tot) {
@@ -586,7 +592,6 @@ __global__ void __launch_bounds__(64) deep_wave_kernel(DeepParams p) {
     __attribute__((address_space(3))) uint32_t *const lvis =
         (__attribute__((address_space(3))) uint32_t *)vis;
     const uint32_t lane = lane_id();
-    const u64 lt_mask = lane ? ((~0ull) >> (64 - lane)) : 0ull;
     Entry *const lq0 = (Entry *)(lds + p.lds_vis_words);
     Entry *const lq1 = lq0 + p.lds_qcap;
     const uint64_t gcap = p.cap - p.lds_qcap;
@@ -689,7 +694,7 @@ __global__ void __launch_bounds__(64) deep_wave_kernel(DeepParams p) {
 #pragma unroll
                     for (int j = 0; j < ELL; ++j) {
                         const u64 bj = __ballot(c > (uint32_t)j);
-                        pre += (uint32_t)__popcll(bj & lt_mask);
+                        pre = mbcnt64(bj, pre);
                         tot += (uint32_t)__popcll(bj);
                     }
                     uint32_t pos = nn + pre;
@@ -1503,8 +1508,13 @@ __device__ __forceinline__ void wide_pull_resolve(const WideParams &p, const Wav
                                                   u64 &my_deg) {
     const uint32_t W = p.W;
     if (x.keep) {
-        // the frontier words stay for the final pass: only the bitmap goes
-        for (uint64_t k = gthread; k < n_front; k += nthreads) fb_cur[qcur[k] >> 5] = 0;
+        // the frontier words stay for the final pass: only the bitmap goes,
+        // as whole-array stores when the frontier outnumbers its words
+        const uint64_t nfw = (uint64_t)p.V / 32 + 1;
+        if (n_front > nfw)
+            for (uint64_t k = gthread; k < nfw; k += nthreads) fb_cur[k] = 0;
+        else
+            for (uint64_t k = gthread; k < n_front; k += nthreads) fb_cur[qcur[k] >> 5] = 0;
     } else {
         const uint64_t nclear = (uint64_t)n_front * W;
         for (uint64_t k = gthread; k < nclear; k += nthreads) {
