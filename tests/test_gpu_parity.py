@@ -452,3 +452,30 @@ def test_random_graphs_every_path(ctx):
                 assert np.array_equal(e.run_ms_bfs(g, src, dist_dtype=np.uint8), to_u8_expect(d32))
             bm = e.run_ms_reachability(g, src)
             assert np.array_equal(bm.words, orc.reachability_from_dist(g.num_vertices, S, d32))
+
+
+def test_one_engine_across_layouts(ctx):
+    """One wide-family engine reused while the workspace layout changes
+    (graph size, source words, deferred levels, reachability-only runs):
+    frontier slots are not cleared between runs, so each run must write
+    every row it later reads.  Bar: bit-exact against the oracle."""
+    rmat = gb.generate_rmat(13, 16, seed=11)
+    hub = hub_and_path()
+    er = gb.generate_erdos_renyi_np(6000, 3.0 / 5999, seed=13)
+    e = gb.GpuBfsEngine(ctx, M.PushPullAdaptive)
+    runs = [(rmat, 2048, "u32"), (rmat, 64, "u16"), (hub, 700, "u32"), (rmat, 2048, "reach"),
+            (er, 65, "u32"), (rmat, 2048, "u16"), (hub, 1, "u32"), (rmat, 2048, "u32"),
+            (er, 700, "reach"), (er, 700, "u32")]
+    for i, (g, S, kind) in enumerate(runs):
+        src = gb.sample_sources(g, S, seed=100 + i)
+        d32 = oracle(g, src)
+        say("layouts", i, g.num_vertices, S, kind)
+        if kind == "reach":
+            bm = e.run_ms_reachability(g, src)
+            assert np.array_equal(bm.words, orc.reachability_from_dist(g.num_vertices, S, d32)), i
+        elif kind == "u16":
+            got = e.run_ms_bfs(g, src, dist_dtype=np.uint16)
+            assert np.array_equal(got, to_u16_expect(d32)), i
+        else:
+            assert np.array_equal(e.run_ms_bfs(g, src), d32), i
+        assert e.last_stats()["family"] == int(gb.Family.WIDE), i

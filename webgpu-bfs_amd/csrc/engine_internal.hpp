@@ -137,7 +137,8 @@ struct msbfs_engine {
     int method = MSBFS_METHOD_AUTO;
     msbfs_tuning tune{};
     std::mutex mu;
-    // wide workspace (kept all-zero between runs by the kernels)
+    // wide workspace (kept all-zero between runs by the kernels, except the
+    // frontier slots `fr`, whose rows are written whole before they are read)
     msbfs::DevBuf seen, fr, nx, qf0, qf1, qc, stamp, ctr, fbA, fbB, lv, zrow, act;
     uint64_t wide_words = 0;  // entries allocated per state array
     uint32_t tick_base = 1;

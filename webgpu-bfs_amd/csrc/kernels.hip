@@ -913,7 +913,18 @@ __global__ void wide_seed_kernel(WideParams p) {
     }
 }
 
+// The level-0 slot rows of the sources' vertices, cleared before the seed
+// kernel ORs their bits in (frontier slots are not cleared after use).
+__global__ void wide_seed_rows_kernel(WideParams p) {
+    const uint64_t n = p.S * p.W;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p.fr[(uint64_t)p.sources[i / p.W] * p.W + i % p.W] = 0;
+}
+
 hipError_t launch_wide_seed(const WideParams &p, hipStream_t st) {
+    const uint64_t rblocks = std::min<uint64_t>((p.S * p.W + 255) / 256, 4096);
+    hipLaunchKernelGGL(wide_seed_rows_kernel, dim3((unsigned)rblocks), dim3(256), 0, st, p);
     const uint64_t blocks = (p.S + 255) / 256;
     hipLaunchKernelGGL(wide_seed_kernel, dim3((unsigned)blocks), dim3(256), 0, st, p);
     return hipGetLastError();
@@ -929,7 +940,6 @@ struct WaveCtx {
     const u64 *act_cur;  // sources whose frontier is not empty at this level (W words)
     uint32_t *blk;       // LDS: pull block's new frontier vertices: [0] count, [1..3]
                          // bitmap words from the block's first, [4..68) the vertices
-    bool keep;           // this level's frontier words stay for the final pass
     bool direct;         // this level's new distances are stored now (not deferred)
     uint32_t *wq;        // this wave's 64 LDS slots for WaveAppender
 };
@@ -967,6 +977,16 @@ __device__ __forceinline__ bool in_frontier(const uint32_t *fb, uint32_t u) {
 __device__ __forceinline__ void mark_frontier(uint32_t *fb, uint32_t v) {
     atomicOr(&fb[v >> 5], 1u << (v & 31));
 }
+// Clears the bitmap of a consumed frontier (vertices qf[0, n)): the words
+// of its vertices, or the whole array when the frontier outnumbers its words.
+__device__ __forceinline__ void clear_frontier(uint32_t *fb, const uint32_t *qf, uint32_t n,
+                                               uint32_t V, uint64_t gthread, uint64_t nthreads) {
+    const uint64_t nfw = (uint64_t)V / 32 + 1;
+    if (n > nfw)
+        for (uint64_t k = gthread; k < nfw; k += nthreads) fb[k] = 0;
+    else
+        for (uint64_t k = gthread; k < n; k += nthreads) fb[qf[k] >> 5] = 0;
+}
 
 // Push, part 1: OR each frontier vertex's words into its children's
 // candidate words (64-bit atomicOr), skipping children whose bits are all
@@ -983,13 +1003,11 @@ __device__ __forceinline__ void wide_push_expand(const WideParams &p, const Wave
     const uint32_t W = p.W, WG = (W + TW - 1) / TW;
     WaveAppender qa{x.wq};
     // The whole wave ORs vertex h's active words into the children along
-    // out-edges [e0, e1), 64 edges per step, one word at a time; with
-    // `consume` the vertex's frontier words are cleared once read.
-    auto walk = [&](uint32_t h, uint32_t e0, uint32_t e1, bool consume) {
+    // out-edges [e0, e1), 64 edges per step, one word at a time.
+    auto walk = [&](uint32_t h, uint32_t e0, uint32_t e1) {
         for (uint32_t w0 = 0; w0 < W; w0 += 64) {
             const uint32_t wl = w0 + x.lane;
             const u64 fl = wl < W ? cur[(uint64_t)h * W + wl] : 0ull;
-            if (consume && fl) cur[(uint64_t)h * W + wl] = 0;
             u64 act = __ballot(fl != 0);
             while (act) {
                 const uint32_t k = (uint32_t)__builtin_ctzll(act);
@@ -1021,7 +1039,7 @@ __device__ __forceinline__ void wide_push_expand(const WideParams &p, const Wave
         for (uint64_t i = x.gwave; i < n_front; i += x.nwaves) {
             const uint32_t u = qcur[i];
             const uint32_t e0 = p.ro[u], e1 = p.ro[u + 1];
-            if (e1 - e0 <= p.heavy_t) walk(u, e0, e1, !x.keep);  // heavy: chunks below
+            if (e1 - e0 <= p.heavy_t) walk(u, e0, e1);  // heavy: chunks below
         }
     } else {
     const uint64_t ntiles = (uint64_t)((n_front + TV - 1) / TV) * WG;
@@ -1037,7 +1055,6 @@ __device__ __forceinline__ void wide_push_expand(const WideParams &p, const Wave
                 const uint64_t idx = (uint64_t)u * W + w;
                 f = cur[idx];
                 if (f) {
-                    if (!x.keep) cur[idx] = 0;
                     e = e0;
                     eend = e1;
                 }
@@ -1082,15 +1099,16 @@ __device__ __forceinline__ void wide_push_expand(const WideParams &p, const Wave
             am &= am - 1;
             walk((uint32_t)__builtin_amdgcn_readlane((int)cx, kc),
                  (uint32_t)__builtin_amdgcn_readlane((int)cy, kc),
-                 (uint32_t)__builtin_amdgcn_readlane((int)cz, kc), false);
+                 (uint32_t)__builtin_amdgcn_readlane((int)cz, kc));
         }
     }
     qa.flush(&c->cand_count, p.qc);
 }
 
 // Push, part 2: new bits of each candidate become its next-frontier words
-// and distances; the candidate joins the next frontier queue.  The consumed
-// frontier (heavy words left by part 1, bitmap words) is cleared here.
+// (all of them, zeros included: the slot is not cleared between uses) and
+// distances; the candidate joins the next frontier queue.  The consumed
+// frontier's bitmap is cleared here.
 template <int TW>
 __device__ __forceinline__ void wide_push_filter(const WideParams &p, const WaveCtx &x, u64 *cur,
                                                  const uint32_t *qcur, uint32_t n_front,
@@ -1121,10 +1139,10 @@ __device__ __forceinline__ void wide_push_filter(const WideParams &p, const Wave
                     if (n) {
                         atomicOr(&x.act_lds[w], n);
                         p.seen[idx] = s | n;
-                        nxt[idx] = n;
                         if (x.direct) wide_write_lane(p, w, v, n, nd);
                     }
                 }
+                nxt[idx] = n;
             }
             any |= n != 0;
             my_bits += __popcll(n);
@@ -1142,13 +1160,7 @@ __device__ __forceinline__ void wide_push_filter(const WideParams &p, const Wave
         }
     }
     qa.flush(&c->next_count, qnext);
-    for (uint64_t k = gthread; k < n_front; k += nthreads) {
-        const uint32_t u = qcur[k];
-        fb_cur[u >> 5] = 0;
-        if (p.ro[u + 1] - p.ro[u] > p.heavy_t)
-            if (!x.keep)
-                for (uint32_t w = 0; w < W; ++w) cur[(uint64_t)u * W + w] = 0;
-    }
+    clear_frontier(fb_cur, qcur, n_front, p.V, gthread, nthreads);
 }
 
 // Scan-then-gather step for tiles of TW >= 8 words (TV = 64/TW lane
@@ -1205,8 +1217,10 @@ __device__ __forceinline__ u64 scan_gather(const WideParams &p, const WaveCtx &x
 // with unseen bits ORs the frontier words of its parents (only those set in
 // the frontier bitmap are read) until all still-needed bits are found (early
 // exit); new bits go to seen, the next frontier slot and, when lds_row is
-// given, to lds_row[w].  Returns whether the vertex got any new bit (the
-// same on all its lanes).
+// given, to lds_row[w].  A vertex that joins the next frontier has all its
+// words written to the slot, zeros included (slots are not cleared between
+// uses).  Returns whether the vertex got any new bit (the same on all its
+// lanes).
 template <int TW>
 __device__ __forceinline__ bool pull_vertex(const WideParams &p, const WaveCtx &x, const u64 *cur,
                                             const uint32_t *fb_cur, u64 *nxt, uint32_t v,
@@ -1215,6 +1229,7 @@ __device__ __forceinline__ bool pull_vertex(const WideParams &p, const WaveCtx &
     constexpr uint32_t TV = 64 / TW;
     const uint32_t W = p.W, WG = (W + TW - 1) / TW;
     bool any = false;
+    u64 n0 = 0;  // this lane's new bits of word group 0
     for (uint32_t g = 0; g < WG; ++g) {
         const uint32_t w = g * TW + x.wi;
         u64 n = 0;
@@ -1242,6 +1257,8 @@ __device__ __forceinline__ bool pull_vertex(const WideParams &p, const WaveCtx &
                 atomicOr(&x.act_lds[w], n);
                 p.seen[idx] = s | n;
                 nxt[idx] = n;
+            } else if (WG > 1 && wok) {
+                nxt[idx] = 0;  // several word groups: the row's other words are not known yet
             }
         } else if (okv && w < W) {
             const uint64_t idx = (uint64_t)v * W + w;
@@ -1278,15 +1295,19 @@ __device__ __forceinline__ bool pull_vertex(const WideParams &p, const WaveCtx &
                     nxt[idx] = n;
                 }
             }
+            if (WG > 1 && !n) nxt[idx] = 0;
         }
         if (lds_row && w < W) lds_row[w] = n;
         any |= n != 0;
+        n0 = g == 0 ? n : n0;
         my_bits += __popcll(n);
     }
     const u64 bal = __ballot(any);
     bool vany = false;
 #pragma unroll
     for (uint32_t k = 0; k < TW; ++k) vany |= ((bal >> (k * TV + x.vi)) & 1ull) != 0;
+    // one word group: the zero words of a vertex that joins the frontier
+    if (WG == 1 && vany && okv && x.wi < W && !n0) nxt[(uint64_t)v * W + x.wi] = 0;
     return vany;
 }
 
@@ -1507,22 +1528,9 @@ __device__ __forceinline__ void wide_pull_resolve(const WideParams &p, const Wav
                                                   uint64_t nthreads, u64 &my_bits,
                                                   u64 &my_deg) {
     const uint32_t W = p.W;
-    if (x.keep) {
-        // the frontier words stay for the final pass: only the bitmap goes,
-        // as whole-array stores when the frontier outnumbers its words
-        const uint64_t nfw = (uint64_t)p.V / 32 + 1;
-        if (n_front > nfw)
-            for (uint64_t k = gthread; k < nfw; k += nthreads) fb_cur[k] = 0;
-        else
-            for (uint64_t k = gthread; k < n_front; k += nthreads) fb_cur[qcur[k] >> 5] = 0;
-    } else {
-        const uint64_t nclear = (uint64_t)n_front * W;
-        for (uint64_t k = gthread; k < nclear; k += nthreads) {
-            const uint32_t u = qcur[k / W];
-            cur[(uint64_t)u * W + (k % W)] = 0;
-            if (k % W == 0) fb_cur[u >> 5] = 0;
-        }
-    }
+    // the frontier words stay (the final pass reads the kept levels'; a
+    // slot's rows are all rewritten when it is next used): only the bitmap goes
+    clear_frontier(fb_cur, qcur, n_front, p.V, gthread, nthreads);
     const uint32_t nh = p.n_heavy_in;
     if (!x.direct) {
         // Distances deferred: tiles of TV heavy vertices x TW words as in the
@@ -1550,9 +1558,9 @@ __device__ __forceinline__ void wide_pull_resolve(const WideParams &p, const Wav
                         if (n) {
                             atomicOr(&x.act_lds[w], n);
                             p.seen[idx] = s | n;
-                            nxt[idx] = n;
                         }
                     }
+                    nxt[idx] = n;  // every word: the slot is not cleared
                 }
                 any |= n != 0;
                 my_bits += __popcll(n);
@@ -1596,11 +1604,11 @@ __device__ __forceinline__ void wide_pull_resolve(const WideParams &p, const Wav
                 if (n) {
                     atomicOr(&x.act_lds[w], n);
                     p.seen[idx] = s | n;
-                    nxt[idx] = n;
                     my_bits += __popcll(n);
                     app = atomicExch(&p.stamp[h], tick) != tick;
                 }
             }
+            nxt[idx] = n;  // every word: the slot is not cleared
         }
         if (x.direct) {
             u64 un = wave_or64(n);
@@ -1782,7 +1790,6 @@ __global__ void __launch_bounds__(kWideThreads) __attribute__((amdgpu_waves_per_
         const uint32_t nd = L + 1;
         const uint32_t tick = p.tick_base + L + 1;
         x.act_cur = p.act + (size_t)(L % 3) * W;
-        x.keep = L < D;
         x.direct = p.out != nullptr && nd >= D;
         u64 my_bits = 0, my_deg = 0;
         if (p.trace && first && L < p.trace_cap) {
@@ -1885,13 +1892,19 @@ __global__ void __launch_bounds__(kWideThreads) wide_final_kernel(WideParams p) 
     const uint32_t V = p.V, W = p.W;
     const uint64_t stride = (uint64_t)V * W;
     const uint32_t kept = min(p.flags[FlagLevels], p.defer);
+    // With every level kept (no distance stored at once) and no
+    // reachability words wanted, the kept slots hold all of a vertex's bits:
+    // the visited words are cleared without being read.
+    const bool use_seen = p.reach != nullptr || p.flags[FlagLevels] > kept;
     // Lane = vertex in the row stores, so a source row's stores coalesce: the
     // distances of the kept levels are read from their slots as bit planes
     // (plane j = OR of the slots whose distance has bit j) and every source
     // row of the tile is written once, with the sentinel for unreached pairs;
     // pairs reached at a later, directly stored level are left alone.  The
-    // kept slots and the visited words are cleared for the next run, and the
-    // reachability words are written when asked.
+    // visited words are cleared for the next run (the kept slots are not:
+    // their rows are rewritten whole when next used, and only rows of
+    // vertices in a level's frontier are ever read), and the reachability
+    // words are written when asked.
     // Work items (tile, word group) in runs of whole tiles per workgroup, so
     // the word groups of one tile (sharing the vertices' cache lines) run on
     // one CU, in order: the last one clears the tile's level masks.
@@ -1914,7 +1927,7 @@ __global__ void __launch_bounds__(kWideThreads) wide_final_kernel(WideParams p) 
         const uint32_t w = (uint32_t)(item % ngrp) * kFinalWords + wl;
         const bool ok = item < i1 && v < V && w < W;
         const uint64_t idx = (uint64_t)v * W + w;
-        sw = ok ? p.seen[idx] : 0ull;
+        sw = (ok && use_seen) ? p.seen[idx] : 0ull;
         const uint32_t lvm = (ok && kept) ? p.lv[v] : 0u;
 #pragma unroll
         for (uint32_t d = 0; d < 16; ++d)
@@ -1937,13 +1950,10 @@ __global__ void __launch_bounds__(kWideThreads) wide_final_kernel(WideParams p) 
             if (d & 8u) pl3 |= f[d];
         }
         if (ok) {
-            if (sw) p.seen[idx] = 0;
+            if (use_seen ? sw != 0 : found != 0) p.seen[idx] = 0;
             // the tile's last word group clears the level mask (this thread's
             // load of it is done; the tile's other items ran before on this block)
             if (wl == 0 && kept && item % ngrp == ngrp - 1) p.lv[v] = 0;
-#pragma unroll
-            for (uint32_t d = 0; d < 16; ++d)
-                if (f[d]) p.fr[(uint64_t)d * stride + idx] = 0;
             if (p.reach) {
                 const u64 r = sw & word_valid(p.S, w);
                 const uint64_t base = (uint64_t)v * p.reach_wpv + 2 * w;
@@ -1951,7 +1961,7 @@ __global__ void __launch_bounds__(kWideThreads) wide_final_kernel(WideParams p) 
                 if (2 * w + 1 < p.reach_wpv) p.reach[base + 1] = (uint32_t)(r >> 32);
             }
         }
-        const u64 miss = ok ? ~sw & word_valid(p.S, w) : 0ull;
+        const u64 miss = ok ? ~(use_seen ? sw : found) & word_valid(p.S, w) : 0ull;
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             const uint32_t sh = 8 * k;
