@@ -1387,17 +1387,18 @@ __device__ __forceinline__ void wide_pull(const WideParams &p, const WaveCtx &x,
     const uint64_t nblk = (V + NB - 1) / NB;
     const uint32_t u16 = p.out_kind;
     if (!x.direct) {
-        const uint64_t ntv = (V + TV - 1) / TV;
+        // Light vertices from their list (vertex and in-list bounds in one
+        // load; vertices without in-edges are not in it), TV per tile;
+        // heavy ones are chunked below.
+        const uint32_t nl = p.n_light_in;
+        const uint64_t ntv = (nl + TV - 1) / TV;
         WaveAppender qa{x.wq};
         for (uint64_t t = x.gwave; t < ntv; t += x.nwaves) {
-            const uint32_t v = (uint32_t)t * TV + x.vi;
-            bool okv = v < V;
-            uint32_t e0 = 0, e1 = 0;
-            if (okv) {
-                e0 = p.co[v];
-                e1 = p.co[v + 1];
-                if (e1 - e0 > p.heavy_t) okv = false;  // chunked below
-            }
+            const uint32_t i = (uint32_t)t * TV + x.vi;
+            const bool okv = i < nl;
+            uint4 it = make_uint4(0u, 0u, 0u, 0u);
+            if (okv) it = p.light_in[i];
+            const uint32_t v = it.x, e0 = it.y, e1 = it.z;
             const bool vany = pull_vertex<TW>(p, x, cur, fb_cur, nxt, v, okv, e0, e1, nullptr,
                                               my_bits);
             const bool app = x.wi == 0 && okv && vany;

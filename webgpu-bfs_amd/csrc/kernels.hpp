@@ -86,6 +86,8 @@ struct WideParams {
     const uint4 *hchunk_out;  // (vertex, first edge, end edge, 0): heavy out-lists in chunks
     const uint4 *hchunk_in;   // same for heavy in-lists (CSC)
     uint32_t n_hchunk_out, n_hchunk_in;
+    const uint4 *light_in;  // (vertex, first in-edge, end, 0) of in-degree 1..heavy_t, increasing
+    uint32_t n_light_in;
     uint32_t pull_nb;       // vertices per workgroup block in pull levels
     const unsigned long long *zrow;  // >= W zero words: gather target of parents not in the frontier
     unsigned long long *act;  // ring of 3 x W words: sources whose frontier is not empty
