@@ -615,8 +615,13 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
         p.n_light_in = g->n_light_in;
         e->tick_base += (uint32_t)(V + 2);
 
+        // Tile width: lanes per vertex.  With an even word count of 16 or
+        // more the deferred pull levels give each lane two adjacent words,
+        // so a wave holds twice the vertices (each with one scan of its
+        // in-list); an explicit tile width keeps one word per lane.
+        p.pair = !e->tune.tile_words && W % 2 == 0 && W >= 16;
         int tw = 1;
-        while (tw < 64 && (uint64_t)tw < W) tw *= 2;
+        while (tw < 64 && (uint64_t)tw < (p.pair ? W / 2 : W)) tw *= 2;
         if (e->tune.tile_words) tw = (int)e->tune.tile_words;
         const uint32_t tv = 64 / tw;
         uint32_t nb = 64;

@@ -879,6 +879,54 @@ __device__ __forceinline__ u64 group_or64(u64 x) {
     return ((u64)hi << 32) | lo;
 }
 
+// A lane's source words in the pull: one (u64) or two adjacent ones (u64x2,
+// 16-byte loads; the source word count is even).
+typedef u64 u64x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ bool nz(u64 a) { return a != 0; }
+__device__ __forceinline__ bool nz(u64x2 a) { return (a.x | a.y) != 0; }
+__device__ __forceinline__ uint32_t popc(u64 a) { return (uint32_t)__popcll(a); }
+__device__ __forceinline__ uint32_t popc(u64x2 a) {
+    return (uint32_t)__popcll(a.x) + (uint32_t)__popcll(a.y);
+}
+template <uint32_t TV>
+__device__ __forceinline__ u64 group_or(u64 a) { return group_or64<TV>(a); }
+template <uint32_t TV>
+__device__ __forceinline__ u64x2 group_or(u64x2 a) {
+    u64x2 r;
+    r.x = group_or64<TV>(a.x);
+    r.y = group_or64<TV>(a.y);
+    return r;
+}
+// All-ones and zero words.
+__device__ __forceinline__ void wfill(u64 &a, u64 v) { a = v; }
+__device__ __forceinline__ void wfill(u64x2 &a, u64 v) {
+    a.x = v;
+    a.y = v;
+}
+template <typename WT>
+__device__ __forceinline__ WT wconst(u64 v) {
+    WT a;
+    wfill(a, v);
+    return a;
+}
+// Valid and active source bits of the lane's words.
+__device__ __forceinline__ u64 words_live(const WideParams &p, const u64 *act, uint32_t w, u64) {
+    return word_valid(p.S, w) & act[w];
+}
+__device__ __forceinline__ u64x2 words_live(const WideParams &p, const u64 *act, uint32_t w, u64x2) {
+    u64x2 r;
+    r.x = word_valid(p.S, w) & act[w];
+    r.y = word_valid(p.S, w + 1) & act[w + 1];
+    return r;
+}
+__device__ __forceinline__ void act_or(u64 *act_lds, uint32_t w, u64 n) {
+    if (n) atomicOr(&act_lds[w], n);
+}
+__device__ __forceinline__ void act_or(u64 *act_lds, uint32_t w, u64x2 n) {
+    if (n.x) atomicOr(&act_lds[w], n.x);
+    if (n.y) atomicOr(&act_lds[w + 1], n.y);
+}
+
 // Per-lane distance stores for scattered vertices (push filter, heavy
 // vertices): one store per set bit.
 __device__ __forceinline__ void wide_write_lane(const WideParams &p, uint32_t w, uint32_t v,
@@ -1173,14 +1221,14 @@ __device__ __forceinline__ void wide_push_filter(const WideParams &p, const Wave
 // Returns the OR of this lane's word over those parents.
 // need / have: the lane's word's still-needed bits and those already found;
 // a group whose words all have their needed bits stops after the round.
-template <int TW>
-__device__ __forceinline__ u64 scan_gather(const WideParams &p, const WaveCtx &x, const u64 *cur,
-                                           const uint32_t *fb, uint32_t ebase, uint32_t eend,
-                                           uint32_t w, bool wok, u64 need, u64 have) {
+template <int TW, typename WT>
+__device__ __forceinline__ WT scan_gather(const WideParams &p, const WaveCtx &x, const u64 *cur,
+                                          const uint32_t *fb, uint32_t ebase, uint32_t eend,
+                                          uint32_t w, bool wok, WT need, WT have) {
     constexpr uint32_t TV = 64 / TW;
     static_assert(TV <= 8, "scan_gather needs tiles of 8 or more words");
     typedef const __attribute__((address_space(1))) uint32_t *g32;
-    typedef const __attribute__((address_space(1))) u64 *g64;
+    typedef const __attribute__((address_space(1))) WT *gw;
     const u64 gmask = (~0ull / ((1ull << TV) - 1ull)) << x.vi;
     const uint32_t e = ebase + x.wi;
     uint32_t uu = 0;
@@ -1190,7 +1238,7 @@ __device__ __forceinline__ u64 scan_gather(const WideParams &p, const WaveCtx &x
         inf = (((g32)fb)[uu >> 5] >> (uu & 31)) & 1u;
     }
     u64 m = __ballot(inf) & gmask;
-    u64 acc = 0;
+    WT acc = wconst<WT>(0);
     const uint64_t W = p.W;
     while (__any(m != 0)) {
         uint32_t j[8];
@@ -1206,8 +1254,8 @@ __device__ __forceinline__ u64 scan_gather(const WideParams &p, const WaveCtx &x
         for (int k = 0; k < 8; ++k) u[k] = (uint32_t)__shfl((int)uu, (int)j[k]);
 #pragma unroll
         for (int k = 0; k < 8; ++k)
-            acc |= *(g64)((ok[k] && wok) ? cur + (uint64_t)u[k] * W + w : p.zrow + (w & 63));
-        if (!(__ballot(((have | acc) & need) != need) & gmask)) m = 0;
+            acc |= *(gw)((ok[k] && wok) ? cur + (uint64_t)u[k] * W + w : p.zrow + (w & 63));
+        if (!(__ballot(nz(((have | acc) & need) ^ need)) & gmask)) m = 0;
     }
     return acc;
 }
@@ -1221,44 +1269,47 @@ __device__ __forceinline__ u64 scan_gather(const WideParams &p, const WaveCtx &x
 // words written to the slot, zeros included (slots are not cleared between
 // uses).  Returns whether the vertex got any new bit (the same on all its
 // lanes).
-template <int TW>
+template <int TW, typename WT = u64>
 __device__ __forceinline__ bool pull_vertex(const WideParams &p, const WaveCtx &x, const u64 *cur,
                                             const uint32_t *fb_cur, u64 *nxt, uint32_t v,
                                             bool okv, uint32_t e0, uint32_t e1, u64 *lds_row,
                                             u64 &my_bits) {
     constexpr uint32_t TV = 64 / TW;
-    const uint32_t W = p.W, WG = (W + TW - 1) / TW;
+    constexpr uint32_t K = sizeof(WT) / 8;  // source words per lane
+    static_assert(K == 1 || TW >= 8, "two words per lane need tiles of 8 or more lanes");
+    const uint32_t W = p.W, WG = (W + TW * K - 1) / (TW * K);
     bool any = false;
-    u64 n0 = 0;  // this lane's new bits of word group 0
+    WT n0 = wconst<WT>(0);  // this lane's new bits of word group 0
     for (uint32_t g = 0; g < WG; ++g) {
-        const uint32_t w = g * TW + x.wi;
-        u64 n = 0;
+        const uint32_t w = (g * TW + x.wi) * K;
+        WT n = wconst<WT>(0);
         if constexpr (TW >= 8) {
             // The vertex's TW lanes scan its in-list together.
             const bool wok = okv && w < W;
             const uint64_t idx = (uint64_t)v * W + w;
-            const u64 s = wok ? p.seen[idx] : ~0ull;
-            const u64 need = wok ? ~s & word_valid(p.S, w) & x.act_cur[w] : 0ull;
+            const WT s = wok ? *(const WT *)(p.seen + idx) : wconst<WT>(~0ull);
+            const WT need = wok ? ~s & words_live(p, x.act_cur, w, s) : wconst<WT>(0);
             const u64 gmask = (~0ull / ((1ull << TV) - 1ull)) << x.vi;
-            bool gon = (__ballot(need != 0) & gmask) != 0 && e0 < e1;
-            u64 acc = 0;
+            bool gon = (__ballot(nz(need)) & gmask) != 0 && e0 < e1;
+            WT acc = wconst<WT>(0);
             uint32_t eb = e0;
             while (__any(gon)) {
-                // a lane whose word already has every needed bit
+                // a lane whose words already have every needed bit
                 // reads the zero row instead of parent words
-                acc |= scan_gather<TW>(p, x, cur, fb_cur, gon ? eb : e1, e1, w,
-                                       wok && (need & ~acc) != 0, need, acc);
+                acc |= scan_gather<TW, WT>(p, x, cur, fb_cur, gon ? eb : e1, e1, w,
+                                           wok && nz(need & ~acc), need, acc);
                 eb += TW;
-                const bool sat = (acc & need) == need;
+                const bool sat = !nz((acc & need) ^ need);
                 gon = gon && eb < e1 && (__ballot(!sat) & gmask) != 0;
             }
             n = acc & need;
-            if (n) {
-                atomicOr(&x.act_lds[w], n);
-                p.seen[idx] = s | n;
-                nxt[idx] = n;
+            if (nz(n)) {
+                act_or(x.act_lds, w, n);
+                *(WT *)(p.seen + idx) = s | n;
+                *(WT *)(nxt + idx) = n;
             } else if (WG > 1 && wok) {
-                nxt[idx] = 0;  // several word groups: the row's other words are not known yet
+                // several word groups: the row's other words are not known yet
+                *(WT *)(nxt + idx) = n;
             }
         } else if (okv && w < W) {
             const uint64_t idx = (uint64_t)v * W + w;
@@ -1297,17 +1348,19 @@ __device__ __forceinline__ bool pull_vertex(const WideParams &p, const WaveCtx &
             }
             if (WG > 1 && !n) nxt[idx] = 0;
         }
-        if (lds_row && w < W) lds_row[w] = n;
-        any |= n != 0;
-        n0 = g == 0 ? n : n0;
-        my_bits += __popcll(n);
+        if constexpr (K == 1)
+            if (lds_row && w < W) lds_row[w] = n;
+        any |= nz(n);
+        if (g == 0) n0 = n;
+        my_bits += popc(n);
     }
     const u64 bal = __ballot(any);
     bool vany = false;
 #pragma unroll
     for (uint32_t k = 0; k < TW; ++k) vany |= ((bal >> (k * TV + x.vi)) & 1ull) != 0;
     // one word group: the zero words of a vertex that joins the frontier
-    if (WG == 1 && vany && okv && x.wi < W && !n0) nxt[(uint64_t)v * W + x.wi] = 0;
+    if (WG == 1 && vany && okv && x.wi * K < W && !nz(n0))
+        *(WT *)(nxt + (uint64_t)v * W + x.wi * K) = wconst<WT>(0);
     return vany;
 }
 
@@ -1317,21 +1370,21 @@ __device__ __forceinline__ bool pull_vertex(const WideParams &p, const WaveCtx &
 // index after it are loaded before this step's gathers, so only the gathers
 // stay on the chain.  Stops once every word's needed bits are found.
 // Returns the OR of this lane's word over the chunk's frontier parents.
-template <int TW>
-__device__ __forceinline__ u64 heavy_gather(const WideParams &p, const WaveCtx &x, const u64 *cur,
-                                            const uint32_t *fb, uint32_t e0, uint32_t e1,
-                                            uint32_t w, bool wok, u64 need) {
+template <int TW, typename WT>
+__device__ __forceinline__ WT heavy_gather(const WideParams &p, const WaveCtx &x, const u64 *cur,
+                                           const uint32_t *fb, uint32_t e0, uint32_t e1,
+                                           uint32_t w, bool wok, WT need) {
     constexpr uint32_t TV = 64 / TW;
     static_assert(TV <= 8, "heavy_gather needs tiles of 8 or more words");
     typedef const __attribute__((address_space(1))) uint32_t *g32;
-    typedef const __attribute__((address_space(1))) u64 *g64;
+    typedef const __attribute__((address_space(1))) WT *gw;
     const u64 gmask = (~0ull / ((1ull << TV) - 1ull)) << x.vi;
     const uint64_t W = p.W;
     uint32_t e = e0 + x.vi * TW + x.wi;
     uint32_t uu = e < e1 ? ((g32)p.ri)[e] : 0u;
     uint32_t fw = e < e1 ? ((g32)fb)[uu >> 5] : 0u;
     uint32_t un = e + 64 < e1 ? ((g32)p.ri)[e + 64] : 0u;
-    u64 acc = 0, got = 0;
+    WT acc = wconst<WT>(0), got = wconst<WT>(0);
     for (uint32_t base = e0; base < e1; base += 64) {
         const bool inf = e < e1 && ((fw >> (uu & 31)) & 1u);
         u64 m = __ballot(inf) & gmask;
@@ -1339,7 +1392,7 @@ __device__ __forceinline__ u64 heavy_gather(const WideParams &p, const WaveCtx &
         const uint32_t fwn = en < e1 ? ((g32)fb)[un >> 5] : 0u;
         const uint32_t unn = en + 64 < e1 ? ((g32)p.ri)[en + 64] : 0u;
         // a lane whose word already has every needed bit reads the zero row
-        const bool lok = wok && (need & ~got) != 0;
+        const bool lok = wok && nz(need & ~got);
         while (__any(m != 0)) {
             uint32_t j[8];
             bool ok[8];
@@ -1354,10 +1407,10 @@ __device__ __forceinline__ u64 heavy_gather(const WideParams &p, const WaveCtx &
             for (int k = 0; k < 8; ++k) u[k] = (uint32_t)__shfl((int)uu, (int)j[k]);
 #pragma unroll
             for (int k = 0; k < 8; ++k)
-                acc |= *(g64)((ok[k] && lok) ? cur + (uint64_t)u[k] * W + w : p.zrow + (w & 63));
+                acc |= *(gw)((ok[k] && lok) ? cur + (uint64_t)u[k] * W + w : p.zrow + (w & 63));
         }
-        got = group_or64<TV>(acc);
-        if (__all((got & need) == need)) break;
+        got = group_or<TV>(acc);
+        if (__all(!nz((got & need) ^ need))) break;
         e = en;
         uu = un;
         fw = fwn;
@@ -1399,8 +1452,17 @@ __device__ __forceinline__ void wide_pull(const WideParams &p, const WaveCtx &x,
             uint4 it = make_uint4(0u, 0u, 0u, 0u);
             if (okv) it = p.light_in[i];
             const uint32_t v = it.x, e0 = it.y, e1 = it.z;
-            const bool vany = pull_vertex<TW>(p, x, cur, fb_cur, nxt, v, okv, e0, e1, nullptr,
-                                              my_bits);
+            bool vany;
+            if constexpr (TW >= 8) {
+                if (p.pair)  // two adjacent words per lane: TV vertices over 2*TW words
+                    vany = pull_vertex<TW, u64x2>(p, x, cur, fb_cur, nxt, v, okv, e0, e1,
+                                                  nullptr, my_bits);
+                else
+                    vany = pull_vertex<TW>(p, x, cur, fb_cur, nxt, v, okv, e0, e1, nullptr,
+                                           my_bits);
+            } else {
+                vany = pull_vertex<TW>(p, x, cur, fb_cur, nxt, v, okv, e0, e1, nullptr, my_bits);
+            }
             const bool app = x.wi == 0 && okv && vany;
             qa.push(app, v, &c->next_count, qnext);
             if (app) {
@@ -1477,6 +1539,32 @@ __device__ __forceinline__ void wide_pull(const WideParams &p, const WaveCtx &x,
     // TV-th in-edge, U parents per lane per step (R-MAT20 S=64, level 1:
     // U = 1 394 us, U = 8 289 us); a chunk stops once its words' needed
     // bits (less those other chunks already found) are in.
+    if constexpr (TW >= 8) {
+        if (p.pair) {
+            // two adjacent words per lane, as the light vertices above
+            const uint32_t WG2 = (W + 2 * TW - 1) / (2 * TW);
+            const uint64_t nh2 = (uint64_t)p.n_hchunk_in * WG2;
+            for (uint64_t t = x.gwave; t < nh2; t += x.nwaves) {
+                const uint4 ch = p.hchunk_in[t / WG2];
+                const uint32_t h = ch.x, e0 = ch.y, e1 = ch.z;
+                const uint32_t w = ((uint32_t)(t % WG2) * TW + x.wi) * 2;
+                const uint64_t idx = (uint64_t)h * W + w;
+                const bool wok = w < W;
+                u64x2 need = wconst<u64x2>(0);
+                if (wok)
+                    need = ~*(const u64x2 *)(p.seen + idx) & ~*(const u64x2 *)(p.nx + idx) &
+                           words_live(p, x.act_cur, w, need);
+                if (!__any(nz(need))) continue;  // wave-uniform
+                const u64x2 acc = heavy_gather<TW, u64x2>(p, x, cur, fb_cur, e0, e1, w, wok, need);
+                const u64x2 g = group_or<TV>(acc) & need;
+                if (x.vi == 0) {
+                    if (g.x) atomicOr(&p.nx[idx], g.x);
+                    if (g.y) atomicOr(&p.nx[idx + 1], g.y);
+                }
+            }
+            return;
+        }
+    }
     const uint64_t nh = (uint64_t)p.n_hchunk_in * WG;
     constexpr uint32_t U = 8;  // (narrow tiles) independent in-edges per lane per step
     for (uint64_t t = x.gwave; t < nh; t += x.nwaves) {
@@ -1495,7 +1583,7 @@ __device__ __forceinline__ void wide_pull(const WideParams &p, const WaveCtx &x,
         const bool wok = w < W;
         u64 acc = 0;
         if constexpr (TW >= 8) {
-            acc = heavy_gather<TW>(p, x, cur, fb_cur, e0, e1, w, wok, need);
+            acc = heavy_gather<TW, u64>(p, x, cur, fb_cur, e0, e1, w, wok, need);
         } else
         for (uint32_t base = e0; base < e1; base += TV * U) {
             uint32_t uu[U];
