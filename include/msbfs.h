@@ -117,7 +117,10 @@ typedef struct msbfs_tuning {
     uint32_t alpha;          /* push->pull when m_f > |E|/alpha (push_pull.rs:22-36), default 14 */
     uint32_t beta;           /* pull->push when n_f < |V|/beta, default 24 */
     uint32_t tile_words;     /* wide family wave tile: 64/tile_words vertices x tile_words words (1,2,4,8) */
-    uint32_t words_per_lane; /* reserved: ignored by this build */
+    uint32_t words_per_lane; /* wide family, pull levels with deferred distances: adjacent source
+                                words per lane, 1 or 2 (two need an even word count and a tile of 8
+                                or more lanes, else one is used); 0 = auto: 2 when the word count
+                                is even and >= 16 and tile_words is 0 */
     uint32_t deep_bits;      /* deep family: sources per instance (1,2,4,8,16,32); 0 = auto */
     uint32_t deep_threads;   /* deep family workgroup size (a multiple of 64, <= 1024; 64 with a
                                 neighbour table: the one-wave kernel) */

@@ -483,17 +483,20 @@ def test_one_engine_across_layouts(ctx):
 
 def test_two_words_per_lane_pull(ctx):
     """Source word counts where deferred pull levels give each lane two
-    adjacent words (even, >= 16: 16, 18 with idle lanes, 64), against an
-    explicit tile width that keeps one word per lane; hub-heavy R-MAT
-    (heavy in-chunks) and the hub with a path (direct levels after the
-    deferred ones).  Bar: bit-exact against the oracle."""
+    adjacent words by default (even, >= 16: 16, 18 with idle lanes, 64),
+    against one word per lane (asked for, or implied by an explicit tile
+    width) and two words on a wider explicit tile (several word groups at
+    S=4096); hub-heavy R-MAT (heavy in-chunks) and the hub with a path
+    (direct levels after the deferred ones).  Bar: bit-exact against the
+    oracle."""
     graphs = {"rmat13": gb.generate_rmat(13, 16, seed=11), "hubpath": hub_and_path()}
     for gname, g in graphs.items():
         for S in (1024, 1090, 4096):
             # repeated sources where S exceeds |V| (independent rows)
             src = np.resize(gb.sample_sources(g, min(S, g.num_vertices), seed=S + 7), S)
             d32 = oracle(g, src)
-            for tune in ({}, {"tile_words": 16}):
+            for tune in ({}, {"tile_words": 16}, {"words_per_lane": 1},
+                         {"tile_words": 8, "words_per_lane": 2}):
                 e = gb.GpuBfsEngine(ctx, M.PushPullAdaptive)
                 if tune:
                     e.set_tuning(**tune)

@@ -319,6 +319,8 @@ extern "C" int msbfs_engine_set_tuning(msbfs_engine *e, const msbfs_tuning *t) {
     if (!e || !t) return fail(MSBFS_ERR_INVALID_ARG, "null argument");
     if (t->tile_words && (t->tile_words > 64 || (t->tile_words & (t->tile_words - 1))))
         return fail(MSBFS_ERR_INVALID_ARG, "tile_words must be a power of two <= 64");
+    if (t->words_per_lane > 2)
+        return fail(MSBFS_ERR_INVALID_ARG, "words_per_lane must be 0, 1 or 2");
     if (t->deep_bits && (t->deep_bits > 32 || (t->deep_bits & (t->deep_bits - 1))))
         return fail(MSBFS_ERR_INVALID_ARG, "deep_bits must be 1, 2, 4, 8, 16 or 32");
     if (t->deep_threads && (t->deep_threads % 64 || t->deep_threads > 1024))
@@ -616,13 +618,22 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
         e->tick_base += (uint32_t)(V + 2);
 
         // Tile width: lanes per vertex.  With an even word count of 16 or
-        // more the deferred pull levels give each lane two adjacent words,
-        // so a wave holds twice the vertices (each with one scan of its
-        // in-list); an explicit tile width keeps one word per lane.
-        p.pair = !e->tune.tile_words && W % 2 == 0 && W >= 16;
-        int tw = 1;
-        while (tw < 64 && (uint64_t)tw < (p.pair ? W / 2 : W)) tw *= 2;
-        if (e->tune.tile_words) tw = (int)e->tune.tile_words;
+        // more the deferred pull levels give each lane two adjacent words by
+        // default, so a wave holds twice the vertices (each with one scan of
+        // its in-list); an explicit tile width keeps one word per lane
+        // unless two are asked for.
+        auto tile_for = [&](uint64_t k) {
+            int t = 1;
+            while (t < 64 && (uint64_t)t * k < W) t *= 2;
+            return t;
+        };
+        const uint32_t wpl = e->tune.words_per_lane;
+        p.pair = W % 2 == 0 && (wpl == 2 || (wpl == 0 && !e->tune.tile_words && W >= 16));
+        int tw = e->tune.tile_words ? (int)e->tune.tile_words : tile_for(p.pair ? 2 : 1);
+        if (p.pair && tw < 8) {  // two words per lane need tiles of 8 or more lanes
+            p.pair = 0;
+            if (!e->tune.tile_words) tw = tile_for(1);
+        }
         const uint32_t tv = 64 / tw;
         uint32_t nb = 64;
         if (W > 64) {
@@ -1103,6 +1114,7 @@ extern "C" int msbfs_engine_autotune(msbfs_engine *e, const msbfs_graph *g,
         // one knob at a time from the defaults, keeping each knob's best
         msbfs_tuning cur = base;
         cur.tile_words = 0;
+        cur.words_per_lane = 0;
         cur.wide_blocks_per_cu = 0;
         cur.direction_rule = 0;
         const float w0 = [&] {
@@ -1142,6 +1154,7 @@ extern "C" int msbfs_engine_autotune(msbfs_engine *e, const msbfs_graph *g,
         std::vector<uint32_t> tws;
         if (tw > 1) tws.push_back(tw / 2);
         if (tw < 64) tws.push_back(tw * 2);
+        knob([](msbfs_tuning &t, uint32_t v) { t.words_per_lane = v; }, {1u});
         knob([](msbfs_tuning &t, uint32_t v) { t.tile_words = v; }, tws);
         knob([](msbfs_tuning &t, uint32_t v) { t.wide_blocks_per_cu = v; }, {1u});
         knob([](msbfs_tuning &t, uint32_t v) { t.direction_rule = v; }, {1u});
