@@ -435,7 +435,7 @@ def test_random_graphs_every_path(ctx):
              (M.WorkgroupPersistentBfs, {"deep_threads": 64}),
              (M.PushPullAdaptive, {"tile_words": 4}), (M.PushPullAdaptive, {"tile_words": 8})]
     for gname, g in graphs.items():
-        assert g.max_degree() > 128 or gname == "er6000", gname
+        assert g.max_degree() > 256 or gname == "er6000", gname
         for S in (1, 64, 65, 700):
             src = gb.sample_sources(g, S, seed=S + 1)
             d32 = oracle(g, src)
