@@ -2010,19 +2010,27 @@ __global__ void __launch_bounds__(kWideThreads) wide_final_kernel(WideParams p) 
     // Software pipeline: an item's loads are issued before the previous
     // item's row stores.
     // Only the slots of levels whose frontier held the vertex are read.
+    // The level mask is loaded one item further ahead than the slots it
+    // selects, so a slot load never waits on a mask load issued behind the
+    // previous item's row stores (a wave's memory counter retires in order).
     u64 sw = 0, f[16];
-    auto load = [&](uint64_t item) {
+    auto mask_of = [&](uint64_t item) -> uint32_t {
+        const uint32_t v = (uint32_t)(item / ngrp) * kFinalVerts + vl;
+        return (item < i1 && v < V && kept) ? p.lv[v] : 0u;
+    };
+    auto load = [&](uint64_t item, uint32_t lvm) {
         const uint32_t v = (uint32_t)(item / ngrp) * kFinalVerts + vl;
         const uint32_t w = (uint32_t)(item % ngrp) * kFinalWords + wl;
         const bool ok = item < i1 && v < V && w < W;
         const uint64_t idx = (uint64_t)v * W + w;
         sw = (ok && use_seen) ? p.seen[idx] : 0ull;
-        const uint32_t lvm = (ok && kept) ? p.lv[v] : 0u;
+        lvm = ok ? lvm : 0u;
 #pragma unroll
         for (uint32_t d = 0; d < 16; ++d)
             f[d] = (d < kept && ((lvm >> d) & 1u)) ? p.fr[(uint64_t)d * stride + idx] : 0ull;
     };
-    load(i0);
+    uint32_t lv_next = mask_of(i0 + 1);
+    load(i0, mask_of(i0));
     for (uint64_t item = i0; item < i1; ++item) {
         const uint32_t v0 = (uint32_t)(item / ngrp) * kFinalVerts;
         const uint32_t w0 = (uint32_t)(item % ngrp) * kFinalWords;
@@ -2060,7 +2068,12 @@ __global__ void __launch_bounds__(kWideThreads) wide_final_kernel(WideParams p) 
                 spread8x4((uint32_t)(miss >> sh)) * 15u;
         }
         L_put[wl * kFinalVerts + vl] = found | miss;
-        load(item + 1);
+        // (the mask of item + 2 is read before this item clears any mask:
+        // a tile's masks are cleared by its last word group, and item + 2
+        // then belongs to the next tile)
+        const uint32_t lv_after = mask_of(item + 2);
+        load(item + 1, lv_next);
+        lv_next = lv_after;
         __syncthreads();
         if (p.out)
             final_tile_rows<T, T2>((T *)p.out, V, v0, w0, W, p.S, L_nib, L_put, wave, lane_id());
