@@ -919,6 +919,20 @@ __device__ __forceinline__ u64x2 words_live(const WideParams &p, const u64 *act,
     r.y = word_valid(p.S, w + 1) & act[w + 1];
     return r;
 }
+// Valid source bits of the lane's words.
+__device__ __forceinline__ u64 words_valid(const WideParams &p, uint32_t w, u64) {
+    return word_valid(p.S, w);
+}
+__device__ __forceinline__ u64x2 words_valid(const WideParams &p, uint32_t w, u64x2) {
+    u64x2 r;
+    r.x = word_valid(p.S, w);
+    r.y = word_valid(p.S, w + 1);
+    return r;
+}
+__device__ __forceinline__ void words_or(u64 *a, u64x2 n) {
+    if (n.x) atomicOr(&a[0], n.x);
+    if (n.y) atomicOr(&a[1], n.y);
+}
 __device__ __forceinline__ void act_or(u64 *act_lds, uint32_t w, u64 n) {
     if (n) atomicOr(&act_lds[w], n);
 }
@@ -1621,6 +1635,54 @@ __device__ __forceinline__ void wide_pull_resolve(const WideParams &p, const Wav
     // slot's rows are all rewritten when it is next used): only the bitmap goes
     clear_frontier(fb_cur, qcur, n_front, p.V, gthread, nthreads);
     const uint32_t nh = p.n_heavy_in;
+    if constexpr (TW >= 8) {
+        if (!x.direct && p.pair) {
+            // as below with two adjacent words per lane (one word group)
+            constexpr uint32_t TV = 64 / TW;
+            const uint32_t WG2 = (W + 2 * TW - 1) / (2 * TW);
+            const uint64_t htiles = (nh + TV - 1) / TV;
+            WaveAppender qa{x.wq};
+            for (uint64_t t = x.gwave; t < htiles; t += x.nwaves) {
+                const uint32_t i = (uint32_t)t * TV + x.vi;
+                const bool okv = i < nh;
+                const uint32_t h = okv ? p.heavy_in[i] : 0;
+                bool any = false;
+                for (uint32_t g = 0; g < WG2; ++g) {
+                    const uint32_t w = (g * TW + x.wi) * 2;
+                    u64x2 n = wconst<u64x2>(0);
+                    if (okv && w < W) {
+                        const uint64_t idx = (uint64_t)h * W + w;
+                        const u64x2 xw = *(const u64x2 *)(p.nx + idx);
+                        if (nz(xw)) {
+                            *(u64x2 *)(p.nx + idx) = wconst<u64x2>(0);
+                            const u64x2 s = *(const u64x2 *)(p.seen + idx);
+                            n = xw & ~s & words_valid(p, w, n);
+                            if (nz(n)) {
+                                words_or(x.act_lds + w, n);
+                                *(u64x2 *)(p.seen + idx) = s | n;
+                            }
+                        }
+                        *(u64x2 *)(nxt + idx) = n;  // every word: the slot is not cleared
+                    }
+                    any |= nz(n);
+                    my_bits += popc(n);
+                }
+                const u64 bal = __ballot(any);
+                bool vany = false;
+#pragma unroll
+                for (uint32_t k = 0; k < TW; ++k) vany |= ((bal >> (k * TV + x.vi)) & 1ull) != 0;
+                const bool app = x.wi == 0 && okv && vany;
+                qa.push(app, h, &c->next_count, qnext);
+                if (app) {
+                    mark_frontier(fb_next, h);
+                    if (nd < p.defer) p.lv[h] |= 1u << nd;
+                    my_deg += p.ro[h + 1] - p.ro[h];
+                }
+            }
+            qa.flush(&c->next_count, qnext);
+            return;
+        }
+    }
     if (!x.direct) {
         // Distances deferred: tiles of TV heavy vertices x TW words as in the
         // push filter, so a vertex's words are read and written as one
