@@ -127,6 +127,7 @@ struct msbfs_graph {
     uint32_t n_hchunk_out = 0, n_hchunk_in = 0;
     // uint4 (v, e0, e1, 0): vertices with 1..kHeavyDegree in-edges, increasing v
     uint32_t *light_in = nullptr;
+    uint32_t *light_in_deg = nullptr;  // the same items by decreasing in-degree
     uint32_t n_light_in = 0;
     uint64_t bytes = 0;
     // Most BFS levels a deep traversal with one source per task has needed
