@@ -99,7 +99,8 @@ struct WideParams {
 
 // Degree above which the wide family gives a vertex's edge list to a whole
 // wave instead of one lane (R-MAT scale 20, ms at 2,048 / 64 sources: 32
-// 5.87 / 1.45, 64 5.50 / 1.25, 128 5.51 / 1.24, 256 5.49 / 1.20).
+// 5.87 / 1.45, 64 5.50 / 1.25, 128 5.51 / 1.24, 256 5.49 / 1.20; with the
+// degree-sorted light list: 128 5.49 / 1.25, 256 5.17 / 1.20, 512 5.03 / 1.53).
 constexpr uint32_t kHeavyDegree = 256;
 // Edges per work item when a heavy vertex's list is split across waves:
 // in-lists (pull; R-MAT scale 20, 2,048 sources, ms: 2048 5.99, 1024 5.66,
