@@ -851,7 +851,9 @@ hipError_t deep_max_blocks_per_cu(int bits, int ell, int threads, size_t lds, in
 // State is vertex-major: the W source words of vertex v are the contiguous
 // run [v*W, v*W + W), so a parent's whole frontier (W*8 bytes, 256 B at
 // S = 2048) is one coalesced read by the TW lanes that share a vertex.
-// Wave tile: TV = 64/TW vertices x TW words (lane = wi*TV + vi).
+// Wave tile: TV = 64/TW vertices x TW lanes (lane = wi*TV + vi), one word
+// per lane, or two adjacent words per lane in deferred pull levels when the
+// parameters ask for it (WideParams::pair).
 
 __device__ __forceinline__ u64 word_valid(uint64_t S, uint32_t w) {
     const uint64_t lo = (uint64_t)w * 64;
@@ -1054,7 +1056,7 @@ __device__ __forceinline__ void clear_frontier(uint32_t *fb, const uint32_t *qf,
 // candidate words (64-bit atomicOr), skipping children whose bits are all
 // seen; the first touch of a child appends it to the candidate queue.
 // Light vertices: TW lanes per vertex walk its edges together.  Heavy
-// vertices: their out-lists come in chunks of kHeavyChunk edges, one wave
+// vertices: their out-lists come in chunks of kHeavyChunkOut edges, one wave
 // per chunk, the vertex's active words one at a time, 64 edges per step.
 template <int TW>
 __device__ __forceinline__ void wide_push_expand(const WideParams &p, const WaveCtx &x, u64 *cur,

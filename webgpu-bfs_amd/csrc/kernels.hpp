@@ -88,7 +88,7 @@ struct WideParams {
     uint32_t n_hchunk_out, n_hchunk_in;
     const uint4 *light_in;  // (vertex, first in-edge, end, 0) of in-degree 1..heavy_t, increasing
     uint32_t n_light_in;
-    uint32_t pair;  // deferred pull levels: two adjacent source words per lane (W even, W <= 2 TW)
+    uint32_t pair;  // deferred pull levels: two adjacent source words per lane (W even, TW >= 8)
     uint32_t pull_nb;       // vertices per workgroup block in pull levels
     const unsigned long long *zrow;  // >= W zero words: gather target of parents not in the frontier
     unsigned long long *act;  // ring of 3 x W words: sources whose frontier is not empty
