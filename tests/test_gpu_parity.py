@@ -504,3 +504,23 @@ def test_two_words_per_lane_pull(ctx):
                 assert np.array_equal(e.run_ms_bfs(g, src), d32), (gname, S, tune)
             e = gb.GpuBfsEngine(ctx, M.BitParallel64Pull)
             assert np.array_equal(e.run_ms_bfs(g, src, dist_dtype=np.uint16), to_u16_expect(d32))
+
+
+def test_deep_reachability_rows(ctx):
+    """Deep-family reachability: each task writes its sources' rows of a
+    source-major bitmap, transposed into the vertex-major words afterwards;
+    both deep kernels (64 threads: the one-wave kernel), several task
+    widths, S not a multiple of 32 and |V| not a multiple of 64.  Bar:
+    bit-exact against the oracle's reachability."""
+    g = gb.generate_road_like(20_000, 160, 900, seed=3)
+    assert g.num_vertices % 64 != 0
+    for S in (70, 300):
+        src = gb.sample_sources(g, S, seed=S)
+        expect = orc.reachability_from_dist(g.num_vertices, S, oracle(g, src))
+        for bits, threads in ((1, 64), (1, 256), (4, 64), (8, 1024)):
+            e = gb.GpuBfsEngine(ctx, M.WorkgroupPersistentBfs)
+            e.set_tuning(deep_bits=bits, deep_threads=threads)
+            say("deep reach", S, bits, threads)
+            bm = e.run_ms_reachability(g, src)
+            assert e.last_stats()["family"] == int(gb.Family.DEEP)
+            assert np.array_equal(bm.words, expect), (S, bits, threads)

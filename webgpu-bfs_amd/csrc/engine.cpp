@@ -740,12 +740,21 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
         p.lds_vis_words = plan.vis_words;
         p.work = e->work.as<uint32_t>();
         p.flags = e->flags.as<uint32_t>();
-        p.reach = reach;
-        p.reach_wpv = reach_wpv;
+        // reachability: each task writes its sources' rows of a source-major
+        // bitmap, transposed into the vertex-major words afterwards
+        const uint32_t vw_n = (uint32_t)((V + 31) / 32);
+        if (reach) {
+            if ((rc = ensure_plain(e->reach_t, (size_t)S * vw_n * 4))) return rc;
+            p.reach_t = e->reach_t.as<uint32_t>();
+            p.reach_vw = vw_n;
+        }
         p.ntasks = (uint32_t)ntasks;
         p.out_kind = dist_kind(dtype);
         HIP_TRY(hipEventRecord(e->ev[1], st));
         HIP_TRY(launch_deep(B, plan.ell, p, blocks, threads, plan.lds, st));
+        if (reach)
+            HIP_TRY(launch_reach_transpose(e->reach_t.as<uint32_t>(), vw_n, S, (uint32_t)V, reach,
+                                           reach_wpv, st));
         HIP_TRY(hipEventRecord(e->ev[2], st));
         e->last.instances = (uint32_t)ntasks;
         e->last.bits_per_instance = (uint32_t)B;

@@ -150,7 +150,7 @@ struct msbfs_engine {
     // deep workspace
     msbfs::DevBuf queues, tasks, graphs, work;
     // shared
-    msbfs::DevBuf flags, sources, out_tmp, out_pack, reach_tmp, trace;
+    msbfs::DevBuf flags, sources, out_tmp, out_pack, reach_tmp, reach_t, trace;
     // host-output path: pinned staging chunks and their copy-done events
     msbfs::PinnedBuf h_stage;
     hipEvent_t stage_ev[3] = {nullptr, nullptr, nullptr};

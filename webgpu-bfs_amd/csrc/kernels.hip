@@ -555,14 +555,18 @@ __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
                         store_sentinel(p.out, KIND, row + v);
             }
         }
-        if (p.reach) {
-            // Sources of a task occupy B consecutive bits that never straddle
-            // a 32-bit word (tasks start at multiples of B).
-            const uint32_t word = (uint32_t)(task.reach_bit / 32), shift = task.reach_bit % 32;
-            const uint32_t valid = (task.nsrc == 32) ? 0xFFFFFFFFu : ((1u << task.nsrc) - 1u);
-            for (uint32_t v = tid; v < V; v += nth) {
-                const uint32_t bits = (vis[v / VPW] >> ((v % VPW) * B)) & valid;
-                if (bits) atomicOr(&p.reach[(uint64_t)v * p.reach_wpv + word], bits << shift);
+        if (p.reach_t) {
+            // each source's row of the source-major bitmap, 64 vertices per
+            // ballot and wave (no atomics: the row belongs to this task)
+            const uint32_t lane = lane_id(), wv = tid / 64, nw = nth / 64;
+            for (uint32_t b = 0; b < task.nsrc; ++b) {
+                uint32_t *const row = p.reach_t + (task.reach_bit + b) * (uint64_t)p.reach_vw;
+                for (uint32_t v0 = wv * 64; v0 < V; v0 += nw * 64) {
+                    const uint32_t v = v0 + lane;
+                    const u64 m = __ballot(v < V && ((vis[v / VPW] >> ((v % VPW) * B + b)) & 1u));
+                    if (lane == 0) row[v0 / 32] = (uint32_t)m;
+                    if (lane == 32 && v0 + 32 < V) row[v0 / 32 + 1] = (uint32_t)(m >> 32);
+                }
             }
         }
         __syncthreads();
@@ -741,12 +745,16 @@ __global__ void __launch_bounds__(64) deep_wave_kernel(DeepParams p) {
                     if (!((vis[v / VPW] >> ((v % VPW) * B + b)) & 1u)) store_sentinel(p.out, KIND, row + v);
             }
         }
-        if (p.reach) {
-            const uint32_t word = (uint32_t)(task.reach_bit / 32), shift = task.reach_bit % 32;
-            const uint32_t valid = (task.nsrc == 32) ? 0xFFFFFFFFu : ((1u << task.nsrc) - 1u);
-            for (uint32_t v = lane; v < V; v += 64) {
-                const uint32_t bits = (vis[v / VPW] >> ((v % VPW) * B)) & valid;
-                if (bits) atomicOr(&p.reach[(uint64_t)v * p.reach_wpv + word], bits << shift);
+        if (p.reach_t) {
+            // as in the workgroup kernel: one ballot per 64 vertices
+            for (uint32_t b = 0; b < task.nsrc; ++b) {
+                uint32_t *const row = p.reach_t + (task.reach_bit + b) * (uint64_t)p.reach_vw;
+                for (uint32_t v0 = 0; v0 < V; v0 += 64) {
+                    const uint32_t v = v0 + lane;
+                    const u64 m = __ballot(v < V && ((vis[v / VPW] >> ((v % VPW) * B + b)) & 1u));
+                    if (lane == 0) row[v0 / 32] = (uint32_t)m;
+                    if (lane == 32 && v0 + 32 < V) row[v0 / 32 + 1] = (uint32_t)(m >> 32);
+                }
             }
         }
     }
@@ -984,6 +992,42 @@ __global__ void wide_seed_rows_kernel(WideParams p) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
          i += (uint64_t)gridDim.x * blockDim.x)
         p.fr[(uint64_t)p.sources[i / p.W] * p.W + i % p.W] = 0;
+}
+
+// Source-major reachability rows (reach_vw words per source) to the
+// vertex-major words reach[v * wpv + s / 32]: one wave per (pair of row
+// words, source word); lane j holds row word (2 vw2 + j / 32) of source
+// 32 w + j % 32, and a ballot per bit position k collects the 32 sources of
+// vertices 64 vw2 + k and 64 vw2 + 32 + k.
+__global__ void reach_transpose_kernel(const uint32_t *rt, uint32_t vw_n, uint64_t S, uint32_t V,
+                                       uint32_t *reach, uint32_t wpv) {
+    const uint64_t gw = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / 64;
+    const uint32_t lane = lane_id();
+    const uint64_t nvw2 = (vw_n + 1) / 2;
+    if (gw >= nvw2 * wpv) return;  // wave-uniform
+    const uint32_t w = (uint32_t)(gw % wpv), vw2 = (uint32_t)(gw / wpv);
+    const uint64_t s = (uint64_t)w * 32 + (lane & 31);
+    const uint32_t vw = 2 * vw2 + lane / 32;
+    const uint32_t word = (s < S && vw < vw_n) ? rt[s * vw_n + vw] : 0u;
+    uint32_t mine = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < 32; ++k) {
+        const u64 m = __ballot((word >> k) & 1u);
+        if (lane == k) mine = (uint32_t)m;
+        if (lane == 32 + k) mine = (uint32_t)(m >> 32);
+    }
+    const uint32_t v = vw2 * 64 + lane;
+    if (v < V) reach[(uint64_t)v * wpv + w] = mine;
+}
+
+hipError_t launch_reach_transpose(const uint32_t *rt, uint32_t vw_n, uint64_t S, uint32_t V,
+                                  uint32_t *reach, uint32_t wpv, hipStream_t st) {
+    const uint64_t waves = (uint64_t)((vw_n + 1) / 2) * wpv;
+    const uint64_t blocks = (waves + 3) / 4;
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(reach_transpose_kernel, dim3((unsigned)blocks), dim3(256), 0, st, rt, vw_n,
+                       S, V, reach, wpv);
+    return hipGetLastError();
 }
 
 hipError_t launch_wide_seed(const WideParams &p, hipStream_t st) {

@@ -38,8 +38,10 @@ struct DeepParams {
     uint32_t lds_vis_words;  // LDS words reserved for visited bits
     uint32_t *work;        // [0]: next task; zeroed before launch
     uint32_t *flags;       // see Flag* below
-    uint32_t *reach;       // reachability words or null
+    uint32_t *reach;       // unused by the deep kernels (see reach_t)
     uint32_t reach_wpv;
+    uint32_t *reach_t;     // reachability, source-major: a row of reach_vw words per source, or null
+    uint32_t reach_vw;     // words per row: ceil(|V| / 32)
     uint32_t ntasks;
     uint32_t out_kind;      // output elements: 0 u32, 1 u16, 2 u8
     uint32_t pad;
@@ -137,6 +139,8 @@ hipError_t launch_wide(int tile_words, const WideParams &p, int blocks, int thre
                        hipStream_t st);
 hipError_t wide_max_blocks_per_cu(int tile_words, int threads, size_t lds, int *out);
 hipError_t launch_wide_seed(const WideParams &p, hipStream_t st);
+hipError_t launch_reach_transpose(const uint32_t *rt, uint32_t vw_n, uint64_t S, uint32_t V,
+                                  uint32_t *reach, uint32_t wpv, hipStream_t st);
 // Final pass of a wide traversal (distances of the kept levels, sentinels,
 // reachability words, workspace clean-up); launched after launch_wide.
 hipError_t launch_wide_final(const WideParams &p, int blocks, hipStream_t st);
