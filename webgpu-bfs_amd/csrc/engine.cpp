@@ -711,6 +711,14 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
         if ((rc = ensure_plain(e->graphs, sizeof(DeepGraph)))) return rc;
         if ((rc = ensure_plain(e->work, 16))) return rc;
         if ((rc = e->h_tasks.ensure(ntasks * sizeof(DeepTask) + sizeof(DeepGraph))) < 0) return -rc;
+        // The task table and the graph record depend only on these values;
+        // when they match the last upload, the device copies are current.
+        const uint64_t key[10] = {S, (uint64_t)B, V, ntasks, (uint64_t)(uintptr_t)g->ro,
+                                  (uint64_t)(uintptr_t)g->ci, (uint64_t)(uintptr_t)g->ell,
+                                  (uint64_t)plan.ell, (uint64_t)(uintptr_t)e->tasks.p,
+                                  (uint64_t)(uintptr_t)e->graphs.p};
+        const bool upload = std::memcmp(key, e->tasks_key, sizeof(key)) != 0;
+        if (upload) {
         // The pinned staging buffer may still be read by a previous enqueue.
         HIP_TRY(hipStreamSynchronize(st));
         auto *ht = (DeepTask *)e->h_tasks.p;
@@ -729,6 +737,8 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
         hg->ell_width = (uint32_t)plan.ell;
         HIP_TRY(hipMemcpyAsync(e->tasks.p, ht, ntasks * sizeof(DeepTask), hipMemcpyHostToDevice, st));
         HIP_TRY(hipMemcpyAsync(e->graphs.p, hg, sizeof(DeepGraph), hipMemcpyHostToDevice, st));
+        std::memcpy(e->tasks_key, key, sizeof(key));
+        }
         HIP_TRY(launch_zero_u32(e->work.as<uint32_t>(), 4, st));
 
         DeepParams p{};

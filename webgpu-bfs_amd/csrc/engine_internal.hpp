@@ -157,6 +157,8 @@ struct msbfs_engine {
     uint32_t trace_cap = 0;
     bool trace_valid = false;
     msbfs::PinnedBuf h_sources, h_flags, h_tasks;
+    // what the device task table and graph record hold (skip re-uploading them)
+    uint64_t tasks_key[10] = {};
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     msbfs_run_stats last{};
     hipStream_t last_stream = nullptr;
