@@ -546,13 +546,21 @@ __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
         if (tid == 0) atomicMax(&p.flags[FlagLevels], level);
 
         // Unreached (source, vertex) pairs get the sentinel; rows are written
-        // by consecutive threads, so the stores coalesce.
+        // by consecutive threads, so the stores coalesce.  A wave first tests
+        // 64 visited words at once and skips their vertices when every bit is
+        // set (a connected graph has nothing to write).
         if constexpr (OUT) {
-            for (uint32_t b = 0; b < task.nsrc; ++b) {
-                const uint64_t row = task.out_off + (uint64_t)b * V;
-                for (uint32_t v = tid; v < V; v += nth)
-                    if (!((vis[v / VPW] >> ((v % VPW) * B + b)) & 1u))
-                        store_sentinel(p.out, KIND, row + v);
+            const uint32_t lane = lane_id(), wv = tid / 64, nw = nth / 64;
+            for (uint32_t i0 = wv * 64; i0 < nwords; i0 += nw * 64) {
+                const uint32_t i = i0 + lane;
+                if (__all(i >= nwords || vis[i] == 0xFFFFFFFFu)) continue;
+                const uint32_t v1 = min(V, (i0 + 64) * VPW);
+                for (uint32_t b = 0; b < task.nsrc; ++b) {
+                    const uint64_t row = task.out_off + (uint64_t)b * V;
+                    for (uint32_t v = i0 * VPW + lane; v < v1; v += 64)
+                        if (!((vis[v / VPW] >> ((v % VPW) * B + b)) & 1u))
+                            store_sentinel(p.out, KIND, row + v);
+                }
             }
         }
         if (p.reach_t) {
@@ -739,10 +747,17 @@ __global__ void __launch_bounds__(64) deep_wave_kernel(DeepParams p) {
         }
         if (lane == 0) atomicMax(&p.flags[FlagLevels], level);
         if constexpr (OUT) {
-            for (uint32_t b = 0; b < task.nsrc; ++b) {
-                const uint64_t row = task.out_off + (uint64_t)b * V;
-                for (uint32_t v = lane; v < V; v += 64)
-                    if (!((vis[v / VPW] >> ((v % VPW) * B + b)) & 1u)) store_sentinel(p.out, KIND, row + v);
+            // sentinels for unreached pairs, skipping runs of 64 full visited words
+            for (uint32_t i0 = 0; i0 < nwords; i0 += 64) {
+                const uint32_t i = i0 + lane;
+                if (__all(i >= nwords || vis[i] == 0xFFFFFFFFu)) continue;
+                const uint32_t v1 = min(V, (i0 + 64) * VPW);
+                for (uint32_t b = 0; b < task.nsrc; ++b) {
+                    const uint64_t row = task.out_off + (uint64_t)b * V;
+                    for (uint32_t v = i0 * VPW + lane; v < v1; v += 64)
+                        if (!((vis[v / VPW] >> ((v % VPW) * B + b)) & 1u))
+                            store_sentinel(p.out, KIND, row + v);
+                }
             }
         }
         if (p.reach_t) {
