@@ -7,14 +7,21 @@ output resident in HBM.  One step = one full MS-BFS of all S sources.
 
 Prints one JSON line: GTEPS (traversed edges per second, counted as the
 reference's CLI does: for every source, the out-degrees of all reached
-vertices), sources/s, the roofline of the traversal kernel and the CPU
-baseline (the oracle's parallel-over-sources restatement of the reference's
-Rayon path, timed on this host's cores).
+vertices, crates/cli/src/bench.rs:478-491), sources/s, per-step spread
+statistics (bench.rs:376-429), the roofline of the traversal kernel, the
+host-buffer path timed like the reference's CLI (wall clock around
+run_ms_bfs_into, bench.rs:336-372), the CPU baseline (the oracle's
+parallel-over-sources restatement of the reference's Rayon path on all of
+this host's allowed cores) and, for the default workload, the secondary
+configurations: R-MAT scale 20 at S=2048 with its pull-phase HBM fraction,
+the 512-graph batch against one call per graph, and the serving loop.
 """
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
+import math
 import os
 import sys
 import time
@@ -25,6 +32,7 @@ for _p in (os.path.join(ROOT, "webgpu-bfs_amd"), ROOT):
         sys.path.insert(0, _p)
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "MS-BFS GTEPS & sources/s at S=2048 on 1 GPU; served p50/p99 latency vs plain"
 
 CONFIGS = {
     # name: (graph builder, S, dist dtype, non-isolated sources)
@@ -39,6 +47,10 @@ CONFIGS = {
 }
 
 
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
 def parse_args():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -47,16 +59,40 @@ def parse_args():
     ap.add_argument("--config", default="road-s2048", choices=sorted(CONFIGS))
     ap.add_argument("--method", default="Auto")
     ap.add_argument("--tune", action="append", default=[], help="tuning knob key=value")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline threads (0: min(16, cores))")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="CPU baseline threads (0: every core this process may use)")
     ap.add_argument("--cpu-budget-s", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
+    ap.add_argument("--no-host-path", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="default workload only: skip the R-MAT, batch and serving results")
     ap.add_argument("--serve-queries", type=int, default=300)
-    ap.add_argument("--serve-rates", default="2000,20000", help="Poisson arrival rates (queries/s)")
+    ap.add_argument("--serve-rates", default="",
+                    help="Poisson arrival rates in queries/s (default: 25/50/75%% of the "
+                         "measured packed capacity, then 2,000 and 20,000)")
     ap.add_argument("--serve-rmat-scale", type=int, default=22)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain device copy of 256 MiB (counter calibration)")
     return ap.parse_args()
+
+
+# ------------------------------------------------------------------ statistics
+
+def run_stats(durations_ms, te, S):
+    """Mean, median, min, max and sample standard deviation of step times,
+    harmonic and arithmetic GTEPS and sources/s, as bench.rs:376-429."""
+    d = sorted(durations_ms)
+    n = len(d)
+    mean = sum(d) / n
+    median = d[n // 2] if n % 2 else (d[n // 2 - 1] + d[n // 2]) / 2
+    var = sum((x - mean) ** 2 for x in d) / (n - 1) if n > 1 else 0.0
+    total_s = sum(d) / 1e3
+    return {"mean_ms": round(mean, 4), "median_ms": round(median, 4), "min_ms": round(d[0], 4),
+            "max_ms": round(d[-1], 4), "stddev_ms": round(math.sqrt(var), 4),
+            "harmonic_gteps": round(n * te / total_s / 1e9, 3) if total_s else 0.0,
+            "arithmetic_gteps": round(te / (mean / 1e3) / 1e9, 3) if mean else 0.0,
+            "sources_per_s": round(S / (mean / 1e3), 1) if mean else 0.0}
 
 
 def level_bytes(dist, S, V, E, deg, dist_bytes):
@@ -69,6 +105,7 @@ def level_bytes(dist, S, V, E, deg, dist_bytes):
       push(L) = 8 A_L + 4 EA_L + b (A_L + 2 EA_L) + b_d N_{L+1}
       pull(L) = 4 (V+1) + 4 E + b V + 2 b R_L + b_d N_{L+1}
       total   = b_d S V + sum_L min(push(L), pull(L)).
+    Returns (total, traversed edges, max distance, [(push(L), pull(L))]).
     `dist` is a torch tensor [S, V] on the device (u32 as int32 or u16 as
     int16)."""
     import torch
@@ -106,13 +143,15 @@ def level_bytes(dist, S, V, E, deg, dist_bytes):
     b = 8 * W
     total = dist_bytes * S * V
     A, EA, cnt_le, hist = A.tolist(), EA.tolist(), cnt_le.tolist(), hist.tolist()
+    per_level = []
     for L in range(maxl + 1):
         R = V - cnt_le[L]
         N = hist[L + 1] if L + 1 <= maxl else 0
         push = 8 * A[L] + 4 * EA[L] + b * (A[L] + 2 * EA[L]) + dist_bytes * N
         pull = 4 * (V + 1) + 4 * E + b * V + 2 * b * R + dist_bytes * N
+        per_level.append((push, pull))
         total += min(push, pull)
-    return total, te, maxl
+    return total, te, maxl, per_level
 
 
 def calibrate_copy():
@@ -128,7 +167,227 @@ def calibrate_copy():
                       "launches": 3}))
 
 
-def bench_batch(args):
+def traffic_of(config):
+    """HBM bytes per launch of this workload's dominant kernels from the
+    committed rocprofv3 counter summary, with where they came from."""
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{config}.json")
+    if not os.path.exists(tfile):
+        return None, None
+    with open(tfile) as f:
+        t = json.load(f)
+    src = (f"profiles/traffic_{config}.json: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of "
+           f"{t.get('kernels', 'the traversal kernels')} at commit {t.get('commit', '?')}")
+    return t.get("hbm_bytes_per_launch"), src
+
+
+def cpu_threads(args):
+    return args.cpu_threads or len(os.sched_getaffinity(0))
+
+
+# ------------------------------------------------------------------ one workload
+
+def measure(args, config, steps, warmup, with_cpu, with_host, with_trace=False):
+    """Device-resident timing of one configuration, its roofline, and
+    optionally the host-buffer path, the CPU baseline and a per-level trace
+    (wide family) for the pull-phase fraction."""
+    import numpy as np
+    import torch
+
+    import gpu_bfs as gb
+    from gpu_bfs import _lib as L
+    from gpu_bfs import workloads as wl
+
+    gname, S, dname, non_iso = CONFIGS[config]
+    t0 = time.time()
+    graph = getattr(wl, gname)()
+    src = wl.sources(graph, S, non_isolated=non_iso)
+    V, E = graph.num_vertices, graph.num_edges
+    log(f"workload {config}: V={V} E={E} S={S} dist={dname} (built in {time.time() - t0:.1f}s)")
+
+    ctx = gb.GpuContext(0)
+    eng = gb.GpuBfsEngine(ctx, gb.GpuBfsMethod[args.method])
+    if args.tune:
+        eng.set_tuning(**{k: int(v) for k, v in (t.split("=") for t in args.tune)})
+    dg = gb.DeviceGraph(ctx, graph)
+    n = S * V
+    tdt = torch.int32 if dname == "u32" else torch.int16
+    npdt = np.uint32 if dname == "u32" else np.uint16
+    out = torch.empty(n, dtype=tdt, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def step(sync):
+        eng.run_ms_bfs_device(dg, src, out.data_ptr(), n, dist_dtype=npdt, stream=stream,
+                              sync=sync)
+
+    for _ in range(warmup):
+        step(True)
+    torch.cuda.synchronize()
+    kernel_ms, device_ms = [], []
+    t_start = time.perf_counter()
+    for _ in range(steps):
+        step(False)
+        st = eng.last_stats()
+        kernel_ms.append(st["kernel_ms"])
+        device_ms.append(st["total_ms"])
+    torch.cuda.synchronize()
+    ms_per_step = (time.perf_counter() - t_start) * 1e3 / steps
+    stats = eng.last_stats()
+    log(f"{config}: timed {steps} steps: {ms_per_step:.3f} ms/step")
+
+    deg = graph.degrees()
+    dist2d = out.view(S, V)
+    b_d = 4 if dname == "u32" else 2
+    model_bytes, te, maxl, per_level = level_bytes(dist2d, S, V, E, deg, b_d)
+    md = int(deg.max()) if V else 0
+    ell = 4 if md <= 4 else (8 if md <= 8 else 0)
+    # Compulsory HBM bytes of one launch: every distance written once, the
+    # graph (CSR, plus the neighbour table of the deep family's one-wave
+    # kernel when the maximum degree is <= 8) and the sources read once.
+    compulsory = b_d * S * V + 4 * (V + 1) + 4 * E + 4 * ell * V + 4 * S
+    deep = stats["family"] == 1
+    # The deep family keeps all per-source state in LDS, so only the
+    # compulsory bytes reach HBM; the wide family keeps source words in HBM
+    # and is measured against the per-level model.
+    alg_bytes = compulsory if deep else model_bytes
+    kmean = float(np.mean(kernel_ms))
+    achieved = alg_bytes / (kmean / 1e3) / 1e9
+    traffic, traffic_src = traffic_of(config)
+    if deep:
+        kernel = ("deep_wave_kernel" if stats["block_threads"] == 64 else "deep_bfs_kernel")
+    else:
+        kernel = "wide_bfs_kernel + wide_final_kernel"
+    res = {
+        "ms_per_step": round(ms_per_step, 4),
+        "gteps": round(te / (ms_per_step / 1e3) / 1e9, 3),
+        "sources_per_s": round(S / (ms_per_step / 1e3), 1),
+        "traversed_edges": te,
+        "kernel_ms": round(kmean, 4),
+        "step_device_stats": run_stats(device_ms, te, S),
+        "config": {"workload": config, "graph": gname, "num_vertices": V, "num_edges": E,
+                   "num_sources": S, "dist": dname, "method": args.method,
+                   "family": ["none", "deep", "wide"][stats["family"]],
+                   "levels": int(stats["levels"]), "max_distance": maxl,
+                   "push_levels": int(stats["push_levels"]),
+                   "pull_levels": int(stats["pull_levels"]),
+                   "bits_per_instance": int(stats["bits_per_instance"]),
+                   "grid_blocks": int(stats["grid_blocks"]),
+                   "block_threads": int(stats["block_threads"]), "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": traffic, "traffic_source": traffic_src,
+                     "algorithmic_bytes": alg_bytes,
+                     "byte_model": "compulsory" if deep else "level",
+                     "compulsory_bytes": compulsory, "level_model_bytes": model_bytes,
+                     "kernel": kernel},
+    }
+    if deep and stats["levels"]:
+        # the deep family is bound by its dependent levels, not by bandwidth
+        res["roofline"]["latency"] = {
+            "levels": int(stats["levels"]),
+            "us_per_level": round(kmean * 1e3 / stats["levels"], 4)}
+
+    if with_trace and not deep:
+        # one traced run: per-level device clock, direction and frontier
+        eng.set_tuning(trace_levels=max(64, maxl + 2))
+        step(True)
+        recs = eng.level_trace()
+        eng.set_tuning(trace_levels=0)
+        pull_b = pull_us = 0.0
+        rows = []
+        for r in recs:
+            L_ = r["level"]
+            if L_ >= len(per_level):
+                continue
+            push_b, pull_bytes = per_level[L_]
+            us = r["end_us"] - r["start_us"]
+            dirn = "pull" if r["direction"] == 1 else "push"
+            b = pull_bytes if dirn == "pull" else push_b
+            rows.append({"level": L_, "dir": dirn, "us": round(us, 1),
+                         "model_gb": round(b / 1e9, 3),
+                         "frac": round(b / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 3) if us > 0 else None})
+            if dirn == "pull":
+                pull_b += b
+                pull_us += us
+        res["levels_trace"] = rows
+        if pull_us > 0:
+            res["pull_phase"] = {"model_bytes": int(pull_b), "us": round(pull_us, 1),
+                                 "achieved_gbs": round(pull_b / (pull_us * 1e-6) / 1e9, 1),
+                                 "frac": round(pull_b / (pull_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)}
+
+    if with_host:
+        # The reference's own call shape: run_ms_bfs_into a reusable host
+        # buffer, wall clock around the whole call (bench.rs:336-372).
+        host = np.empty(n, dtype=np.uint32)
+        srcc = np.ascontiguousarray(src, dtype=np.uint32)
+        lib = L.lib()
+
+        def host_call():
+            L.check(lib.msbfs_run_ms_bfs(eng.handle, dg.handle,
+                                         srcc.ctypes.data_as(ctypes.c_void_p), S, L.DIST_U32,
+                                         host.ctypes.data_as(ctypes.c_void_p), n))
+
+        host_call()
+        hd = []
+        for _ in range(3):
+            t = time.perf_counter()
+            host_call()
+            hd.append((time.perf_counter() - t) * 1e3)
+        res["host_path"] = dict(run_stats(hd, te, S), iters=3, warmup=1,
+                                call="msbfs_run_ms_bfs into a pageable u32 host array")
+        res["host_path"]["gteps"] = res["host_path"]["harmonic_gteps"]
+        log(f"{config}: host path {res['host_path']['mean_ms']:.2f} ms")
+        if not args.no_parity:
+            dv = out.cpu().numpy().view(npdt).astype(np.uint32)
+            if npdt == np.uint16:
+                dv = np.where(dv == 0xFFFF, 0xFFFFFFFF, dv).astype(np.uint32)
+            res["host_path"]["identical_to_device"] = bool(np.array_equal(host, dv))
+
+    if with_cpu:
+        from oracle import oracle as orc
+
+        threads = cpu_threads(args)
+        # Bounded sample: as many sources as fit the time budget, estimated
+        # from a 16-source probe.
+        t = time.perf_counter()
+        orc.ms_bfs_parallel(V, graph.row_offsets, graph.col_indices, src[:16], threads)
+        per_src = (time.perf_counter() - t) / 16
+        ns = int(min(S, max(16, args.cpu_budget_s / 4 / max(per_src, 1e-9))))
+        sample = src[:ns]
+        times = []
+        cpu_out = None
+        deadline = time.perf_counter() + args.cpu_budget_s
+        while len(times) < 3 and (not times or time.perf_counter() < deadline):
+            t = time.perf_counter()
+            cpu_out = orc.ms_bfs_parallel(V, graph.row_offsets, graph.col_indices, sample, threads)
+            times.append(time.perf_counter() - t)
+        cpu_s = float(np.median(times))
+        cpu_te = orc.traversed_edges(V, graph.row_offsets, ns, cpu_out)
+        res["cpu_baseline"] = {
+            "value": round(cpu_te / cpu_s / 1e9, 4), "unit": "GTEPS", "cores": threads,
+            "kind": "port",
+            "sample": f"{ns} of {S} sources, {len(times)} timed runs (median {cpu_s*1e3:.1f} ms), "
+                      "parallel-over-sources FIFO BFS incl. output allocation and fill",
+            "sources_per_s": round(ns / cpu_s, 1),
+            "threads_used": threads, "affinity_cpus": len(os.sched_getaffinity(0)),
+            "os_cpu_count": os.cpu_count()}
+        if "host_path" in res:
+            res["host_path"]["vs_cpu_baseline"] = round(
+                res["host_path"]["harmonic_gteps"] / res["cpu_baseline"]["value"], 2)
+        if not args.no_parity:
+            gpu = dist2d[:ns].cpu().numpy().view(npdt).reshape(-1)
+            ref = cpu_out.astype(np.uint64)
+            if npdt == np.uint16:
+                ref = np.where(cpu_out == 0xFFFFFFFF, 0xFFFF, cpu_out).astype(np.uint16)
+            res["parity"] = bool(np.array_equal(gpu, ref.astype(npdt)))
+            res["parity_sources_checked"] = ns
+    del out
+    torch.cuda.empty_cache()
+    return res
+
+
+# ------------------------------------------------------------------ batch
+
+def bench_batch(args, steps, warmup):
     """512 G(n,p) graphs of uneven size, each with its own source set, in one
     persistent hipGraph-replayed launch, against one engine call per graph."""
     import numpy as np
@@ -141,9 +400,9 @@ def bench_batch(args):
     pairs = wl.batch512()
     graphs = [g for g, _ in pairs]
     sources = [s for _, s in pairs]
-    print(f"[bench] batch512: {len(graphs)} graphs, V={sum(g.num_vertices for g in graphs)} "
-          f"E={sum(g.num_edges for g in graphs)} S={sum(len(s) for s in sources)} "
-          f"(built in {time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+    log(f"batch512: {len(graphs)} graphs, V={sum(g.num_vertices for g in graphs)} "
+        f"E={sum(g.num_edges for g in graphs)} S={sum(len(s) for s in sources)} "
+        f"(built in {time.time() - t0:.1f}s)")
     ctx = gb.GpuContext(0)
     eng = gb.GpuBfsEngine(ctx, gb.GpuBfsMethod.Auto)
     dgs = [gb.DeviceGraph(ctx, g) for g in graphs]
@@ -151,22 +410,20 @@ def bench_batch(args):
     out = torch.empty(b.total, dtype=torch.int32, device="cuda")
     stream = torch.cuda.current_stream().cuda_stream
     tuned, tuned_ms = b.autotune(out.data_ptr(), b.total)
-    print(f"[bench] autotune: {tuned['deep_bits']} sources/task, {tuned['deep_threads']} threads, "
-          f"{tuned_ms:.3f} ms", file=sys.stderr, flush=True)
+    log(f"batch512 autotune: {tuned} {tuned_ms:.3f} ms")
     b.capture(out.data_ptr(), b.total, stream=stream)
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         b.replay(stream=stream)
-    kernel_ms = []
     torch.cuda.synchronize()
+    kernel_ms = []
     t_start = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         b.replay(stream=stream)
         kernel_ms.append(b.stats()["kernel_ms"])
     torch.cuda.synchronize()
-    ms_per_step = (time.perf_counter() - t_start) * 1e3 / args.steps
+    ms_per_step = (time.perf_counter() - t_start) * 1e3 / steps
     st = b.stats()
 
-    # traversed edges per graph, on the device
     te = 0
     for g, (off, S, V) in enumerate(zip(b.offsets, b.counts, b.num_vertices)):
         if S == 0:
@@ -175,51 +432,56 @@ def bench_batch(args):
         deg = torch.as_tensor(graphs[g].degrees(), device="cuda")
         te += int(((d != -1).to(torch.int64) * deg[None, :]).sum())
 
-    # plain path: one engine call per graph, same outputs
-    plain = torch.empty_like(out)
+    # plain path: one engine call per graph into the same layout
+    plain = torch.full_like(out, -5)
     eng_plain = gb.GpuBfsEngine(ctx, gb.GpuBfsMethod.Auto)
-    torch.cuda.synchronize()
-    for g in range(len(dgs)):  # warm every per-graph workspace once
-        if len(sources[g]):
-            eng_plain.run_ms_bfs_device(dgs[g], sources[g], plain.data_ptr() + 4 * int(b.offsets[g]),
-                                        int(b.counts[g]) * b.num_vertices[g], stream=stream)
+
+    def plain_all():
+        for g in range(len(dgs)):
+            if len(sources[g]):
+                eng_plain.run_ms_bfs_device(dgs[g], sources[g],
+                                            plain.data_ptr() + 4 * int(b.offsets[g]),
+                                            int(b.counts[g]) * b.num_vertices[g], stream=stream,
+                                            sync=False)
+        eng_plain.check()
+
+    plain_all()  # warm every per-graph workspace once
     torch.cuda.synchronize()
     t = time.perf_counter()
-    for g in range(len(dgs)):
-        if len(sources[g]):
-            eng_plain.run_ms_bfs_device(dgs[g], sources[g], plain.data_ptr() + 4 * int(b.offsets[g]),
-                                        int(b.counts[g]) * b.num_vertices[g], stream=stream)
+    plain_all()
     torch.cuda.synchronize()
     plain_ms = (time.perf_counter() - t) * 1e3
     same = bool(torch.equal(plain, out))
+    res = {"ms_per_step": round(ms_per_step, 4),
+           "gteps": round(te / (ms_per_step / 1e3) / 1e9, 3),
+           "kernel_ms": round(float(np.mean(kernel_ms)), 4),
+           "traversed_edges": te,
+           "plain_per_graph_ms": round(plain_ms, 3),
+           "plain_gteps": round(te / (plain_ms / 1e3) / 1e9, 3),
+           "speedup_vs_plain": round(plain_ms / ms_per_step, 2),
+           "identical_to_plain": same,
+           "autotune": {"chosen": tuned, "ms": round(tuned_ms, 4)},
+           "config": {"workload": "batch512", "graphs": len(graphs),
+                      "num_vertices": int(sum(g.num_vertices for g in graphs)),
+                      "num_edges": int(sum(g.num_edges for g in graphs)),
+                      "num_sources": int(sum(len(s) for s in sources)),
+                      "tasks": int(st["instances"]), "sources_per_task": int(st["bits_per_instance"]),
+                      "threads": int(st["block_threads"]), "grid_blocks": int(st["grid_blocks"]),
+                      "hipgraph": True}}
+    del out, plain
+    torch.cuda.empty_cache()
+    return res
 
-    result = {
-        "metric": "batched MS-BFS: per-batch wall time and GTEPS vs one launch per graph",
-        "value": round(te / (ms_per_step / 1e3) / 1e9, 3), "unit": "GTEPS", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "u32 distances", "data": "synthetic (512 seeded G(n,p) graphs)",
-        "config": {"workload": "batch512", "graphs": len(graphs),
-                   "num_vertices": int(sum(g.num_vertices for g in graphs)),
-                   "num_edges": int(sum(g.num_edges for g in graphs)),
-                   "num_sources": int(sum(len(s) for s in sources)),
-                   "tasks": int(st["instances"]), "sources_per_task": int(st["bits_per_instance"]),
-                   "threads": int(tuned["deep_threads"]), "grid_blocks": int(st["grid_blocks"]),
-                   "hipgraph": True},
-        "kernel_ms": round(float(np.mean(kernel_ms)), 4),
-        "traversed_edges": te,
-        "plain_per_graph_ms": round(plain_ms, 3),
-        "speedup_vs_plain": round(plain_ms / ms_per_step, 2),
-        "identical_to_plain": same,
-    }
-    print(json.dumps(result))
 
+# ------------------------------------------------------------------ serving
 
 def bench_serve(args):
     """Serving: the road-like graph and an R-MAT graph (scale 22) resident;
     Poisson arrivals, each query picks a graph 50/50 and brings k
     log-uniform in [1, 512] sources; u16 distances kept on the device.
-    Packed continuous batching against one query per traversal."""
+    Packed continuous batching against one query per traversal, first at
+    saturation (all queries at once) to measure capacity, then at offered
+    rates below and above it."""
     import threading
 
     import numpy as np
@@ -231,9 +493,9 @@ def bench_serve(args):
 
     t0 = time.time()
     graphs = [wl.road(), wl.rmat(args.serve_rmat_scale)]
-    print(f"[bench] serve: road V={graphs[0].num_vertices} E={graphs[0].num_edges}; "
-          f"rmat{args.serve_rmat_scale} V={graphs[1].num_vertices} E={graphs[1].num_edges} "
-          f"(built in {time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+    log(f"serve: road V={graphs[0].num_vertices} E={graphs[0].num_edges}; "
+        f"rmat{args.serve_rmat_scale} V={graphs[1].num_vertices} E={graphs[1].num_edges} "
+        f"(built in {time.time() - t0:.1f}s)")
     # traversed edges of a source = out-degree sum of its connected component
     comp_te = []
     for g in graphs:
@@ -254,28 +516,20 @@ def bench_serve(args):
     te_q = [int(comp_te[lane][src].sum()) for lane, src in queries]
     ctx = gb.GpuContext(0)
     dgs = [gb.DeviceGraph(ctx, g) for g in graphs]
-
-    # Output pool: room for a full 2048-source launch on the larger graph
-    # plus the same again for results not yet released.
     pool = 2 * 2 * 2048 * max(g.num_vertices for g in graphs)
-    print(f"[bench] serve: {n} queries, output pool {pool / 2**30:.1f} GiB", file=sys.stderr,
-          flush=True)
 
     def run(rate, pack):
-        print(f"[bench] serve: rate {rate:.0f} q/s, pack={pack}", file=sys.stderr, flush=True)
         srv = gb.Server(ctx, pool_bytes=pool, dist_dtype=np.uint16, pack=pack)
         for dg in dgs:
             srv.add_graph(dg)
         srv.start()
-        # warm both lanes
-        for lane in (0, 1):
+        for lane in (0, 1):  # warm both lanes
             q = srv.submit(lane, nonzero[lane][:1])
             assert srv.next_completed(120.0) == q
             srv.release(q)
         srv.reset_stats()
-        gaps = rng.exponential(1.0 / rate, size=n)
+        gaps = rng.exponential(1.0 / rate, size=n) if rate else np.zeros(n)
         arrive = np.cumsum(gaps)
-        ids = [None] * n
 
         def client():
             t_base = time.perf_counter()
@@ -283,12 +537,10 @@ def bench_serve(args):
                 delay = arrive[i] - (time.perf_counter() - t_base)
                 if delay > 0:
                     time.sleep(delay)
-                ids[i] = srv.submit(lane, src)
+                srv.submit(lane, src)
 
         th = threading.Thread(target=client)
-        t_start = time.perf_counter()
         th.start()
-        # consume results in completion order and release them at once
         lat = []
         while len(lat) < n:
             qid = srv.next_completed(120.0)
@@ -297,205 +549,100 @@ def bench_serve(args):
                 raise RuntimeError(f"query {qid} failed with status {info['status']}")
             lat.append(info["latency_ms"])
             srv.release(qid)
-            if len(lat) % 50 == 0:
-                print(f"[bench] serve: {len(lat)}/{n} done", file=sys.stderr, flush=True)
         th.join()
-        wall = time.perf_counter() - t_start
         st = srv.stats()
         srv.stop()
         lat = np.array(lat)
         span = st["span_s"]
-        return {"offered_qps": rate, "queries": n, "launches": int(st["launches"]),
-                "p50_ms": round(float(np.percentile(lat, 50)), 3),
-                "p99_ms": round(float(np.percentile(lat, 99)), 3),
-                "achieved_qps": round(n / span, 1),
-                "sources_per_s": round(sum(len(s) for _, s in queries) / span, 1),
-                "gteps": round(sum(te_q) / span / 1e9, 3), "span_s": round(span, 3),
-                "wall_s": round(wall, 3)}
+        r = {"offered_qps": rate if rate else "saturation", "queries": n,
+             "launches": int(st["launches"]),
+             "p50_ms": round(float(np.percentile(lat, 50)), 3),
+             "p99_ms": round(float(np.percentile(lat, 99)), 3),
+             "achieved_qps": round(n / span, 1),
+             "sources_per_s": round(sum(len(s) for _, s in queries) / span, 1),
+             "gteps": round(sum(te_q) / span / 1e9, 3)}
+        log(f"serve pack={pack} rate={rate}: {r}")
+        return r
 
-    rates = [float(r) for r in args.serve_rates.split(",")]
+    packed_sat = run(0, True)
+    single_sat = run(0, False)
+    cap = packed_sat["achieved_qps"]
+    if args.serve_rates:
+        rates = [float(r) for r in args.serve_rates.split(",")]
+    else:
+        rates = [round(cap * f) for f in (0.25, 0.5, 0.75)] + [2000.0, 20000.0]
     packed = [run(r, True) for r in rates]
-    for r in packed:
-        print(f"[bench] packed  {r}", file=sys.stderr, flush=True)
     single = [run(r, False) for r in rates]
-    for r in single:
-        print(f"[bench] single  {r}", file=sys.stderr, flush=True)
-    best = max(packed, key=lambda r: r["gteps"])
-    result = {
-        "metric": "served MS-BFS: p50/p99 latency and throughput, packed vs one query per launch",
-        "value": best["gteps"], "unit": "GTEPS", "n_gpus": 1, "steps": len(rates),
-        "warmup": 1, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "u64 source words / u16 distances",
-        "data": "synthetic (road-like + R-MAT, Poisson arrivals, seeded)",
-        "config": {"workload": "serve", "graphs": ["road-like 114,599 V",
-                                                   f"rmat{args.serve_rmat_scale}"],
-                   "queries_per_rate": n, "k": "log-uniform [1, 512]",
-                   "admission": "u16 x k x |V| output bytes", "pool_bytes": pool},
-        "packed": packed, "one_query_per_launch": single,
-    }
-    print(json.dumps(result))
+    return {"capacity_qps": {"packed": cap, "one_query_per_launch": single_sat["achieved_qps"]},
+            "saturation": {"packed": packed_sat, "one_query_per_launch": single_sat},
+            "packed": packed, "one_query_per_launch": single,
+            "config": {"workload": "serve", "graphs": ["road-like 114,599 V",
+                                                       f"rmat{args.serve_rmat_scale}"],
+                       "queries_per_rate": n, "k": "log-uniform [1, 512]",
+                       "admission": "u16 x k x |V| output bytes", "pool_bytes": pool}}
+
+
+# ------------------------------------------------------------------ main
+
+def line(args, res, workload_dtype):
+    """The one JSON line of the driver's contract."""
+    out = {"metric": METRIC, "value": res["gteps"], "unit": "GTEPS", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": res["ms_per_step"],
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": workload_dtype,
+           "data": "synthetic (seeded generators; graph and distances resident in HBM)"}
+    out.update({k: v for k, v in res.items() if k not in ("gteps", "ms_per_step")})
+    return out
 
 
 def main():
     args = parse_args()
-    if args.config == "serve":
-        if args.gpus != 1:
-            print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
-            sys.exit(2)
-        bench_serve(args)
-        return
-    if args.config == "batch512":
-        if args.gpus != 1:
-            print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
-            sys.exit(2)
-        bench_batch(args)
-        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
                           "gpus_requested": args.gpus}))
         sys.exit(2)
-    import numpy as np
-    import torch
-
-    import gpu_bfs as gb
-    from gpu_bfs import workloads as wl
-
     if args.calibrate_copy:
         calibrate_copy()
         return
+    if args.config == "serve":
+        res = bench_serve(args)
+        best = max(res["packed"], key=lambda r: r["gteps"])
+        out = {"metric": "served MS-BFS: p50/p99 latency and throughput, packed vs one query "
+                         "per launch", "value": best["gteps"], "unit": "GTEPS", "n_gpus": 1,
+               "steps": len(res["packed"]), "warmup": 1, "ms_per_step": None,
+               "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+               "dtype": "u64 source words / u16 distances",
+               "data": "synthetic (road-like + R-MAT, Poisson arrivals, seeded)"}
+        out.update(res)
+        print(json.dumps(out))
+        return
+    if args.config == "batch512":
+        res = bench_batch(args, args.steps, args.warmup)
+        out = {"metric": "batched MS-BFS: per-batch wall time and GTEPS vs one launch per graph",
+               "value": res["gteps"], "unit": "GTEPS", "n_gpus": 1, "steps": args.steps,
+               "warmup": args.warmup, "ms_per_step": res["ms_per_step"],
+               "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+               "dtype": "u32 distances", "data": "synthetic (512 seeded G(n,p) graphs)"}
+        out.update({k: v for k, v in res.items() if k not in ("gteps", "ms_per_step")})
+        print(json.dumps(out))
+        return
 
-    gname, S, dname, non_iso = CONFIGS[args.config]
-    t0 = time.time()
-    graph = getattr(wl, gname)()
-    src = wl.sources(graph, S, non_isolated=non_iso)
-    V, E = graph.num_vertices, graph.num_edges
-    gen_s = time.time() - t0
-    print(f"[bench] workload {args.config}: V={V} E={E} S={S} dist={dname} (built in {gen_s:.1f}s)",
-          file=sys.stderr, flush=True)
-
-    ctx = gb.GpuContext(0)
-    eng = gb.GpuBfsEngine(ctx, gb.GpuBfsMethod[args.method])
-    if args.tune:
-        eng.set_tuning(**{k: int(v) for k, v in (t.split("=") for t in args.tune)})
-    dg = gb.DeviceGraph(ctx, graph)
-    n = S * V
-    tdt = torch.int32 if dname == "u32" else torch.int16
-    npdt = np.uint32 if dname == "u32" else np.uint16
-    out = torch.empty(n, dtype=tdt, device="cuda")
-    stream = torch.cuda.current_stream().cuda_stream
-
-    def step(sync):
-        eng.run_ms_bfs_device(dg, src, out.data_ptr(), n, dist_dtype=npdt, stream=stream,
-                              sync=sync)
-
-    for _ in range(args.warmup):
-        step(True)
-    torch.cuda.synchronize()
-    print("[bench] warmup done", file=sys.stderr, flush=True)
-
-    kernel_ms = []
-    torch.cuda.synchronize()
-    t_start = time.perf_counter()
-    for _ in range(args.steps):
-        step(False)
-        kernel_ms.append(eng.last_stats()["kernel_ms"])
-    torch.cuda.synchronize()
-    t_end = time.perf_counter()
-    stats = eng.last_stats()
-    ms_per_step = (t_end - t_start) * 1e3 / args.steps
-    print(f"[bench] timed {args.steps} steps: {ms_per_step:.3f} ms/step", file=sys.stderr, flush=True)
-
-    deg = graph.degrees()
-    dist2d = out.view(S, V)
-    b_d = 4 if dname == "u32" else 2
-    model_bytes, te, maxl = level_bytes(dist2d, S, V, E, deg, b_d)
-    # Compulsory HBM bytes of one launch: every distance written once, the
-    # graph (CSR, plus the neighbour table the deep family reads when the
-    # maximum degree is <= 8) and the sources read once.
-    md = int(deg.max()) if V else 0
-    ell = 4 if md <= 4 else (8 if md <= 8 else 0)
-    compulsory = b_d * S * V + 4 * (V + 1) + 4 * E + 4 * ell * V + 4 * S
-    deep = stats["family"] == 1
-    # The deep family keeps all per-source state in LDS, so only the
-    # compulsory bytes reach HBM; the wide family keeps source words in HBM
-    # and is measured against the per-level model.
-    alg_bytes = compulsory if deep else model_bytes
-    kmean = float(np.mean(kernel_ms))
-    gteps = te / (ms_per_step / 1e3) / 1e9
-    achieved = alg_bytes / (kmean / 1e3) / 1e9
-
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"traffic_{args.config}.json")
-    if os.path.exists(tfile):
-        with open(tfile) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
-
-    result = {
-        "metric": "MS-BFS GTEPS & sources/s at S=2048 on 1 GPU; served p50/p99 latency vs plain",
-        "value": round(gteps, 3),
-        "unit": "GTEPS",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "u64 source words / u32 distances" if dname == "u32" else "u64 source words / u16 distances",
-        "data": "synthetic (seeded generators; graph and distances resident in HBM)",
-        "config": {"workload": args.config, "graph": gname, "num_vertices": V,
-                   "num_edges": E, "num_sources": S, "dist": dname, "method": args.method,
-                   "family": ["none", "deep", "wide"][stats["family"]],
-                   "levels": int(stats["levels"]), "max_distance": maxl,
-                   "bits_per_instance": int(stats["bits_per_instance"]),
-                   "grid_blocks": int(stats["grid_blocks"]),
-                   "block_threads": int(stats["block_threads"]), "parallelism": "single GPU"},
-        "sources_per_s": round(S / (ms_per_step / 1e3), 1),
-        "traversed_edges": te,
-        "kernel_ms": round(kmean, 4),
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                     "traffic": traffic, "algorithmic_bytes": alg_bytes,
-                     "byte_model": "compulsory" if deep else "level",
-                     "compulsory_bytes": compulsory, "level_model_bytes": model_bytes,
-                     "kernel": ("deep_wave_kernel" if stats["block_threads"] == 64 else "deep_bfs_kernel")
-                     if deep else "wide_bfs_kernel + wide_final_kernel"},
-    }
-
-    if not args.no_cpu_baseline:
-        from oracle import oracle as orc
-
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        # Bounded sample: as many sources as fit the time budget, estimated
-        # from a 16-source probe.
-        t = time.perf_counter()
-        orc.ms_bfs_parallel(V, graph.row_offsets, graph.col_indices, src[:16], threads)
-        per_src = (time.perf_counter() - t) / 16
-        ns = int(min(S, max(16, args.cpu_budget_s / 4 / max(per_src, 1e-9))))
-        sample = src[:ns]
-        times = []
-        cpu_out = None
-        deadline = time.perf_counter() + args.cpu_budget_s
-        while len(times) < 3 and (not times or time.perf_counter() < deadline):
-            t = time.perf_counter()
-            cpu_out = orc.ms_bfs_parallel(V, graph.row_offsets, graph.col_indices, sample, threads)
-            times.append(time.perf_counter() - t)
-        cpu_s = float(np.median(times))
-        cpu_te = orc.traversed_edges(V, graph.row_offsets, ns, cpu_out)
-        result["cpu_baseline"] = {
-            "value": round(cpu_te / cpu_s / 1e9, 4), "unit": "GTEPS", "cores": threads,
-            "kind": "port",
-            "sample": f"{ns} of {S} sources, {len(times)} timed runs (median {cpu_s*1e3:.1f} ms), "
-                      "parallel-over-sources FIFO BFS incl. output allocation and fill",
-            "sources_per_s": round(ns / cpu_s, 1)}
-        if not args.no_parity:
-            gpu = dist2d[:ns].cpu().numpy().view(npdt).reshape(-1)
-            ref = cpu_out.astype(np.uint64)
-            if npdt == np.uint16:
-                ref = np.where(cpu_out == 0xFFFFFFFF, 0xFFFF, cpu_out).astype(np.uint16)
-            result["parity"] = bool(np.array_equal(gpu, ref.astype(npdt)))
-            result["parity_sources_checked"] = ns
-    print(json.dumps(result))
+    dname = CONFIGS[args.config][2]
+    res = measure(args, args.config, args.steps, args.warmup, not args.no_cpu_baseline,
+                  not args.no_host_path, with_trace=args.config.startswith("rmat"))
+    out = line(args, res, f"u64 source words / {dname} distances")
+    if args.config == "road-s2048" and not args.no_secondary:
+        sec = {}
+        t0 = time.time()
+        r = measure(args, "rmat-s2048", 5, 2, with_cpu=False, with_host=False, with_trace=True)
+        sec["rmat-s2048"] = {k: r[k] for k in ("ms_per_step", "gteps", "sources_per_s",
+                                               "kernel_ms", "roofline", "pull_phase",
+                                               "levels_trace", "config") if k in r}
+        sec["batch512"] = bench_batch(args, 5, 2)
+        sec["serve"] = bench_serve(args)
+        out["secondary"] = sec
+        out["secondary_wall_s"] = round(time.time() - t0, 1)
+    print(json.dumps(out))
 
 
 if __name__ == "__main__":

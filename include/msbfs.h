@@ -48,8 +48,8 @@ typedef enum msbfs_status {
  * (crates/gpu_bfs/src/engine.rs:33-54).  Each method maps onto one of this
  * library's traversal families; all of them return identical distances. */
 typedef enum msbfs_method {
-    MSBFS_METHOD_NAIVE_SEQUENTIAL = 0,      /* out of scope: MSBFS_ERR_UNSUPPORTED */
-    MSBFS_METHOD_BATCHED_DENSE_SPMM = 1,    /* out of scope: MSBFS_ERR_UNSUPPORTED */
+    MSBFS_METHOD_NAIVE_SEQUENTIAL = 0,      /* deep family, one source per task */
+    MSBFS_METHOD_BATCHED_DENSE_SPMM = 1,    /* wide family, pull only */
     MSBFS_METHOD_BIT_PARALLEL_32 = 2,       /* wide family, push only */
     MSBFS_METHOD_BIT_PARALLEL_64 = 3,       /* wide family, push only */
     MSBFS_METHOD_BIT_PARALLEL_64_PULL = 4,  /* wide family, pull only */
@@ -128,6 +128,10 @@ typedef struct msbfs_tuning {
     uint32_t direction_rule; /* 0: measured frontier degree sum and size (default);
                                 1: the reference's estimate n_f = ceil(bits/S), m_f = n_f*|E|/|V| */
     uint32_t trace_levels;   /* wide family: record up to this many levels (0 = off) */
+    uint32_t stepped;        /* wide family launch: 0 one cooperative kernel for all levels (grid
+                                barriers, the device to itself); 1 one launch per phase of each
+                                level, no co-residency needed, so other traversals may run beside
+                                it (used by the serving loop) */
 } msbfs_tuning;
 
 /* One BFS level of the last wide-family run (recorded when trace_levels > 0). */
@@ -190,6 +194,11 @@ int msbfs_engine_destroy(msbfs_engine *e);
 int msbfs_engine_method(const msbfs_engine *e);
 /* PushPullConfig { alpha, beta } (crates/gpu_bfs/src/dispatch/push_pull.rs:22-36) */
 int msbfs_engine_set_push_pull(msbfs_engine *e, uint32_t alpha, uint32_t beta);
+/* Checks a tuning record without applying it (MSBFS_ERR_INVALID_ARG with a
+ * message for a field out of range).  Every set_tuning entry point (engine,
+ * batch, server lane) applies the same checks.  No reference counterpart:
+ * the reference's knobs are compile-time shader constants. */
+int msbfs_tuning_validate(const msbfs_tuning *t);
 int msbfs_engine_set_tuning(msbfs_engine *e, const msbfs_tuning *t);
 int msbfs_engine_get_tuning(const msbfs_engine *e, msbfs_tuning *t);
 int msbfs_engine_last_stats(msbfs_engine *e, msbfs_run_stats *st);

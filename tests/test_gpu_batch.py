@@ -118,3 +118,16 @@ def test_batch_u16_and_errors(ctx):
     with pytest.raises(gb.MsbfsError) as ei:
         b.run(out.data_ptr(), b.total - 1)
     assert ei.value.code == 4
+
+
+def test_batch_rejects_sub_wave_workgroups(setup):
+    """A workgroup narrower than one wave is rejected on the host before any
+    launch (the deep kernels stride by whole waves); the batch keeps its
+    previous plan and still runs correctly."""
+    graphs, sources, eng, b, out = setup
+    for bad in (32, 1, 96):
+        with pytest.raises(gb.MsbfsError) as ei:
+            b.set_tuning(deep_threads=bad)
+        assert ei.value.code == 1, bad
+    b.run(out.data_ptr(), b.total)
+    check(b, out, expect_all(graphs, sources))

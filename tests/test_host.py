@@ -225,3 +225,50 @@ def test_context_without_device_raises():
             "except gb.NoDeviceError:\n    print('nodevice')\n") % os.path.join(ROOT, "webgpu-bfs_amd")
     out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
     assert out.stdout.strip() in ("device", "nodevice"), out.stderr
+
+
+def test_tuning_validation_rejects_bad_fields():
+    """Every set_tuning entry point (engine, batch, server lane) shares
+    msbfs_tuning_validate; a workgroup narrower than one wave would make the
+    deep kernels' per-wave strides zero, so it must never reach a launch."""
+    import ctypes
+
+    lib = gb.lib()
+
+    def rc(**kw):
+        t = _lib.Tuning()
+        for k, v in kw.items():
+            setattr(t, k, v)
+        return lib.msbfs_tuning_validate(ctypes.byref(t))
+
+    assert rc() == _lib.OK
+    assert rc(deep_threads=64, deep_bits=1, tile_words=16, words_per_lane=2) == _lib.OK
+    for bad in (dict(deep_threads=32), dict(deep_threads=1), dict(deep_threads=96),
+                dict(deep_threads=2048), dict(deep_bits=3), dict(deep_bits=64),
+                dict(tile_words=3), dict(tile_words=128), dict(words_per_lane=3),
+                dict(direction_rule=2), dict(wide_blocks_per_cu=17)):
+        assert rc(**bad) == _lib.ERR_INVALID_ARG, bad
+        assert _lib.last_error()
+    assert lib.msbfs_tuning_validate(None) == _lib.ERR_INVALID_ARG
+
+
+def test_shutdown_order_and_idempotence():
+    """gpu_bfs.shutdown() (also run by atexit, registered once the library is
+    loaded) destroys live objects servers-first, contexts-last, each once."""
+    calls = []
+
+    class Obj:
+        pass
+
+    objs = []
+    for kind in ("context", "graph", "engine", "batch", "server", "graph"):
+        o = Obj()
+        objs.append(o)
+        _lib.track(kind, o, calls.append, kind)
+    _lib.shutdown()
+    assert calls == ["server", "batch", "engine", "graph", "graph", "context"]
+    _lib.shutdown()
+    del objs
+    assert len(calls) == 6
+    import atexit  # noqa: F401  (registered by lib())
+    assert _lib._shutdown_registered

@@ -100,7 +100,7 @@ int batch_plan(msbfs_batch *b) {
     });
     b->ntasks = tasks.size();
     deep_layout_v(c, b->tune, b->max_v, b->ell, b->ntasks, p, 0);
-    HIP_TRY(deep_prepare(B, p.ell, p.lds));
+    HIP_TRY(deep_prepare(B, p.ell, p.threads, p.lds));
     int occ = 0;
     HIP_TRY(deep_max_blocks_per_cu(B, p.ell, p.threads, p.lds, &occ));
     if (occ < 1) return fail(MSBFS_ERR_HIP, "batch: deep kernel cannot be resident");
@@ -140,6 +140,7 @@ DeepParams batch_params(msbfs_batch *b, void *out) {
     p.cap = spill + b->plan.qcap;
     p.lds_qcap = b->plan.qcap;
     p.lds_vis_words = b->plan.vis_words;
+    p.lds_sink_off = b->plan.sink_off;
     p.work = b->work.as<uint32_t>();
     p.flags = b->flags.as<uint32_t>();
     p.reach = nullptr;
@@ -414,7 +415,9 @@ extern "C" int msbfs_batch_stats(msbfs_batch *b, msbfs_run_stats *st) {
 
 extern "C" int msbfs_batch_set_tuning(msbfs_batch *b, const msbfs_tuning *t) {
     if (!b || !t) return fail(MSBFS_ERR_INVALID_ARG, "null argument");
-    int rc = batch_finish(b);
+    int rc = msbfs_tuning_validate(t);
+    if (rc) return rc;
+    rc = batch_finish(b);
     if (rc) return rc;
     b->tune = *t;
     return batch_plan(b);

@@ -329,16 +329,29 @@ extern "C" int msbfs_engine_set_push_pull(msbfs_engine *e, uint32_t alpha, uint3
     return MSBFS_OK;
 }
 
-extern "C" int msbfs_engine_set_tuning(msbfs_engine *e, const msbfs_tuning *t) {
-    if (!e || !t) return fail(MSBFS_ERR_INVALID_ARG, "null argument");
+extern "C" int msbfs_tuning_validate(const msbfs_tuning *t) {
+    if (!t) return fail(MSBFS_ERR_INVALID_ARG, "null tuning");
     if (t->tile_words && (t->tile_words > 64 || (t->tile_words & (t->tile_words - 1))))
         return fail(MSBFS_ERR_INVALID_ARG, "tile_words must be a power of two <= 64");
     if (t->words_per_lane > 2)
         return fail(MSBFS_ERR_INVALID_ARG, "words_per_lane must be 0, 1 or 2");
     if (t->deep_bits && (t->deep_bits > 32 || (t->deep_bits & (t->deep_bits - 1))))
         return fail(MSBFS_ERR_INVALID_ARG, "deep_bits must be 1, 2, 4, 8, 16 or 32");
+    // the deep kernels stride whole waves: a workgroup is 1..16 waves
     if (t->deep_threads && (t->deep_threads % 64 || t->deep_threads > 1024))
         return fail(MSBFS_ERR_INVALID_ARG, "deep_threads must be a multiple of 64 <= 1024");
+    if (t->wide_blocks_per_cu > 16)
+        return fail(MSBFS_ERR_INVALID_ARG, "wide_blocks_per_cu must be <= 16");
+    if (t->direction_rule > 1)
+        return fail(MSBFS_ERR_INVALID_ARG, "direction_rule must be 0 or 1");
+    if (t->stepped > 1) return fail(MSBFS_ERR_INVALID_ARG, "stepped must be 0 or 1");
+    return MSBFS_OK;
+}
+
+extern "C" int msbfs_engine_set_tuning(msbfs_engine *e, const msbfs_tuning *t) {
+    if (!e || !t) return fail(MSBFS_ERR_INVALID_ARG, "null argument");
+    int rc = msbfs_tuning_validate(t);
+    if (rc) return rc;
     std::lock_guard<std::mutex> lk(e->mu);
     e->tune = *t;
     return MSBFS_OK;
@@ -433,12 +446,36 @@ void deep_layout_v(const msbfs_context *c, const msbfs_tuning &t, uint64_t max_v
     auto dyn_budget = [&](int k) { return c->lds_max / k - kDeepStaticLds; };
     while (per_cu > 1 && p.vis_words * 4 + min_q > dyn_budget(per_cu)) --per_cu;
     const size_t per_wg = dyn_budget(per_cu);
-    size_t qbytes = per_wg > p.vis_words * 4 ? per_wg - p.vis_words * 4 : 0;
+    // Logged u32 output (one-wave kernel, one source per task): one LDS
+    // counter per bin of 2^kDeepLogBits vertices after the queues; a
+    // (vertex in bin, level) entry is below 0xFFFFFFFF (the kernel's "no
+    // entry" mark) when level < 2^(32 - kDeepLogBits) - 1, which
+    // |V| < 2^(32 - kDeepLogBits) guarantees (level <= |V| - 1).
+    // Only with four or more tasks per CU: rewriting a task's row costs
+    // about 7 us per bin when the task ends, hidden behind other tasks when
+    // there are many (road-like, 114,599 vertices, ms logged / direct:
+    // S=2048 (8 per CU) 2.81 / 2.95, S=256 2.14 / 1.89, S=1 1.71 / 1.48).
+    const bool logged = p.threads == 64 && B == 1 && ell > 0 && max_v > 0 &&
+                        max_v < (1ull << (32 - kDeepLogBits)) &&
+                        ntasks >= 4 * (uint64_t)std::max(1, c->cus);
+    const uint64_t nbins = logged ? (max_v + (1u << kDeepLogBits) - 1) >> kDeepLogBits : 0;
+    size_t qbytes = per_wg > p.vis_words * 4 + nbins * 4 ? per_wg - p.vis_words * 4 - nbins * 4 : 0;
     uint64_t qcap = qbytes / (2 * esz);
     qcap = std::min<uint64_t>(qcap, std::max<uint64_t>((uint64_t)B * max_v, 1));
     qcap = std::min<uint64_t>(qcap, 1u << 20);
     p.qcap = (uint32_t)qcap;
     p.lds = p.vis_words * 4 + 2 * qcap * esz;
+    // the task's end rewrites each bin's span through LDS: the visited bits
+    // and queues must hold one bin
+    p.log_bw = 0;
+    if (logged && std::min<uint64_t>(max_v, 1u << kDeepLogBits) * 4 <= p.lds) {
+        p.log_bw = kDeepLogBits;
+        p.bin_off = (uint32_t)(p.lds / 4);
+        p.lds += nbins * 4;
+    }
+    // the one-wave kernel's 64 sink entries come out of the static reservation
+    p.sink_off = (uint32_t)((p.lds + 7) / 8 * 2);
+    if (p.threads == 64 && ell > 0) p.lds = p.sink_off * 4 + 64 * esz;
 }
 
 int ensure_zeroed(DevBuf &b, size_t bytes, bool force, hipStream_t st) {
@@ -478,12 +515,18 @@ Plan plan_for(const msbfs_engine *e, const msbfs_graph *g, uint64_t S) {
             p.family = MSBFS_FAMILY_WIDE;
             p.policy = MSBFS_DIR_ADAPTIVE;
         }
-    } else if (p.method == MSBFS_METHOD_NAIVE_SEQUENTIAL ||
-               p.method == MSBFS_METHOD_BATCHED_DENSE_SPMM) {
-        p.family = MSBFS_FAMILY_NONE;
-        return p;
-    } else if (p.method == MSBFS_METHOD_WORKGROUP_PERSISTENT_BFS) {
+    } else if (p.method == MSBFS_METHOD_WORKGROUP_PERSISTENT_BFS ||
+               p.method == MSBFS_METHOD_NAIVE_SEQUENTIAL) {
+        // NaiveSequential (dispatch/naive_seq.rs: one single-source search
+        // after another) maps to the deep family with one source per task:
+        // independent single-source searches, run side by side.
         p.family = MSBFS_FAMILY_DEEP;
+    } else if (p.method == MSBFS_METHOD_BATCHED_DENSE_SPMM) {
+        // BatchedDenseSpMM (dispatch/batched_spmm.rs: every vertex's parents
+        // combined each level, all sources at once) maps to the wide
+        // family's pull, which is that product with early exit.
+        p.family = MSBFS_FAMILY_WIDE;
+        p.policy = MSBFS_DIR_PULL;
     } else {
         p.family = MSBFS_FAMILY_WIDE;
         p.policy = is_pull_only(p.method)   ? MSBFS_DIR_PULL
@@ -491,7 +534,9 @@ Plan plan_for(const msbfs_engine *e, const msbfs_graph *g, uint64_t S) {
                                             : MSBFS_DIR_ADAPTIVE;
     }
     if (p.family == MSBFS_FAMILY_DEEP) {
-        p.bits = deep_bits_for(e->ctx, e->tune, g->V, S);
+        msbfs_tuning t = e->tune;
+        if (p.method == MSBFS_METHOD_NAIVE_SEQUENTIAL) t.deep_bits = 1;
+        p.bits = deep_bits_for(e->ctx, t, g->V, S);
         if (p.bits == 0) {
             // The task's visited bits do not fit in LDS: the device-wide
             // family takes over (the reference likewise falls back to its
@@ -499,7 +544,7 @@ Plan plan_for(const msbfs_engine *e, const msbfs_graph *g, uint64_t S) {
             p.family = MSBFS_FAMILY_WIDE;
             p.policy = MSBFS_DIR_ADAPTIVE;
         } else {
-            deep_layout_v(e->ctx, e->tune, g->V, g->ell_width, (S + p.bits - 1) / p.bits, p,
+            deep_layout_v(e->ctx, t, g->V, g->ell_width, (S + p.bits - 1) / p.bits, p,
                           g->deep_levels.load(std::memory_order_relaxed));
         }
     }
@@ -533,7 +578,10 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
     if ((rc = ensure_plain(e->sources, S * 4))) return rc;
     if ((rc = ensure_plain(e->flags, FlagCount * 4))) return rc;
     if ((rc = e->h_flags.ensure(FlagCount * 4)) < 0) return -rc;
-    const bool coop = plan.family == MSBFS_FAMILY_WIDE;
+    // the cooperative wide kernel needs the device to itself; stepped wide
+    // launches and deep launches share it
+    const bool stepped = plan.family == MSBFS_FAMILY_WIDE && e->tune.stepped;
+    const bool coop = plan.family == MSBFS_FAMILY_WIDE && !stepped;
     LaunchOrder order(c, st, coop);
     if ((rc = order.begin())) return rc;
     HIP_TRY(hipEventRecord(e->ev[0], st));
@@ -579,7 +627,10 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
         }
         // level counters (4 x 32 B), then the grid barrier words from byte 256
         static_assert(4 * sizeof(WideCounters) <= 256, "level counters overlap the barrier");
-        if ((rc = ensure_plain(e->ctr, 256 + kGridBarrierWords * 4))) return rc;
+        // ... and the stepped launches' level control after the barrier words
+        constexpr size_t kCtlOff = 256 + kGridBarrierWords * 4;
+        static_assert(kCtlOff % 16 == 0 && sizeof(WideCtl) % 4 == 0, "level control layout");
+        if ((rc = ensure_plain(e->ctr, kCtlOff + sizeof(WideCtl)))) return rc;
         // zero row (one word per source word), read in place of frontier
         // words of parents outside the frontier; never written
         if ((rc = ensure_zeroed(e->zrow, std::max<uint64_t>(W, 64) * 8, false, st))) return rc;
@@ -677,10 +728,25 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
         per_cu = std::min(per_cu, occ);
         const int blocks = c->cus * per_cu;
 
-        HIP_TRY(launch_zero_u32((uint32_t *)e->ctr.p, (256 + kGridBarrierWords * 4) / 4, st));
+        HIP_TRY(launch_zero_u32((uint32_t *)e->ctr.p, (kCtlOff + sizeof(WideCtl)) / 4, st));
         HIP_TRY(launch_wide_seed(p, st));
         HIP_TRY(hipEventRecord(e->ev[1], st));
-        HIP_TRY(launch_wide(tw, p, blocks, kWideThreads, lds, st));
+        if (stepped) {
+            // as many levels as this graph has needed before, and a margin;
+            // finish_run enqueues more if the frontier is not empty by then
+            p.ctl = (WideCtl *)((char *)e->ctr.p + kCtlOff);
+            const uint32_t seen = g->wide_levels.load(std::memory_order_relaxed);
+            const uint32_t nlev = seen ? seen + 2 : 16;
+            HIP_TRY(launch_wide_steps(tw, p, blocks, kWideThreads, lds, nlev, st));
+            e->step_p = p;
+            e->step_tw = tw;
+            e->step_blocks = blocks;
+            e->step_lds = lds;
+            e->step_levels = nlev;
+            e->trace_valid = false;
+        } else {
+            HIP_TRY(launch_wide(tw, p, blocks, kWideThreads, lds, st));
+        }
         {
             // final pass: whole tiles per workgroup, up to four workgroups per CU
             const uint64_t ntile = (V + kFinalVerts - 1) / kFinalVerts;
@@ -697,7 +763,13 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
         const int B = plan.bits;
         const uint64_t ntasks = (S + B - 1) / B;
         const int threads = plan.threads;
-        HIP_TRY(deep_prepare(B, plan.ell, plan.lds));
+        {
+            const hipError_t pe = deep_prepare(B, plan.ell, threads, plan.lds);
+            if (pe != hipSuccess)
+                return fail(pe == hipErrorOutOfMemory ? MSBFS_ERR_OOM : MSBFS_ERR_HIP,
+                            "deep kernel set-up (%d sources per task, %d threads, %zu bytes of "
+                            "LDS): %s", B, plan.threads, plan.lds, hipGetErrorString(pe));
+        }
         int occ = 0;
         HIP_TRY(deep_max_blocks_per_cu(B, plan.ell, threads, plan.lds, &occ));
         if (occ < 1) return fail(MSBFS_ERR_HIP, "deep kernel cannot be resident (lds %zu)", plan.lds);
@@ -750,6 +822,14 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
         p.cap = spill + plan.qcap;
         p.lds_qcap = plan.qcap;
         p.lds_vis_words = plan.vis_words;
+        // rows written through the binned log: u32 output only
+        p.log_bw = (out && dtype == MSBFS_DIST_U32) ? plan.log_bw : 0;
+        p.lds_bin_off = plan.bin_off;
+        p.lds_sink_off = plan.sink_off;
+        if (p.log_bw) {
+            if ((rc = ensure_plain(e->log_sink, (size_t)blocks * 64 * 4))) return rc;
+            p.log_sink = e->log_sink.as<uint32_t>();
+        }
         p.work = e->work.as<uint32_t>();
         p.flags = e->flags.as<uint32_t>();
         // reachability: each task writes its sources' rows of a source-major
@@ -780,7 +860,8 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
     e->pending = true;
     e->last_stream = st;
     e->pending_dtype = dtype;
-    if (plan.family == MSBFS_FAMILY_DEEP && plan.bits == 1 && g->ell_width > 0) {
+    e->pending_stepped = stepped;
+    if (plan.family == MSBFS_FAMILY_WIDE || (plan.bits == 1 && g->ell_width > 0)) {
         msbfs_graph *const gm = const_cast<msbfs_graph *>(g);
         gm->refs.fetch_add(1);
         e->pending_graph = gm;
@@ -801,12 +882,40 @@ int finish_run(msbfs_engine *e) {
         return hip_fail(er, "traversal");
     }
     const uint32_t *f = (const uint32_t *)e->h_flags.p;
+    // A stepped traversal with a frontier left after its enqueued levels:
+    // twice as many more levels (plus the final pass) until it ends.
+    while (e->pending_stepped && f[FlagIncomplete] && !f[FlagAbort]) {
+        hipStream_t st = e->last_stream;
+        uint32_t *const dflags = e->flags.as<uint32_t>();
+        int rc = MSBFS_OK;
+        e->step_levels = std::min<uint32_t>(2 * e->step_levels, e->step_p.V + 2);
+        if (launch_zero_u32(dflags + FlagIncomplete, 1, st) != hipSuccess ||
+            launch_wide_steps(e->step_tw, e->step_p, e->step_blocks, kWideThreads, e->step_lds,
+                              e->step_levels, st) != hipSuccess ||
+            launch_wide_final(e->step_p, (int)std::max<uint64_t>(1, std::min<uint64_t>(
+                                  (e->step_p.V + kFinalVerts - 1) / kFinalVerts,
+                                  (uint64_t)e->ctx->cus * 16)), st) != hipSuccess ||
+            hipEventRecord(e->ev[2], st) != hipSuccess ||
+            hipMemcpyAsync(e->h_flags.p, e->flags.p, FlagCount * 4, hipMemcpyDeviceToHost, st) !=
+                hipSuccess ||
+            hipEventRecord(e->ev[3], st) != hipSuccess)
+            rc = fail(MSBFS_ERR_HIP, "stepped traversal: continuing launches failed");
+        if (rc == MSBFS_OK && (er = hipStreamSynchronize(st)) != hipSuccess)
+            rc = hip_fail(er, "traversal");
+        if (rc) {
+            graph_release(pg);
+            e->dirty = true;
+            return rc;
+        }
+    }
     if (pg) {
         // record the level count for later plans on this graph
         if (!f[FlagQueueOverflow] && !f[FlagAbort]) {
-            uint32_t cur = pg->deep_levels.load(std::memory_order_relaxed);
+            std::atomic<uint32_t> &lv = e->last.family == MSBFS_FAMILY_WIDE ? pg->wide_levels
+                                                                            : pg->deep_levels;
+            uint32_t cur = lv.load(std::memory_order_relaxed);
             while (f[FlagLevels] > cur &&
-                   !pg->deep_levels.compare_exchange_weak(cur, f[FlagLevels], std::memory_order_relaxed)) {
+                   !lv.compare_exchange_weak(cur, f[FlagLevels], std::memory_order_relaxed)) {
             }
         }
         graph_release(pg);
@@ -911,11 +1020,6 @@ int run_common(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, u
                int dtype, void *out_dev, uint32_t *reach_dev, uint32_t reach_wpv, hipStream_t st,
                bool sync) {
     const Plan plan = plan_for(e, g, S);
-    if (plan.family == MSBFS_FAMILY_NONE)
-        return fail(MSBFS_ERR_UNSUPPORTED,
-                    "method %d (NaiveSequential / BatchedDenseSpMM baselines) is not provided by "
-                    "this engine",
-                    e->method);
     int rc = finish_run(e);
     if (rc) return rc;
     rc = enqueue_run(e, g, sources, S, dtype, out_dev, reach_dev, reach_wpv, st, plan);

@@ -133,6 +133,9 @@ struct msbfs_graph {
     // Most BFS levels a deep traversal with one source per task has needed
     // on this graph (0: none yet); |V| / levels estimates its frontier size.
     std::atomic<uint32_t> deep_levels{0};
+    // Most levels a wide traversal has needed on this graph (0: none yet):
+    // how many levels a stepped launch enqueues up front.
+    std::atomic<uint32_t> wide_levels{0};
 };
 
 struct msbfs_engine {
@@ -148,7 +151,7 @@ struct msbfs_engine {
     uint32_t tick_base = 1;
     bool dirty = true;
     // deep workspace
-    msbfs::DevBuf queues, tasks, graphs, work;
+    msbfs::DevBuf queues, tasks, graphs, work, log_sink;
     // shared
     msbfs::DevBuf flags, sources, out_tmp, out_pack, reach_tmp, reach_t, trace;
     // host-output path: pinned staging chunks and their copy-done events
@@ -165,6 +168,13 @@ struct msbfs_engine {
     bool pending = false;
     int pending_dtype = 0;
     msbfs_graph *pending_graph = nullptr;  // held until its level count is recorded
+    // stepped wide launch of the pending run: its parameters, for enqueueing
+    // more levels when the frontier outlasts those enqueued
+    bool pending_stepped = false;
+    msbfs::WideParams step_p{};
+    int step_tw = 0, step_blocks = 0;
+    size_t step_lds = 0;
+    uint32_t step_levels = 0;
 };
 
 
@@ -179,6 +189,9 @@ struct Plan {
     int threads = 0;    // deep: workgroup size
     uint32_t vis_words = 0;  // deep: LDS words of visited bits
     uint32_t qcap = 0;  // deep: LDS entries per frontier queue
+    uint32_t log_bw = 0;   // deep: vertex bin bits of the logged u32 output (0: none)
+    uint32_t bin_off = 0;  // deep: LDS word offset of the log's bin counters
+    uint32_t sink_off = 0; // deep one-wave kernel: LDS word offset of its append sink
     size_t lds = 0;     // deep: dynamic LDS bytes per workgroup
 };
 

@@ -550,7 +550,7 @@ __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
         // 64 visited words at once and skips their vertices when every bit is
         // set (a connected graph has nothing to write).
         if constexpr (OUT) {
-            const uint32_t lane = lane_id(), wv = tid / 64, nw = nth / 64;
+            const uint32_t lane = lane_id(), wv = tid / 64, nw = max(1u, nth / 64);
             for (uint32_t i0 = wv * 64; i0 < nwords; i0 += nw * 64) {
                 const uint32_t i = i0 + lane;
                 if (__all(i >= nwords || vis[i] == 0xFFFFFFFFu)) continue;
@@ -566,7 +566,7 @@ __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
         if (p.reach_t) {
             // each source's row of the source-major bitmap, 64 vertices per
             // ballot and wave (no atomics: the row belongs to this task)
-            const uint32_t lane = lane_id(), wv = tid / 64, nw = nth / 64;
+            const uint32_t lane = lane_id(), wv = tid / 64, nw = max(1u, nth / 64);
             for (uint32_t b = 0; b < task.nsrc; ++b) {
                 uint32_t *const row = p.reach_t + (task.reach_bit + b) * (uint64_t)p.reach_vw;
                 for (uint32_t v0 = wv * 64; v0 < V; v0 += nw * 64) {
@@ -586,9 +586,23 @@ __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
 // reserved with ballot prefix sums instead of an LDS atomic, and the wave
 // needs no barrier (its LDS operations execute in order).  LDS layout,
 // queues with global spill, outputs and flags as in deep_bfs_kernel.
+//
+// DT = 4 (u32 rows, one source per task) writes no distance in place while
+// the search runs.  Scattered 4-byte stores into 2,048 rows of 458 KB fill
+// each 128-byte line over many levels, and the lines leave L2 and the
+// Infinity Cache partly written (measured: 6.4x the output written, and a
+// third of the kernel time at road-like S=2048).  Instead each expanded
+// entry appends (vertex mod 2^log_bw, level) to its vertex bin's log, and
+// the bin's log lives in the bin's own span of the task's output row (a
+// bin of n vertices receives at most n entries), so the appends are
+// sequential streams that L2 merges into whole lines.  When the task ends,
+// each bin's log is scattered into LDS (sentinels first) and the span is
+// rewritten in order with coalesced stores.
 template <int B, int ELL, int DT>
 __global__ void __launch_bounds__(64) deep_wave_kernel(DeepParams p) {
     static_assert(ELL == 4 || ELL == 8, "the wave kernel reads neighbour tables");
+    static_assert(DT != 4 || B == 1, "logged output holds one source per task");
+    constexpr bool LOG = DT == 4;
     constexpr bool OUT = DT != 0;
     constexpr uint32_t KIND = DT == 1 ? 1u : DT == 3 ? 2u : 0u;
     constexpr uint32_t ESZ = DT == 1 ? 2u : DT == 3 ? 1u : 4u;
@@ -597,8 +611,10 @@ __global__ void __launch_bounds__(64) deep_wave_kernel(DeepParams p) {
     typedef __attribute__((address_space(3))) Entry *lq_t;
     typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
     constexpr uint32_t VPW = 32 / B;
+    // No static LDS: the visited bits start at address 0, so claim
+    // addresses need no base added.
     extern __shared__ uint32_t lds[];
-    __shared__ Entry s_sink[64];  // target of the appends a lane does not make
+    Entry *const s_sink = (Entry *)(lds + p.lds_sink_off);  // appends a lane does not make
 
     uint32_t *const vis = lds;
     __attribute__((address_space(3))) uint32_t *const lvis =
@@ -609,6 +625,11 @@ __global__ void __launch_bounds__(64) deep_wave_kernel(DeepParams p) {
     const uint64_t gcap = p.cap - p.lds_qcap;
     Entry *const gq0 = (Entry *)p.queues + (uint64_t)blockIdx.x * 2 * gcap;
     Entry *const gq1 = gq0 + gcap;
+    typedef __attribute__((address_space(1))) uint32_t *g32_t;
+    typedef __attribute__((address_space(3))) uint32_t *l32_t;
+    const uint32_t lbw = LOG ? p.log_bw : 0u, lmask = (1u << lbw) - 1u;
+    const l32_t lbin = (l32_t)(lds + p.lds_bin_off);
+    const g32_t sink = LOG ? (g32_t)p.log_sink + blockIdx.x * 64 + lane : nullptr;
 
     for (;;) {
         uint32_t t = 0;
@@ -625,6 +646,9 @@ __global__ void __launch_bounds__(64) deep_wave_kernel(DeepParams p) {
             (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)ell_a));
         const uint32_t nwords = (V + VPW - 1) / VPW;
         for (uint32_t i = lane; i < nwords; i += 64) vis[i] = 0;
+        const uint32_t nbins = LOG ? (V + lmask) >> lbw : 0u;
+        if constexpr (LOG)
+            for (uint32_t i = lane; i < nbins; i += 64) lbin[i] = 0;
         DeepQueue<Entry> qa{lq0, gq0, p.lds_qcap, gcap};
         DeepQueue<Entry> qb{lq1, gq1, p.lds_qcap, gcap};
         if (lane < task.nsrc) {
@@ -636,6 +660,11 @@ __global__ void __launch_bounds__(64) deep_wave_kernel(DeepParams p) {
         void *const out_task = OUT ? (void *)((char *)p.out + task.out_off * ESZ) : nullptr;
         uint32_t n = task.nsrc;  // entries of this level (wave-uniform)
         uint32_t level = 0;
+        // logged output: the previous pass's log entry, stored once the next
+        // pass's row load is in flight (its address waits on an LDS atomic,
+        // and a store issued late in a pass would hold up the next pass's
+        // first wait on the memory counter)
+        uint32_t pend_at = 0xFFFFFFFFu, pend_val = 0;
         while (n) {
             if (OUT && level > DMAX && lane == 0) atomicOr(&p.flags[FlagU16Overflow], 1u);
             const uint32_t dval = min(level, DMAX);
@@ -669,9 +698,24 @@ __global__ void __launch_bounds__(64) deep_wave_kernel(DeepParams p) {
                             nb[4 * k + 3] = r.w;
                         }
                     }
-                    // the entry's lowest source bit, one straight-line store
-                    // per lane (a lane without an entry repeats lane 0's)
-                    if constexpr (OUT) {
+                    if constexpr (LOG) {
+                        // append (vertex in bin, level) to the bin's log
+                        // one unconditional store per lane (lanes without an entry
+                        // write their sink word), so the wait for this pass's row
+                        // load may leave it in flight
+                        *(pend_at != 0xFFFFFFFFu ? (g32_t)out_task + pend_at : sink) = pend_val;
+                        pend_at = 0xFFFFFFFFu;
+                        if (act) {
+                            const uint32_t bin = u >> lbw;
+                            const uint32_t pos = __hip_atomic_fetch_add(
+                                lbin + bin, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                            pend_at = (bin << lbw) + pos;
+                            pend_val = (u & lmask) | (level << lbw);
+                        }
+                    } else if constexpr (OUT) {
+                        // the entry's lowest source bit, one straight-line
+                        // store per lane (a lane without an entry repeats
+                        // lane 0's)
                         const uint32_t b0 = B == 1 ? 0u : __builtin_ctz(f | 0x80000000u);
                         const uint32_t u_l0 = (uint32_t)__builtin_amdgcn_readlane((int)u, 0);
                         const uint32_t b_l0 = (uint32_t)__builtin_amdgcn_readlane((int)b0, 0);
@@ -685,7 +729,6 @@ __global__ void __launch_bounds__(64) deep_wave_kernel(DeepParams p) {
                     }
                     // claims: all slots' LDS atomics, then one wait
                     uint32_t newly[ELL];
-                    uint32_t c = 0;
                     if (act) {  // (lanes without an entry would all hit vertex 0's words)
                         uint32_t old[ELL];
 #pragma unroll
@@ -693,38 +736,37 @@ __global__ void __launch_bounds__(64) deep_wave_kernel(DeepParams p) {
                             old[k] = __hip_atomic_fetch_or(lvis + nb[k] / VPW, f << ((nb[k] % VPW) * B),
                                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 #pragma unroll
-                        for (int k = 0; k < ELL; ++k) {
-                            newly[k] = f & ~(old[k] >> ((nb[k] % VPW) * B));
-                            c += newly[k] != 0;
-                        }
+                        for (int k = 0; k < ELL; ++k) newly[k] = f & ~(old[k] >> ((nb[k] % VPW) * B));
                     } else {
 #pragma unroll
                         for (int k = 0; k < ELL; ++k) newly[k] = 0;
                     }
-                    // slots: exclusive prefix of c over the wave from ELL ballots
-                    uint32_t pre = 0, tot = 0;
+                    // slots, slot-major: the new entries of slot k follow those
+                    // of slots < k, in lane order within the slot (one ballot
+                    // and one v_mbcnt pair per slot; queue order is free)
+                    u64 bk[ELL];
+                    uint32_t base_k[ELL];
+                    uint32_t tot = 0;
 #pragma unroll
-                    for (int j = 0; j < ELL; ++j) {
-                        const u64 bj = __ballot(c > (uint32_t)j);
-                        pre = mbcnt64(bj, pre);
-                        tot += (uint32_t)__popcll(bj);
+                    for (int k = 0; k < ELL; ++k) {
+                        bk[k] = __ballot(newly[k] != 0);
+                        base_k[k] = nn + tot;
+                        tot += (uint32_t)__popcll(bk[k]);
                     }
-                    uint32_t pos = nn + pre;
                     if (nn + tot <= qb.lcap) {
                         lq_t const q = (lq_t)qb.lds;
                         lq_t const sk = (lq_t)s_sink + lane;
 #pragma unroll
-                        for (int k = 0; k < ELL; ++k) {
-                            *(newly[k] ? q + pos : sk) = pack_entry(nb[k], newly[k], (Entry *)nullptr);
-                            pos += newly[k] != 0;
-                        }
+                        for (int k = 0; k < ELL; ++k)
+                            *(newly[k] ? q + mbcnt64(bk[k], base_k[k]) : sk) =
+                                pack_entry(nb[k], newly[k], (Entry *)nullptr);
                     } else {
 #pragma unroll
                         for (int k = 0; k < ELL; ++k) {
                             if (newly[k]) {
-                                if (!qb.put(pos, pack_entry(nb[k], newly[k], (Entry *)nullptr)))
+                                if (!qb.put(mbcnt64(bk[k], base_k[k]),
+                                            pack_entry(nb[k], newly[k], (Entry *)nullptr)))
                                     atomicOr(&p.flags[FlagQueueOverflow], 1u);
-                                ++pos;
                             }
                         }
                     }
@@ -745,7 +787,64 @@ __global__ void __launch_bounds__(64) deep_wave_kernel(DeepParams p) {
             n = nn;
             ++level;
         }
+        if constexpr (LOG)
+            if (pend_at != 0xFFFFFFFFu) ((g32_t)out_task)[pend_at] = pend_val;
         if (lane == 0) atomicMax(&p.flags[FlagLevels], level);
+        if (p.reach_t) {
+            // as in the workgroup kernel: one ballot per 64 vertices
+            for (uint32_t b = 0; b < task.nsrc; ++b) {
+                uint32_t *const row = p.reach_t + (task.reach_bit + b) * (uint64_t)p.reach_vw;
+                for (uint32_t v0 = 0; v0 < V; v0 += 64) {
+                    const uint32_t v = v0 + lane;
+                    const u64 m = __ballot(v < V && ((vis[v / VPW] >> ((v % VPW) * B + b)) & 1u));
+                    if (lane == 0) row[v0 / 32] = (uint32_t)m;
+                    if (lane == 32 && v0 + 32 < V) row[v0 / 32 + 1] = (uint32_t)(m >> 32);
+                }
+            }
+        }
+        if constexpr (LOG) {
+            // The log entries were stored by this wave; make them visible
+            // to its loads (the vector L1 is not kept coherent with them).
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
+            const l32_t chunk = (l32_t)lds;  // visited bits and queues are free now
+            // A bin holds at most 4,096 entries: a lane loads its 64 at once,
+            // and the next bin's loads are issued before this bin is
+            // scattered, so the rewrite costs about one memory round trip
+            // per bin.
+            static_assert(kDeepLogBits == 12, "64 log entries per lane and bin");
+            uint32_t cur[64], nxt[64];
+            auto load_bin = [&](uint32_t c, uint32_t (&e)[64]) {
+                const uint32_t cnt = (uint32_t)__builtin_amdgcn_readfirstlane((int)lbin[c]);
+                const g32_t span = (g32_t)out_task + (c << lbw);
+#pragma unroll
+                for (int q = 0; q < 64; ++q) {
+                    const uint32_t j = q * 64 + lane;
+                    e[q] = j < cnt ? span[j] : 0xFFFFFFFFu;
+                }
+            };
+            auto rewrite_bin = [&](uint32_t c, const uint32_t (&e)[64]) {
+                const uint32_t c0 = c << lbw;
+                const uint32_t len = min(lmask + 1u, V - c0);
+                for (uint32_t x = lane; x < len; x += 64) chunk[x] = 0xFFFFFFFFu;
+                __builtin_amdgcn_wave_barrier();
+#pragma unroll
+                for (int q = 0; q < 64; ++q)
+                    if (e[q] != 0xFFFFFFFFu) chunk[e[q] & lmask] = e[q] >> lbw;
+                __builtin_amdgcn_wave_barrier();
+                uint32_t *const span = (uint32_t *)out_task + c0;
+                for (uint32_t x = lane; x < len; x += 64) __builtin_nontemporal_store(chunk[x], span + x);
+                __builtin_amdgcn_wave_barrier();
+            };
+            if (nbins) load_bin(0, cur);
+            for (uint32_t c = 0; c < nbins; c += 2) {
+                if (c + 1 < nbins) load_bin(c + 1, nxt);
+                rewrite_bin(c, cur);
+                if (c + 1 < nbins) {
+                    if (c + 2 < nbins) load_bin(c + 2, cur);
+                    rewrite_bin(c + 1, nxt);
+                }
+            }
+        } else
         if constexpr (OUT) {
             // sentinels for unreached pairs, skipping runs of 64 full visited words
             for (uint32_t i0 = 0; i0 < nwords; i0 += 64) {
@@ -760,38 +859,36 @@ __global__ void __launch_bounds__(64) deep_wave_kernel(DeepParams p) {
                 }
             }
         }
-        if (p.reach_t) {
-            // as in the workgroup kernel: one ballot per 64 vertices
-            for (uint32_t b = 0; b < task.nsrc; ++b) {
-                uint32_t *const row = p.reach_t + (task.reach_bit + b) * (uint64_t)p.reach_vw;
-                for (uint32_t v0 = 0; v0 < V; v0 += 64) {
-                    const uint32_t v = v0 + lane;
-                    const u64 m = __ballot(v < V && ((vis[v / VPW] >> ((v % VPW) * B + b)) & 1u));
-                    if (lane == 0) row[v0 / 32] = (uint32_t)m;
-                    if (lane == 32 && v0 + 32 < V) row[v0 / 32 + 1] = (uint32_t)(m >> 32);
-                }
-            }
-        }
     }
 }
 
 // The deep kernel for a launch: the one-wave kernel for 64-thread
 // workgroups on a neighbour table, the workgroup kernel otherwise.
+// DT = 4 (logged u32 rows) exists only as the one-wave kernel with one
+// source per task; it is null otherwise.
 template <int B, int ELL, int DT>
 static const void *deep_kernel_for(int threads) {
-    if constexpr (ELL > 0)
-        if (threads == 64) return (const void *)deep_wave_kernel<B, ELL, DT>;
-    return (const void *)deep_bfs_kernel<B, ELL, DT>;
+    if constexpr (DT == 4) {
+        if constexpr (ELL > 0 && B == 1)
+            if (threads == 64) return (const void *)deep_wave_kernel<B, ELL, 4>;
+        return nullptr;
+    } else {
+        if constexpr (ELL > 0)
+            if (threads == 64) return (const void *)deep_wave_kernel<B, ELL, DT>;
+        return (const void *)deep_bfs_kernel<B, ELL, DT>;
+    }
 }
 
 template <int B, int ELL>
 static hipError_t deep_launch_t(const DeepParams &p, int blocks, int threads, size_t lds,
                                 hipStream_t st) {
-    const int dt = !p.out ? 0 : p.out_kind == 1 ? 1 : p.out_kind == 2 ? 3 : 2;
+    const int dt = !p.out ? 0 : p.out_kind == 1 ? 1 : p.out_kind == 2 ? 3 : p.log_bw ? 4 : 2;
     const void *k = dt == 0   ? deep_kernel_for<B, ELL, 0>(threads)
                     : dt == 1 ? deep_kernel_for<B, ELL, 1>(threads)
                     : dt == 3 ? deep_kernel_for<B, ELL, 3>(threads)
+                    : dt == 4 ? deep_kernel_for<B, ELL, 4>(threads)
                               : deep_kernel_for<B, ELL, 2>(threads);
+    if (!k) return hipErrorInvalidConfiguration;
     void *args[] = {(void *)&p};
     return hipLaunchKernel(k, dim3(blocks), dim3(threads), args, lds, st);
 }
@@ -805,32 +902,35 @@ static hipError_t deep_prepare_one(const void *k, size_t lds) {
     return hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 }
 
+// Only the kernel shape that will run (one-wave or workgroup) is prepared:
+// their static LDS differs, so one dynamic size need not suit both.
 template <int B, int ELL, int DT>
-static hipError_t deep_prepare_dt(size_t lds) {
-    hipError_t e = deep_prepare_one<B, ELL, DT>(deep_kernel_for<B, ELL, DT>(256), lds);
-    if (e == hipSuccess && ELL > 0) e = deep_prepare_one<B, ELL, DT>(deep_kernel_for<B, ELL, DT>(64), lds);
-    return e;
+static hipError_t deep_prepare_dt(int threads, size_t lds) {
+    const void *k = deep_kernel_for<B, ELL, DT>(threads);
+    return k ? deep_prepare_one<B, ELL, DT>(k, lds) : hipSuccess;
 }
 
 template <int B, int ELL>
-static hipError_t deep_prepare_t(size_t lds) {
-    hipError_t e = deep_prepare_dt<B, ELL, 0>(lds);
-    if (e == hipSuccess) e = deep_prepare_dt<B, ELL, 1>(lds);
-    if (e == hipSuccess) e = deep_prepare_dt<B, ELL, 2>(lds);
-    if (e == hipSuccess) e = deep_prepare_dt<B, ELL, 3>(lds);
+static hipError_t deep_prepare_t(int threads, size_t lds) {
+    hipError_t e = deep_prepare_dt<B, ELL, 0>(threads, lds);
+    if (e == hipSuccess) e = deep_prepare_dt<B, ELL, 1>(threads, lds);
+    if (e == hipSuccess) e = deep_prepare_dt<B, ELL, 2>(threads, lds);
+    if (e == hipSuccess) e = deep_prepare_dt<B, ELL, 3>(threads, lds);
+    if (e == hipSuccess) e = deep_prepare_dt<B, ELL, 4>(threads, lds);
     return e;
 }
 
 // Resident workgroups per CU: the least over the output variants.
 template <int B, int ELL>
 static hipError_t deep_occ_t(int threads, size_t lds, int *out) {
-    int o[4] = {0, 0, 0, 0};
-    const void *k[4] = {deep_kernel_for<B, ELL, 0>(threads), deep_kernel_for<B, ELL, 1>(threads),
-                        deep_kernel_for<B, ELL, 2>(threads), deep_kernel_for<B, ELL, 3>(threads)};
+    int o[5] = {1 << 30, 1 << 30, 1 << 30, 1 << 30, 1 << 30};
+    const void *k[5] = {deep_kernel_for<B, ELL, 0>(threads), deep_kernel_for<B, ELL, 1>(threads),
+                        deep_kernel_for<B, ELL, 2>(threads), deep_kernel_for<B, ELL, 3>(threads),
+                        deep_kernel_for<B, ELL, 4>(threads)};
     hipError_t e = hipSuccess;
-    for (int i = 0; i < 4 && e == hipSuccess; ++i)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&o[i], k[i], threads, lds);
-    *out = std::min(std::min(o[0], o[1]), std::min(o[2], o[3]));
+    for (int i = 0; i < 5 && e == hipSuccess; ++i)
+        if (k[i]) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&o[i], k[i], threads, lds);
+    *out = *std::min_element(o, o + 5);
     return e;
 }
 
@@ -863,7 +963,9 @@ hipError_t launch_deep(int bits, int ell, const DeepParams &p, int blocks, int t
     DEEP_SWITCH(deep_launch_t, p, blocks, threads, lds, st)
 }
 
-hipError_t deep_prepare(int bits, int ell, size_t lds) { DEEP_SWITCH(deep_prepare_t, lds) }
+hipError_t deep_prepare(int bits, int ell, int threads, size_t lds) {
+    DEEP_SWITCH(deep_prepare_t, threads, lds)
+}
 
 hipError_t deep_max_blocks_per_cu(int bits, int ell, int threads, size_t lds, int *out) {
     DEEP_SWITCH(deep_occ_t, threads, lds, out)
@@ -2094,6 +2196,149 @@ __global__ void __launch_bounds__(kWideThreads) __attribute__((amdgpu_waves_per_
     }
 }
 
+// ---------------------------------------------------------------- stepped wide
+//
+// The same levels as wide_bfs_kernel, one launch per phase: the kernel
+// boundaries take the place of the grid barriers, so no launch needs its
+// whole grid resident and other work (another lane's traversal) may share
+// the device.  wide_ctl_kernel (one thread) runs before every level: it
+// reads the previous level's counters, applies the direction rule, clears
+// the ring slots the level after next will use, and records the level in
+// WideCtl; once the frontier is empty it marks the traversal done and the
+// remaining enqueued launches return at once.
+
+__global__ void wide_ctl_kernel(WideParams p) {
+    WideCtl *ct = p.ctl;
+    if (ct->done) return;
+    const uint32_t L = ct->next, V = p.V, W = p.W;
+    uint32_t n_front, dir = ct->dir;
+    u64 prev_bits, prev_deg;
+    if (L == 0) {
+        n_front = p.ctr[3].next_count;
+        prev_bits = p.S;
+        prev_deg = 0;
+        dir = (p.policy == 2) ? DirPull : DirPush;
+    } else {
+        const WideCounters *c = &p.ctr[(L - 1) & 3];
+        n_front = c->next_count;
+        prev_bits = c->active_bits;
+        prev_deg = c->deg_sum;
+    }
+    if (n_front == 0 || L > V) {
+        ct->done = 1;
+        p.flags[FlagLevels] = L;
+        p.flags[FlagPushLevels] = ct->push_levels;
+        p.flags[FlagPullLevels] = ct->pull_levels;
+        return;
+    }
+    if (p.policy == 0 && L > 0) {
+        // the rule of wide_bfs_kernel (crates/gpu_bfs/src/dispatch/push_pull.rs:483-498)
+        u64 n_f, m_f;
+        if (p.rule == 1) {
+            n_f = (prev_bits + p.S - 1) / p.S;
+            m_f = V ? (n_f * p.E) / V : 0;
+        } else {
+            n_f = n_front;
+            m_f = prev_deg;
+        }
+        if (dir == DirPush) {
+            if (m_f > p.E / (p.alpha ? p.alpha : 1)) dir = DirPull;
+        } else {
+            if (n_f < (u64)V / (p.beta ? p.beta : 1)) dir = DirPush;
+        }
+    }
+    if (dir == DirPush)
+        ++ct->push_levels;
+    else
+        ++ct->pull_levels;
+    // slots of level L + 2, last used at level L - 1 (act) and L - 2 (ctr)
+    WideCounters *z = &p.ctr[(L + 2) & 3];
+    z->next_count = 0;
+    z->cand_count = 0;
+    z->active_bits = 0;
+    z->deg_sum = 0;
+    for (uint32_t w = 0; w < W; ++w) p.act[(size_t)((L + 2) % 3) * W + w] = 0;
+    ct->L = L;
+    ct->dir = dir;
+    ct->n_front = n_front;
+    ct->next = L + 1;
+    __threadfence();
+}
+
+// Phase A (push expand / pull) or phase B (push filter / pull resolve) of
+// level WideCtl::L; per-workgroup sums and active-source words are flushed
+// at the end of each phase.
+template <int TW, int PHASE>
+__global__ void __launch_bounds__(kWideThreads) __attribute__((amdgpu_waves_per_eu(4))) wide_step_kernel(WideParams p) {
+    extern __shared__ u64 lds_n[];
+    const WideCtl *ct = p.ctl;
+    if (ld_dev_u32(&ct->done)) return;
+    const uint32_t L = ld_dev_u32(&ct->L), dir = ld_dev_u32(&ct->dir);
+    const uint32_t n_front = ld_dev_u32(&ct->n_front);
+    constexpr uint32_t TV = 64 / TW;
+    WaveCtx x;
+    x.lane = lane_id();
+    x.vi = x.lane % TV;
+    x.wi = x.lane / TV;
+    x.gwave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / 64;
+    x.nwaves = (uint64_t)gridDim.x * blockDim.x / 64;
+    const uint64_t gthread = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    const uint64_t nthreads = (uint64_t)gridDim.x * blockDim.x;
+    const uint32_t V = p.V, W = p.W;
+    x.act_lds = lds_n + (size_t)p.pull_nb * (W + 1);
+    for (uint32_t w = threadIdx.x; w < W; w += blockDim.x) x.act_lds[w] = 0;
+    __shared__ uint32_t s_blk[4 + 64];
+    __shared__ uint32_t s_wq[kWideThreads];
+    __shared__ u64 s_sum[2];
+    if (threadIdx.x < 2) s_sum[threadIdx.x] = 0;
+    x.blk = s_blk;
+    x.wq = s_wq + (threadIdx.x / 64) * 64;
+    if (threadIdx.x < 4) s_blk[threadIdx.x] = 0;
+    __syncthreads();
+
+    const uint64_t stride = (uint64_t)V * W;
+    const uint32_t D = p.defer;
+    auto slot = [&](uint32_t d) { return p.fr + (d < D ? d : D + ((d - D) & 1u)) * stride; };
+    u64 *const cur = slot(L), *const nxt = slot(L + 1);
+    uint32_t *const fb_cur = (L & 1) ? p.fbB : p.fbA, *const fb_next = (L & 1) ? p.fbA : p.fbB;
+    uint32_t *const qcur = (L & 1) ? p.qf1 : p.qf0, *const qnext = (L & 1) ? p.qf0 : p.qf1;
+    WideCounters *const c = &p.ctr[L & 3];
+    const uint32_t nd = L + 1;
+    const uint32_t tick = p.tick_base + L + 1;
+    x.act_cur = p.act + (size_t)(L % 3) * W;
+    x.direct = p.out != nullptr && nd >= D;
+    u64 my_bits = 0, my_deg = 0;
+    if (PHASE == 0) {
+        if (dir == DirPush)
+            wide_push_expand<TW>(p, x, cur, qcur, n_front, fb_cur, c, tick);
+        else
+            wide_pull<TW>(p, x, cur, fb_cur, nxt, qnext, fb_next, c, nd, lds_n, my_bits, my_deg);
+    } else {
+        if (dir == DirPush)
+            wide_push_filter<TW>(p, x, cur, qcur, n_front, fb_cur, nxt, qnext, fb_next, c, nd,
+                                 gthread, nthreads, my_bits, my_deg);
+        else
+            wide_pull_resolve<TW>(p, x, cur, qcur, n_front, fb_cur, nxt, qnext, fb_next, c, nd,
+                                  tick, gthread, nthreads, my_bits, my_deg);
+    }
+    {
+        const u64 sb = wave_sum64(my_bits), sd = wave_sum64(my_deg);
+        if (x.lane == 0) {
+            if (sb) atomicAdd(&s_sum[0], sb);
+            if (sd) atomicAdd(&s_sum[1], sd);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (s_sum[0]) atomicAdd(&c->active_bits, s_sum[0]);
+        if (s_sum[1]) atomicAdd(&c->deg_sum, s_sum[1]);
+    }
+    for (uint32_t w = threadIdx.x; w < W; w += blockDim.x) {
+        const u64 a = x.act_lds[w];
+        if (a) atomicOr(&p.act[(size_t)((L + 1) % 3) * W + w], a);
+    }
+}
+
 // Final pass of a wide traversal, launched after wide_bfs_kernel on the same
 // stream (its own launch, so its registers and LDS are sized for it alone).
 template <typename T, typename T2>
@@ -2101,6 +2346,12 @@ __global__ void __launch_bounds__(kWideThreads) wide_final_kernel(WideParams p) 
     __shared__ uint32_t L_nib[kFinalWords * 8 * kFinalVerts];  // [kFinalWords][8][kFinalVerts]
     __shared__ u64 L_put[kFinalWords * kFinalVerts];            // [kFinalWords][kFinalVerts]
     if (p.flags[FlagAbort]) return;  // the traversal did not finish
+    if (p.ctl && !ld_dev_u32(&p.ctl->done)) {
+        // a stepped traversal that needs more levels than were enqueued:
+        // the host enqueues more and this pass again
+        if (blockIdx.x == 0 && threadIdx.x == 0) p.flags[FlagIncomplete] = 1;
+        return;
+    }
     const uint32_t V = p.V, W = p.W;
     const uint64_t stride = (uint64_t)V * W;
     const uint32_t kept = min(p.flags[FlagLevels], p.defer);
@@ -2226,6 +2477,27 @@ static hipError_t wide_launch_t(const WideParams &p, int blocks, int threads, si
 }
 
 template <int TW>
+static hipError_t wide_steps_t(const WideParams &p, int blocks, int threads, size_t lds,
+                               uint32_t levels, hipStream_t st) {
+    hipError_t e = hipFuncSetAttribute((const void *)wide_step_kernel<TW, 0>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void *)wide_step_kernel<TW, 1>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    for (uint32_t l = 0; l < levels && e == hipSuccess; ++l) {
+        hipLaunchKernelGGL(wide_ctl_kernel, dim3(1), dim3(1), 0, st, p);
+        hipLaunchKernelGGL((wide_step_kernel<TW, 0>), dim3(blocks), dim3(threads), lds, st, p);
+        hipLaunchKernelGGL((wide_step_kernel<TW, 1>), dim3(blocks), dim3(threads), lds, st, p);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(wide_ctl_kernel, dim3(1), dim3(1), 0, st, p);
+        e = hipGetLastError();
+    }
+    return e;
+}
+
+template <int TW>
 static hipError_t wide_occ_t(int threads, size_t lds, int *out) {
     hipError_t e = hipFuncSetAttribute((const void *)wide_bfs_kernel<TW>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -2248,6 +2520,11 @@ static hipError_t wide_occ_t(int threads, size_t lds, int *out) {
 hipError_t launch_wide(int tw, const WideParams &p, int blocks, int threads, size_t lds,
                        hipStream_t st) {
     WIDE_SWITCH(wide_launch_t, p, blocks, threads, lds, st)
+}
+
+hipError_t launch_wide_steps(int tw, const WideParams &p, int blocks, int threads, size_t lds,
+                             uint32_t levels, hipStream_t st) {
+    WIDE_SWITCH(wide_steps_t, p, blocks, threads, lds, levels, st)
 }
 
 hipError_t wide_max_blocks_per_cu(int tw, int threads, size_t lds, int *out) {

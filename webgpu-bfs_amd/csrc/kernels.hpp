@@ -44,8 +44,15 @@ struct DeepParams {
     uint32_t reach_vw;     // words per row: ceil(|V| / 32)
     uint32_t ntasks;
     uint32_t out_kind;      // output elements: 0 u32, 1 u16, 2 u8
-    uint32_t pad;
+    // u32 rows written through a binned log (one-wave kernel, one source per
+    // task): vertices are binned by v >> log_bw; 0 = direct stores.
+    uint32_t log_bw;
+    uint32_t lds_bin_off;   // LDS word offset of the per-bin log counters
+    uint32_t *log_sink;     // 64 words per workgroup: stores of lanes without a log entry
+    uint32_t lds_sink_off;  // one-wave kernel: LDS word offset of its 64 sink entries
 };
+// Vertex bins of the logged output: 4,096 vertices (16 KiB of u32 in LDS).
+constexpr uint32_t kDeepLogBits = 12;
 
 // ---------------------------------------------------------------- wide family
 struct WideCounters {
@@ -97,6 +104,19 @@ struct WideParams {
     uint32_t *gbar;         // grid barrier words (kGridBarrierWords, zero at launch)
     uint32_t trace_cap;     // levels the trace buffer holds
     unsigned long long *trace;  // per level: start clock, dir<<32|frontier, mid clock, new bits
+    struct WideCtl *ctl;    // stepped launches: level control (null for the cooperative kernel)
+};
+
+// Level control of a stepped wide traversal (one launch per phase of each
+// level instead of one cooperative kernel): written by wide_ctl_kernel
+// before every level, read by the phase kernels.
+struct WideCtl {
+    uint32_t L;             // level the phase kernels run
+    uint32_t next;          // level wide_ctl_kernel decides next
+    uint32_t dir;           // DirPush / DirPull of level L
+    uint32_t n_front;       // frontier vertices of level L
+    uint32_t done;          // no frontier left: the remaining launches return at once
+    uint32_t push_levels, pull_levels, pad;
 };
 
 // Degree above which the wide family gives a vertex's edge list to a whole
@@ -123,6 +143,7 @@ enum : int {
     FlagPushLevels = 3,
     FlagPullLevels = 4,
     FlagAbort = 5,          // a grid barrier wait timed out
+    FlagIncomplete = 6,     // stepped launch: more levels than were enqueued
     FlagCount = 8
 };
 
@@ -133,10 +154,15 @@ hipError_t deep_max_blocks_per_cu(int bits, int ell, int threads, size_t lds_byt
 // sink), reserved out of the per-workgroup budget; deep_prepare fails if the
 // compiled kernel needs more.
 constexpr size_t kDeepStaticLds = 1024;
-hipError_t deep_prepare(int bits, int ell, size_t lds_bytes);
+hipError_t deep_prepare(int bits, int ell, int threads, size_t lds_bytes);
 
 hipError_t launch_wide(int tile_words, const WideParams &p, int blocks, int threads, size_t lds,
                        hipStream_t st);
+// Stepped wide traversal: `levels` x (control, phase A, phase B) launches on
+// `st`, then one more control launch; no grid barrier, so the launches need
+// no co-resident grid and may share the device with other work.
+hipError_t launch_wide_steps(int tile_words, const WideParams &p, int blocks, int threads,
+                             size_t lds, uint32_t levels, hipStream_t st);
 hipError_t wide_max_blocks_per_cu(int tile_words, int threads, size_t lds, int *out);
 hipError_t launch_wide_seed(const WideParams &p, hipStream_t st);
 hipError_t launch_reach_transpose(const uint32_t *rt, uint32_t vw_n, uint64_t S, uint32_t V,

@@ -5,7 +5,7 @@ Mirrors the reference's `gpu_bfs` and `webgpu_bfs_core` crates
 hop distances dist[s*|V|+v] out.  All traversal runs in hand-written HIP
 kernels for gfx950 behind the C ABI in include/msbfs.h.
 """
-from ._lib import (DIST_U8, DIST_U16, DIST_U32, LIB_PATH, MsbfsError, MtxParseError, NoDeviceError,
+from ._lib import (DIST_U8, DIST_U16, DIST_U32, LIB_PATH, MsbfsError, MtxParseError, NoDeviceError, shutdown,
                    SourceOutOfBoundsError, lib)
 from .core import (CscGraph, CsrGraph, ReachabilityBitmask, csr_from_undirected_pairs,
                    generate_erdos_renyi_np, generate_graph500_rmat, generate_grid2d,
@@ -21,5 +21,5 @@ __all__ = [
     "MtxParseError", "parse_mtx", "read_mtx", "write_mtx",
     "generate_grid2d", "generate_road_like", "generate_rmat", "generate_graph500_rmat",
     "generate_erdos_renyi_np", "csr_from_undirected_pairs", "sample_sources", "traversed_edges",
-    "DIST_U32", "DIST_U16", "DIST_U8", "LIB_PATH", "lib",
+    "DIST_U32", "DIST_U16", "DIST_U8", "LIB_PATH", "lib", "shutdown",
 ]

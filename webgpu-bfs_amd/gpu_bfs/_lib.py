@@ -105,7 +105,7 @@ class GraphInfo(ctypes.Structure):
 class Tuning(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint32) for n in (
         "alpha", "beta", "tile_words", "words_per_lane", "deep_bits", "deep_threads",
-        "wide_blocks_per_cu", "direction_rule", "trace_levels")]
+        "wide_blocks_per_cu", "direction_rule", "trace_levels", "stepped")]
 
 
 class LevelRecord(ctypes.Structure):
@@ -170,6 +170,7 @@ SIGNATURES = {
     "msbfs_engine_method": (I32, [P]),
     "msbfs_engine_set_push_pull": (I32, [P, U32, U32]),
     "msbfs_engine_set_tuning": (I32, [P, ctypes.POINTER(Tuning)]),
+    "msbfs_tuning_validate": (I32, [ctypes.POINTER(Tuning)]),
     "msbfs_engine_get_tuning": (I32, [P, ctypes.POINTER(Tuning)]),
     "msbfs_engine_last_stats": (I32, [P, ctypes.POINTER(RunStats)]),
     "msbfs_engine_level_trace": (I32, [P, P, U32, ctypes.POINTER(U32)]),
@@ -226,6 +227,38 @@ SIGNATURES = {
 
 _lib = None
 
+# Live library objects by teardown rank: servers first (they hold engines,
+# graphs and a worker thread), then batches, engines, graphs, contexts.
+RANK = {"server": 0, "batch": 1, "engine": 2, "graph": 3, "context": 4}
+_live: list = []  # (rank, weakref.finalize) in creation order
+_shutdown_registered = False
+
+
+def track(kind: str, obj, destroy, handle):
+    """Destroys `handle` with `destroy` when `obj` is collected or at
+    shutdown(), whichever comes first; returns the finalizer."""
+    import weakref
+
+    fin = weakref.finalize(obj, destroy, handle)
+    fin.atexit = False  # shutdown() orders these itself
+    _live.append((RANK[kind], fin))
+    if len(_live) > 4096:
+        _live[:] = [x for x in _live if x[1].alive]
+    return fin
+
+
+def shutdown() -> None:
+    """Destroys every live server, batch, engine, graph and context, in that
+    order, while the HIP runtime is still up.  Registered with atexit right
+    after torch is imported, so it runs before torch's own exit handlers (and
+    before any profiler's, which run at C exit); safe to call more than once.
+    Objects created afterwards are tracked again."""
+    live = sorted(((r, i, f) for i, (r, f) in enumerate(_live) if f.alive),
+                  key=lambda x: (x[0], -x[1]))
+    _live.clear()
+    for _, _, fin in live:
+        fin()
+
 
 def _bind_one_hip_runtime() -> None:
     """PyTorch-ROCm wheels ship their own libamdhip64.so.7, with the same
@@ -255,6 +288,12 @@ def lib():
             fn.restype = res
             fn.argtypes = args
         _lib = L
+        global _shutdown_registered
+        if not _shutdown_registered:
+            import atexit
+
+            atexit.register(shutdown)
+            _shutdown_registered = True
     return _lib
 
 

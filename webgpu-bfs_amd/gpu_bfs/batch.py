@@ -8,7 +8,6 @@ has no counterpart: it makes one engine call per graph and source set
 from __future__ import annotations
 
 import ctypes
-import weakref
 
 import numpy as np
 
@@ -35,7 +34,7 @@ class Batch:
             flat.ctypes.data_as(ctypes.c_void_p) if flat.size else None, self.dtype,
             ctypes.byref(h)))
         self._h = h
-        self._finalizer = weakref.finalize(self, L.lib().msbfs_batch_destroy, h)
+        self._finalizer = L.track("batch", self, L.lib().msbfs_batch_destroy, h)
         total = ctypes.c_uint64()
         self.offsets = np.zeros(max(n, 1), dtype=np.uint64)
         L.check(L.lib().msbfs_batch_layout(h, ctypes.byref(total),

@@ -51,7 +51,7 @@ class GpuContext:
         L.check(L.lib().msbfs_context_create(device, ctypes.byref(h)))
         self._h = h
         self.device = device
-        self._finalizer = weakref.finalize(self, L.lib().msbfs_context_destroy, h)
+        self._finalizer = L.track("context", self, L.lib().msbfs_context_destroy, h)
 
     @classmethod
     def init_blocking(cls, device: int = 0) -> "GpuContext":
@@ -93,7 +93,7 @@ class DeviceGraph:
         self.ctx = ctx
         self.num_vertices = graph.num_vertices
         self.num_edges = graph.num_edges
-        self._finalizer = weakref.finalize(self, L.lib().msbfs_graph_destroy, h)
+        self._finalizer = L.track("graph", self, L.lib().msbfs_graph_destroy, h)
 
     @property
     def handle(self):
@@ -128,7 +128,7 @@ class GpuBfsEngine:
         self._method = GpuBfsMethod(method)
         self._graphs: "weakref.WeakKeyDictionary[CsrGraph, tuple]" = weakref.WeakKeyDictionary()
         self._lock = threading.Lock()
-        self._finalizer = weakref.finalize(self, L.lib().msbfs_engine_destroy, h)
+        self._finalizer = L.track("engine", self, L.lib().msbfs_engine_destroy, h)
 
     # constructors mirroring engine.rs:68-154
     @classmethod

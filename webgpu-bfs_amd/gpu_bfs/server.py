@@ -7,7 +7,6 @@ The reference has no serving path; its engine serves one call at a time
 from __future__ import annotations
 
 import ctypes
-import weakref
 
 import numpy as np
 
@@ -32,7 +31,7 @@ class Server:
         self._h = h
         self.ctx = ctx
         self.graphs = []
-        self._finalizer = weakref.finalize(self, L.lib().msbfs_server_destroy, h)
+        self._finalizer = L.track("server", self, L.lib().msbfs_server_destroy, h)
 
     def add_graph(self, graph, method: GpuBfsMethod = GpuBfsMethod.Auto) -> int:
         dg = graph if isinstance(graph, DeviceGraph) else DeviceGraph(self.ctx, graph)
