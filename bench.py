@@ -556,6 +556,8 @@ def bench_serve(args):
         span = st["span_s"]
         r = {"offered_qps": rate if rate else "saturation", "queries": n,
              "launches": int(st["launches"]),
+             "overlapped_launches": int(st["overlapped_launches"]),
+             "max_in_flight": int(st["max_in_flight"]),
              "p50_ms": round(float(np.percentile(lat, 50)), 3),
              "p99_ms": round(float(np.percentile(lat, 99)), 3),
              "achieved_qps": round(n / span, 1),
@@ -579,7 +581,8 @@ def bench_serve(args):
             "config": {"workload": "serve", "graphs": ["road-like 114,599 V",
                                                        f"rmat{args.serve_rmat_scale}"],
                        "queries_per_rate": n, "k": "log-uniform [1, 512]",
-                       "admission": "u16 x k x |V| output bytes", "pool_bytes": pool}}
+                       "admission": "u16 x k x |V| output bytes", "pool_bytes": pool,
+                       "slots_per_lane": 4, "wide_lanes": "stepped launches"}}
 
 
 # ------------------------------------------------------------------ main

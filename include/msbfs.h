@@ -292,16 +292,23 @@ int msbfs_batch_autotune(msbfs_batch *b, void *out_device, uint64_t out_len, msb
 /* ---- serving ----
  * Resident graphs ("lanes") and a scheduler thread that packs queries of
  * 1..512 sources into shared 64-bit source words, admitting them by the
- * device memory their distance outputs need (continuous batching: queries
- * arriving while a lane's traversal runs join its next traversal; lanes run
- * concurrently).  Results stay in device memory until released.  No
- * counterpart in the reference. */
+ * device memory their distance outputs need.  Continuous batching: each
+ * lane has several launch slots (engine + stream); a query arriving while
+ * the lane's traversals run is launched as soon as a slot and output memory
+ * are free, without waiting for the running traversals, and the traversals
+ * of every lane and slot run side by side on the device (wide-family lanes
+ * use stepped launches, which need no co-resident grid).  Queries that
+ * arrive while every slot is busy are packed into the next launch.  Results
+ * stay in device memory until released.  No counterpart in the reference
+ * (its engine serves one call at a time, crates/gpu_bfs/src/engine.rs:61-63). */
 typedef struct msbfs_server_config {
     uint64_t output_pool_bytes;       /* device memory for results; 0 = 16 GiB */
     uint32_t max_sources_per_launch;  /* 0 = 2048 */
     uint32_t max_queries_per_launch;  /* 0 = 512 */
     int32_t dist_type;                /* MSBFS_DIST_U32 or MSBFS_DIST_U16 */
     int32_t pack;                     /* 1: many queries per traversal; 0: one per traversal */
+    uint32_t slots_per_lane;          /* traversals of one lane in flight at once; 0 = 4 */
+    uint32_t reserved;
 } msbfs_server_config;
 
 typedef struct msbfs_query_info {
@@ -321,6 +328,8 @@ typedef struct msbfs_server_stats {
     uint64_t pool_bytes, pool_used;
     double span_s;        /* first submit -> last completion */
     double p50_ms, p99_ms, mean_ms, max_ms;
+    uint64_t overlapped_launches; /* launches started while another traversal was in flight */
+    uint64_t max_in_flight;       /* most traversals in flight at once (all lanes) */
 } msbfs_server_stats;
 
 int msbfs_server_create(msbfs_context *ctx, const msbfs_server_config *cfg, msbfs_server **out);

@@ -585,3 +585,42 @@ def test_every_method_enum_matches_oracle(ctx):
                     assert st["bits_per_instance"] == 1
                 if m == M.BatchedDenseSpMM:
                     assert st["family"] == int(gb.Family.WIDE) and st["push_levels"] == 0
+
+
+def test_stepped_wide_matches_oracle(ctx):
+    """The stepped wide launch (one launch per phase of each level, no grid
+    barrier; the serving loop's mode) against the oracle on the sweep's
+    graphs, S at word boundaries, push-only, pull-only and adaptive, u32
+    (device and host), u16 and reachability.  The hub-and-path graph needs
+    more levels than the first batch of launches enqueues on a new graph, so
+    the continuation runs; later runs on that graph enqueue enough at once.
+    The results also match the cooperative kernel's."""
+    import torch
+
+    graphs = {
+        "rmat13": gb.generate_rmat(13, 16, seed=11),
+        "rmat12d": gb.generate_rmat(12, 16, seed=12, directed=True),
+        "hubpath": hub_and_path(),
+        "er6000": gb.generate_erdos_renyi_np(6000, 3.0 / 5999, seed=13),
+    }
+    for gname, g in graphs.items():
+        dg = gb.DeviceGraph(ctx, g)
+        for S in (1, 64, 130, 700):
+            src = gb.sample_sources(g, S, seed=S + 3)
+            d32 = oracle(g, src)
+            for m in (M.PushPullAdaptive, M.BitParallel64Pull, M.BitParallel64):
+                e = gb.GpuBfsEngine(ctx, m)
+                e.set_tuning(stepped=1)
+                say("stepped", gname, S, m.name)
+                out = torch.full((S * g.num_vertices,), 9, dtype=torch.int32, device="cuda")
+                e.run_ms_bfs_device(dg, src, out.data_ptr(), out.numel())
+                assert np.array_equal(out.cpu().numpy().view(np.uint32), d32), (gname, S, m.name)
+                st = e.last_stats()
+                assert st["family"] == int(gb.Family.WIDE)
+                assert st["levels"] == int(d32[d32 != INF32].max(initial=0)) + 1, st
+                assert np.array_equal(e.run_ms_bfs(dg, src), d32)
+            e = gb.GpuBfsEngine(ctx, M.PushPullAdaptive)
+            e.set_tuning(stepped=1)
+            assert np.array_equal(e.run_ms_bfs(dg, src, dist_dtype=np.uint16), to_u16_expect(d32))
+            bm = e.run_ms_reachability(dg, src)
+            assert np.array_equal(bm.words, orc.reachability_from_dist(g.num_vertices, S, d32))

@@ -123,14 +123,16 @@ def test_admission_waits_for_memory(ctx, graphs):
 
 
 def test_cooperative_lane_beside_deep_lanes(ctx, graphs):
-    """A lane on the cooperative (grid-barrier) family shares the server with
-    deep-family lanes; the library keeps each cooperative traversal alone on
-    the device, and every result is still the oracle's."""
+    """Lanes on the wide family share the server with deep-family lanes: one
+    with stepped launches (the server's default) and one forced back to the
+    cooperative grid-barrier kernel, which the library keeps alone on the
+    device; every result is still the oracle's."""
     road, rmat = graphs
     s = gb.Server(ctx, dist_dtype=np.uint16, pool_bytes=1 << 30)
     s.add_graph(road)                                     # deep
-    s.add_graph(rmat, method=gb.GpuBfsMethod.PushPullAdaptive)  # wide
-    s.add_graph(rmat)                                     # deep (Auto)
+    s.add_graph(rmat, method=gb.GpuBfsMethod.PushPullAdaptive)  # wide, stepped
+    s.add_graph(rmat)                                     # wide (Auto), cooperative below
+    s.set_tuning(2, stepped=0)
     s.start()
     rng = np.random.default_rng(7)
     qs = []
@@ -151,3 +153,48 @@ def test_cooperative_lane_beside_deep_lanes(ctx, graphs):
     assert st["completed"] == len(qs) and st["failed"] == 0
     for qid, (lane, src) in zip(ids, qs):
         assert np.array_equal(got[qid], expect([road, rmat, rmat], lane, src)), lane
+
+
+def test_mid_flight_admission(ctx, graphs):
+    """Queries submitted while a long traversal of their lane runs are
+    admitted at once into free launch slots: each starts (queue_ms) before
+    the long one finishes, the server reports launches started while others
+    were in flight, and every result equals the oracle's, alone or mixed."""
+    from gpu_bfs import workloads as wl
+
+    big_g = wl.road()
+    lanes = [big_g] + list(graphs)
+    s = gb.Server(ctx, dist_dtype=np.uint16, pool_bytes=4 << 30)
+    for g in lanes:
+        s.add_graph(g)
+    s.start()
+    big_src = wl.sources(big_g, 2048)
+    rng = np.random.default_rng(11)
+    small = []
+    for i in range(12):
+        lane = i % 3
+        g = lanes[lane]
+        pool = np.nonzero(g.degrees() > 0)[0]
+        small.append((lane, rng.choice(pool, size=int(rng.integers(1, 40))).astype(np.uint32)))
+    say("mid-flight: four 512-source road queries, then", len(small), "small queries")
+    bigs = [s.submit(0, big_src[i:i + 512]) for i in range(0, 2048, 512)]
+    ids = [s.submit(lane, src) for lane, src in small]
+    got = {}
+    infos = {}
+    for _ in range(len(bigs) + len(small)):
+        qid = s.next_completed(120.0)
+        infos[qid] = s.info(qid)
+        assert infos[qid]["state"] == 2, infos[qid]
+        got[qid] = s.result(qid)
+        s.release(qid)
+    st = s.stats()
+    s.stop()
+    say("mid-flight stats", st, "big", [infos[b] for b in bigs])
+    assert st["overlapped_launches"] >= 1 and st["max_in_flight"] >= 2, st
+    t_big = min(infos[b]["latency_ms"] for b in bigs)
+    started_early = [q for q in ids if infos[q]["queue_ms"] < t_big]
+    assert len(started_early) >= len(ids) // 2, ([infos[q]["queue_ms"] for q in ids], t_big)
+    for i, b in enumerate(bigs):
+        assert np.array_equal(got[b], expect(lanes, 0, big_src[512 * i:512 * (i + 1)]))
+    for qid, (lane, src) in zip(ids, small):
+        assert np.array_equal(got[qid], expect(lanes, lane, src)), lane

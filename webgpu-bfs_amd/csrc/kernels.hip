@@ -2207,30 +2207,35 @@ __global__ void __launch_bounds__(kWideThreads) __attribute__((amdgpu_waves_per_
 // WideCtl; once the frontier is empty it marks the traversal done and the
 // remaining enqueued launches return at once.
 
-__global__ void wide_ctl_kernel(WideParams p) {
+// `probe` (the launch after the last enqueued level) only marks the
+// traversal done when no frontier is left; it decides no level, so launches
+// enqueued later continue from the next level.
+__global__ void wide_ctl_kernel(WideParams p, uint32_t probe) {
     WideCtl *ct = p.ctl;
-    if (ct->done) return;
-    const uint32_t L = ct->next, V = p.V, W = p.W;
-    uint32_t n_front, dir = ct->dir;
+    if (ld_dev_u32(&ct->done)) return;
+    const uint32_t L = ld_dev_u32(&ct->next), V = p.V, W = p.W;
+    uint32_t n_front, dir = ld_dev_u32(&ct->dir);
     u64 prev_bits, prev_deg;
     if (L == 0) {
-        n_front = p.ctr[3].next_count;
+        n_front = ld_dev_u32(&p.ctr[3].next_count);
         prev_bits = p.S;
         prev_deg = 0;
         dir = (p.policy == 2) ? DirPull : DirPush;
     } else {
-        const WideCounters *c = &p.ctr[(L - 1) & 3];
-        n_front = c->next_count;
-        prev_bits = c->active_bits;
-        prev_deg = c->deg_sum;
+        WideCounters *c = &p.ctr[(L - 1) & 3];
+        n_front = ld_dev_u32(&c->next_count);
+        prev_bits = ld_dev_u64(&c->active_bits);
+        prev_deg = ld_dev_u64(&c->deg_sum);
     }
     if (n_front == 0 || L > V) {
         ct->done = 1;
         p.flags[FlagLevels] = L;
         p.flags[FlagPushLevels] = ct->push_levels;
         p.flags[FlagPullLevels] = ct->pull_levels;
+        __threadfence();
         return;
     }
+    if (probe) return;
     if (p.policy == 0 && L > 0) {
         // the rule of wide_bfs_kernel (crates/gpu_bfs/src/dispatch/push_pull.rs:483-498)
         u64 n_f, m_f;
@@ -2485,13 +2490,13 @@ static hipError_t wide_steps_t(const WideParams &p, int blocks, int threads, siz
         e = hipFuncSetAttribute((const void *)wide_step_kernel<TW, 1>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     for (uint32_t l = 0; l < levels && e == hipSuccess; ++l) {
-        hipLaunchKernelGGL(wide_ctl_kernel, dim3(1), dim3(1), 0, st, p);
+        hipLaunchKernelGGL(wide_ctl_kernel, dim3(1), dim3(1), 0, st, p, 0u);
         hipLaunchKernelGGL((wide_step_kernel<TW, 0>), dim3(blocks), dim3(threads), lds, st, p);
         hipLaunchKernelGGL((wide_step_kernel<TW, 1>), dim3(blocks), dim3(threads), lds, st, p);
         e = hipGetLastError();
     }
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(wide_ctl_kernel, dim3(1), dim3(1), 0, st, p);
+        hipLaunchKernelGGL(wide_ctl_kernel, dim3(1), dim3(1), 0, st, p, 1u);
         e = hipGetLastError();
     }
     return e;

@@ -1,17 +1,23 @@
 // Serving loop: resident graphs, continuous batching of BFS queries.
 //
-// Each resident graph is a lane with its own engine workspace and HIP
-// stream.  Queries (a graph and 1..512 sources) wait in the lane's FIFO.
-// Whenever a lane has no traversal in flight, the scheduler thread admits
-// the longest prefix of the FIFO whose distance outputs fit in the output
-// pool (u16/u32 x k x |V| bytes each) and whose sources fit the launch cap,
-// packs their sources back to back into the launch's 64-bit source words
-// (so queries share words), and enqueues one traversal whose output rows
-// land directly in the admitted queries' pool slices.  Queries that arrive
-// meanwhile join the next launch of their lane; the lanes run concurrently
-// on separate streams, so traversals of different graphs overlap on the
-// device.  Results stay in device memory until the client releases them,
-// which returns their bytes to the pool and lets waiting queries in.
+// Each resident graph is a lane with several launch slots, each an engine
+// workspace and a HIP stream.  Queries (a graph and 1..512 sources) wait in
+// the lane's FIFO.  Whenever a lane has a free slot, the scheduler thread
+// admits the longest prefix of the FIFO whose distance outputs fit in the
+// output pool (u16/u32 x k x |V| bytes each) and whose sources fit the
+// launch cap, packs their sources back to back into the launch's 64-bit
+// source words (so queries share words), and enqueues one traversal whose
+// output rows land directly in the admitted queries' pool slices.  A query
+// that arrives while the lane's traversals run therefore starts as soon as
+// a slot and its output memory are free, not when the running traversals
+// end: its kernels take the workgroup slots the running ones give up (a
+// persistent deep launch retires workgroups as its task queue drains) and
+// run beside them.  Queries arriving while every slot is busy accumulate
+// and are packed into the next launch.  Wide-family lanes run stepped
+// launches (no grid barrier, no co-resident grid needed), so they share the
+// device with the other lanes too.  Results stay in device memory until the
+// client releases them, which returns their bytes to the pool and lets
+// waiting queries in.
 //
 // The reference has no serving path; its engine serves one call at a time
 // (crates/gpu_bfs/src/engine.rs:61-63 serialises calls on pool mutexes).
@@ -88,16 +94,22 @@ struct Query {
     uint64_t launch = 0;
 };
 
-struct Lane {
-    const msbfs_graph *g = nullptr;
+// One launch slot of a lane: its own engine workspace, stream and
+// completion event, and the queries of the traversal it runs.
+struct Slot {
     msbfs_engine *eng = nullptr;
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
-    std::deque<uint64_t> pending;
     std::vector<uint64_t> running;
     bool inflight = false;
-    uint64_t launches = 0, sources = 0;
     std::vector<uint32_t> packed;
+};
+
+struct Lane {
+    const msbfs_graph *g = nullptr;
+    std::vector<Slot> slots;
+    std::deque<uint64_t> pending;
+    uint64_t launches = 0, sources = 0;
 };
 
 }  // namespace
@@ -115,6 +127,7 @@ struct msbfs_server {
     std::thread worker;
     bool running = false, stopping = false;
     uint64_t completed = 0, failed = 0, launches = 0, sources_done = 0;
+    uint64_t overlapped = 0, max_in_flight = 0, in_flight = 0;
     Clock::time_point t_first_submit, t_last_done;
     bool any_submit = false;
     std::vector<double> latencies_ms;
@@ -127,81 +140,92 @@ namespace {
 
 size_t elem_bytes(const msbfs_server *s) { return s->cfg.dist_type == MSBFS_DIST_U16 ? 2 : 4; }
 
-// Admits and launches work on every idle lane; called with the lock held.
-// A launch takes the longest FIFO prefix that fits the source and query caps
-// (one query when packing is off) and whose outputs fit one block of the
-// pool; the block is carved into the queries' slices in order, so query i's
-// rows follow query i-1's and one packed traversal fills them all.
+// Admits and launches work on every free slot of every lane; called with
+// the lock held.  A launch takes the longest FIFO prefix that fits the source
+// and query caps (one query when packing is off) and whose outputs fit one
+// block of the pool; the block is carved into the queries' slices in order,
+// so query i's rows follow query i-1's and one packed traversal fills them
+// all.
 void schedule(msbfs_server *s) {
     for (uint32_t li = 0; li < s->lanes.size(); ++li) {
         Lane &L = s->lanes[li];
-        if (L.inflight || L.pending.empty()) continue;
-        std::vector<uint64_t> take;
-        uint64_t nsrc = 0, total = 0;
-        for (uint64_t id : L.pending) {
-            const Query &q = s->queries[id];
-            const uint64_t k = q.sources.size();
-            if (!take.empty() && (nsrc + k > s->cfg.max_sources_per_launch ||
-                                  take.size() >= s->cfg.max_queries_per_launch || !s->cfg.pack))
-                break;
-            take.push_back(id);
-            nsrc += k;
-            total += q.bytes;
-        }
-        uint64_t base = 0;
-        while (!take.empty() && !s->pool.alloc(total, &base)) {
-            const Query &q = s->queries[take.back()];
-            total -= q.bytes;
-            nsrc -= q.sources.size();
-            take.pop_back();
-        }
-        if (take.empty()) continue;  // the head query waits for released memory
-        L.pending.erase(L.pending.begin(), L.pending.begin() + take.size());
-        L.packed.clear();
-        const Clock::time_point now = Clock::now();
-        uint64_t off = base;
-        for (uint64_t id : take) {
-            Query &q = s->queries[id];
-            q.off = off;
-            off += q.bytes;
-            q.state = 1;
-            q.t_start = now;
-            q.launch = s->launches;
-            L.packed.insert(L.packed.end(), q.sources.begin(), q.sources.end());
-        }
-        int rc = msbfs_run_ms_bfs_device(L.eng, L.g, L.packed.data(), L.packed.size(),
-                                         s->cfg.dist_type, (char *)s->pool_buf.p + base,
-                                         L.packed.size() * L.g->V, L.stream, 0);
-        if (rc == MSBFS_OK && hipEventRecord(L.done, L.stream) != hipSuccess) rc = MSBFS_ERR_HIP;
-        if (rc != MSBFS_OK) {
-            const std::string msg = msbfs_last_error();
+        for (Slot &sl : L.slots) {
+            if (L.pending.empty()) break;
+            if (sl.inflight) continue;
+            std::vector<uint64_t> take;
+            uint64_t nsrc = 0, total = 0;
+            for (uint64_t id : L.pending) {
+                const Query &q = s->queries[id];
+                const uint64_t k = q.sources.size();
+                if (!take.empty() && (nsrc + k > s->cfg.max_sources_per_launch ||
+                                      take.size() >= s->cfg.max_queries_per_launch ||
+                                      !s->cfg.pack))
+                    break;
+                take.push_back(id);
+                nsrc += k;
+                total += q.bytes;
+            }
+            uint64_t base = 0;
+            while (!take.empty() && !s->pool.alloc(total, &base)) {
+                const Query &q = s->queries[take.back()];
+                total -= q.bytes;
+                nsrc -= q.sources.size();
+                take.pop_back();
+            }
+            if (take.empty()) break;  // the head query waits for released memory
+            L.pending.erase(L.pending.begin(), L.pending.begin() + take.size());
+            sl.packed.clear();
+            const Clock::time_point now = Clock::now();
+            uint64_t off = base;
             for (uint64_t id : take) {
                 Query &q = s->queries[id];
-                q.state = 3;
-                q.status = rc;
-                q.error = msg;
-                q.t_done = Clock::now();
-                s->pool.release(q.off, q.bytes);
-                ++s->failed;
-                s->completions.push_back(id);
+                q.off = off;
+                off += q.bytes;
+                q.state = 1;
+                q.t_start = now;
+                q.launch = s->launches;
+                sl.packed.insert(sl.packed.end(), q.sources.begin(), q.sources.end());
             }
-            s->cv_done.notify_all();
-            continue;
+            int rc = msbfs_run_ms_bfs_device(sl.eng, L.g, sl.packed.data(), sl.packed.size(),
+                                             s->cfg.dist_type, (char *)s->pool_buf.p + base,
+                                             sl.packed.size() * L.g->V, sl.stream, 0);
+            if (rc == MSBFS_OK && hipEventRecord(sl.done, sl.stream) != hipSuccess)
+                rc = MSBFS_ERR_HIP;
+            if (rc != MSBFS_OK) {
+                const std::string msg = msbfs_last_error();
+                for (uint64_t id : take) {
+                    Query &q = s->queries[id];
+                    q.state = 3;
+                    q.status = rc;
+                    q.error = msg;
+                    q.t_done = Clock::now();
+                    s->pool.release(q.off, q.bytes);
+                    ++s->failed;
+                    s->completions.push_back(id);
+                }
+                s->cv_done.notify_all();
+                continue;
+            }
+            if (s->in_flight) ++s->overlapped;
+            ++s->in_flight;
+            s->max_in_flight = std::max(s->max_in_flight, s->in_flight);
+            if (s->log) {
+                int m = 0;
+                const int fam = msbfs_engine_resolve(sl.eng, L.g, sl.packed.size(), &m);
+                fprintf(stderr, "[msbfs-server] %.3f ms launch #%llu lane %u slot %zu: %zu queries, "
+                        "%zu sources, family %d, %llu in flight, pool %.2f/%.2f GiB\n",
+                        std::chrono::duration<double, std::milli>(Clock::now() - s->t0).count(),
+                        (unsigned long long)s->launches, li, (size_t)(&sl - L.slots.data()),
+                        take.size(), sl.packed.size(), fam, (unsigned long long)s->in_flight,
+                        s->pool.used() / 1073741824.0, s->pool.size() / 1073741824.0);
+                fflush(stderr);
+            }
+            sl.running = take;
+            sl.inflight = true;
+            L.launches++;
+            L.sources += nsrc;
+            s->launches++;
         }
-        if (s->log) {
-            int m = 0;
-            const int fam = msbfs_engine_resolve(L.eng, L.g, L.packed.size(), &m);
-            fprintf(stderr, "[msbfs-server] %.3f ms launch #%llu lane %u: %zu queries, %zu sources, family %d, pool %.2f/%.2f GiB\n",
-                    std::chrono::duration<double, std::milli>(Clock::now() - s->t0).count(),
-                    (unsigned long long)s->launches, li, take.size(), L.packed.size(), fam,
-                    s->pool.used() / 1073741824.0, s->pool.size() / 1073741824.0);
-            fflush(stderr);
-        }
-        L.running = take;
-        L.inflight = true;
-        L.launches++;
-        L.sources += nsrc;
-        s->launches++;
     }
 }
 
@@ -209,39 +233,43 @@ void schedule(msbfs_server *s) {
 bool reap(msbfs_server *s) {
     bool any = false;
     for (Lane &L : s->lanes) {
-        if (!L.inflight) continue;
-        if (hipEventQuery(L.done) != hipSuccess) continue;
-        const int rc = msbfs_engine_check(L.eng);
-        const std::string msg = rc ? msbfs_last_error() : "";
-        const Clock::time_point now = Clock::now();
-        for (uint64_t id : L.running) {
-            Query &q = s->queries[id];
-            q.t_done = now;
-            s->completions.push_back(id);
-            if (rc == MSBFS_OK) {
-                q.state = 2;
-                ++s->completed;
-                s->sources_done += q.sources.size();
-                s->latencies_ms.push_back(
-                    std::chrono::duration<double, std::milli>(q.t_done - q.t_submit).count());
-            } else {
-                q.state = 3;
-                q.status = rc;
-                q.error = msg;
-                s->pool.release(q.off, q.bytes);
-                ++s->failed;
+        for (Slot &sl : L.slots) {
+            if (!sl.inflight) continue;
+            if (hipEventQuery(sl.done) != hipSuccess) continue;
+            const int rc = msbfs_engine_check(sl.eng);
+            const std::string msg = rc ? msbfs_last_error() : "";
+            const Clock::time_point now = Clock::now();
+            for (uint64_t id : sl.running) {
+                Query &q = s->queries[id];
+                q.t_done = now;
+                s->completions.push_back(id);
+                if (rc == MSBFS_OK) {
+                    q.state = 2;
+                    ++s->completed;
+                    s->sources_done += q.sources.size();
+                    s->latencies_ms.push_back(
+                        std::chrono::duration<double, std::milli>(q.t_done - q.t_submit).count());
+                } else {
+                    q.state = 3;
+                    q.status = rc;
+                    q.error = msg;
+                    s->pool.release(q.off, q.bytes);
+                    ++s->failed;
+                }
             }
+            s->t_last_done = now;
+            if (s->log) {
+                fprintf(stderr, "[msbfs-server] %.3f ms retire lane %zu slot %zu: %zu queries, status %d\n",
+                        std::chrono::duration<double, std::milli>(now - s->t0).count(),
+                        (size_t)(&L - s->lanes.data()), (size_t)(&sl - L.slots.data()),
+                        sl.running.size(), rc);
+                fflush(stderr);
+            }
+            sl.running.clear();
+            sl.inflight = false;
+            --s->in_flight;
+            any = true;
         }
-        s->t_last_done = now;
-        if (s->log) {
-            fprintf(stderr, "[msbfs-server] %.3f ms retire lane %zu: %zu queries, status %d\n",
-                    std::chrono::duration<double, std::milli>(now - s->t0).count(),
-                    (size_t)(&L - s->lanes.data()), L.running.size(), rc);
-            fflush(stderr);
-        }
-        L.running.clear();
-        L.inflight = false;
-        any = true;
     }
     if (any) s->cv_done.notify_all();
     return any;
@@ -253,8 +281,7 @@ void worker_main(msbfs_server *s) {
     while (!s->stopping) {
         reap(s);
         schedule(s);
-        bool busy = false;
-        for (Lane &L : s->lanes) busy = busy || L.inflight;
+        const bool busy = s->in_flight != 0;
         if (busy) {
             lk.unlock();
             std::this_thread::sleep_for(std::chrono::microseconds(20));
@@ -265,9 +292,7 @@ void worker_main(msbfs_server *s) {
     }
     // drain
     for (;;) {
-        bool busy = false;
-        for (Lane &L : s->lanes) busy = busy || L.inflight;
-        if (!busy) break;
+        if (s->in_flight == 0) break;
         lk.unlock();
         std::this_thread::sleep_for(std::chrono::microseconds(50));
         lk.lock();
@@ -290,6 +315,11 @@ extern "C" int msbfs_server_create(msbfs_context *c, const msbfs_server_config *
     if (!s->cfg.output_pool_bytes) s->cfg.output_pool_bytes = 16ull << 30;
     if (!s->cfg.max_sources_per_launch) s->cfg.max_sources_per_launch = 2048;
     if (!s->cfg.max_queries_per_launch) s->cfg.max_queries_per_launch = 512;
+    if (!s->cfg.slots_per_lane) s->cfg.slots_per_lane = 4;
+    if (s->cfg.slots_per_lane > 16) {
+        msbfs_server_destroy(s);
+        return fail(MSBFS_ERR_INVALID_ARG, "slots_per_lane must be <= 16");
+    }
     if (s->cfg.dist_type != MSBFS_DIST_U32 && s->cfg.dist_type != MSBFS_DIST_U16) {
         msbfs_server_destroy(s);
         return fail(MSBFS_ERR_INVALID_ARG, "unknown dist type %d", s->cfg.dist_type);
@@ -317,23 +347,47 @@ extern "C" int msbfs_server_add_graph(msbfs_server *s, msbfs_graph *g, int metho
     if (s->running) return fail(MSBFS_ERR_INVALID_ARG, "server already started");
     Lane L;
     L.g = g;
-    int rc = msbfs_engine_create(s->ctx, method, &L.eng);
-    if (rc) return rc;
-    hipError_t e = hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&L.done, hipEventDisableTiming);
-    if (e != hipSuccess) {
-        msbfs_engine_destroy(L.eng);
-        return hip_fail(e, "lane stream");
+    auto drop = [](Slot &sl) {
+        if (sl.done) (void)hipEventDestroy(sl.done);
+        if (sl.stream) (void)hipStreamDestroy(sl.stream);
+        if (sl.eng) msbfs_engine_destroy(sl.eng);
+    };
+    for (uint32_t k = 0; k < s->cfg.slots_per_lane; ++k) {
+        Slot sl;
+        int rc = msbfs_engine_create(s->ctx, method, &sl.eng);
+        if (rc) {
+            for (Slot &o : L.slots) drop(o);
+            return rc;
+        }
+        // wide-family traversals as stepped launches: no co-resident grid,
+        // so they run beside the other slots' and lanes' traversals
+        msbfs_tuning t{};
+        msbfs_engine_get_tuning(sl.eng, &t);
+        t.stepped = 1;
+        msbfs_engine_set_tuning(sl.eng, &t);
+        hipError_t e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
+        if (e != hipSuccess) {
+            drop(sl);
+            for (Slot &o : L.slots) drop(o);
+            return hip_fail(e, "lane stream");
+        }
+        L.slots.push_back(sl);
     }
     g->refs.fetch_add(1);
-    s->lanes.push_back(L);
+    s->lanes.push_back(std::move(L));
     if (lane_id) *lane_id = (uint32_t)(s->lanes.size() - 1);
     return MSBFS_OK;
 }
 
 extern "C" int msbfs_server_set_tuning(msbfs_server *s, uint32_t lane, const msbfs_tuning *t) {
     if (!s || !t || lane >= s->lanes.size()) return fail(MSBFS_ERR_INVALID_ARG, "bad argument");
-    return msbfs_engine_set_tuning(s->lanes[lane].eng, t);
+    int rc = msbfs_tuning_validate(t);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (Slot &sl : s->lanes[lane].slots)
+        if ((rc = msbfs_engine_set_tuning(sl.eng, t))) return rc;
+    return MSBFS_OK;
 }
 
 extern "C" int msbfs_server_start(msbfs_server *s) {
@@ -432,7 +486,6 @@ extern "C" int msbfs_server_copy_result(msbfs_server *s, uint64_t id, void *host
     if (!s || !host) return fail(MSBFS_ERR_INVALID_ARG, "null argument");
     void *src = nullptr;
     uint64_t n = 0;
-    hipStream_t st = nullptr;
     {
         std::lock_guard<std::mutex> lk(s->mu);
         auto it = s->queries.find(id);
@@ -441,10 +494,8 @@ extern "C" int msbfs_server_copy_result(msbfs_server *s, uint64_t id, void *host
         const Query &q = it->second;
         n = (uint64_t)q.sources.size() * s->lanes[q.lane].g->V;
         src = (char *)s->pool_buf.p + q.off;
-        st = s->lanes[q.lane].stream;
     }
     if (len < n) return fail(MSBFS_ERR_OUT_TOO_SMALL, "result has %llu elements", (unsigned long long)n);
-    (void)st;
     HIP_TRY(hipMemcpy(host, src, n * elem_bytes(s), hipMemcpyDeviceToHost));
     return MSBFS_OK;
 }
@@ -479,6 +530,8 @@ extern "C" int msbfs_server_get_stats(msbfs_server *s, msbfs_server_stats *st) {
     if (s->any_submit && s->completed)
         st->span_s = std::chrono::duration<double>(s->t_last_done - s->t_first_submit).count();
     std::vector<double> v = s->latencies_ms;
+    st->overlapped_launches = s->overlapped;
+    st->max_in_flight = s->max_in_flight;
     if (!v.empty()) {
         std::sort(v.begin(), v.end());
         auto pct = [&](double p) {
@@ -499,6 +552,8 @@ extern "C" int msbfs_server_reset_stats(msbfs_server *s) {
     if (!s) return fail(MSBFS_ERR_INVALID_ARG, "null server");
     std::lock_guard<std::mutex> lk(s->mu);
     s->completed = s->failed = s->launches = s->sources_done = 0;
+    s->overlapped = 0;
+    s->max_in_flight = s->in_flight;
     s->latencies_ms.clear();
     s->any_submit = false;
     return MSBFS_OK;
@@ -534,10 +589,12 @@ extern "C" int msbfs_server_destroy(msbfs_server *s) {
     msbfs_server_stop(s);
     (void)hipSetDevice(s->ctx->device);
     for (Lane &L : s->lanes) {
-        if (L.stream) (void)hipStreamSynchronize(L.stream);
-        if (L.done) (void)hipEventDestroy(L.done);
-        if (L.stream) (void)hipStreamDestroy(L.stream);
-        if (L.eng) engine_release(L.eng);
+        for (Slot &sl : L.slots) {
+            if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+            if (sl.done) (void)hipEventDestroy(sl.done);
+            if (sl.stream) (void)hipStreamDestroy(sl.stream);
+            if (sl.eng) engine_release(sl.eng);
+        }
         if (L.g) graph_release(const_cast<msbfs_graph *>(L.g));
     }
     s->lanes.clear();

@@ -118,7 +118,8 @@ class LevelRecord(ctypes.Structure):
 class ServerConfig(ctypes.Structure):
     _fields_ = [("output_pool_bytes", ctypes.c_uint64), ("max_sources_per_launch", ctypes.c_uint32),
                 ("max_queries_per_launch", ctypes.c_uint32), ("dist_type", ctypes.c_int32),
-                ("pack", ctypes.c_int32)]
+                ("pack", ctypes.c_int32), ("slots_per_lane", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
 
 
 class QueryInfo(ctypes.Structure):
@@ -131,7 +132,8 @@ class QueryInfo(ctypes.Structure):
 class ServerStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in ("completed", "failed", "launches", "sources",
                                                "pending", "pool_bytes", "pool_used")] + \
-               [(n, ctypes.c_double) for n in ("span_s", "p50_ms", "p99_ms", "mean_ms", "max_ms")]
+               [(n, ctypes.c_double) for n in ("span_s", "p50_ms", "p99_ms", "mean_ms", "max_ms")] + \
+               [(n, ctypes.c_uint64) for n in ("overlapped_launches", "max_in_flight")]
 
 
 class RunStats(ctypes.Structure):
