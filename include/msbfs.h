@@ -256,6 +256,24 @@ int msbfs_run_ms_bfs_device(msbfs_engine *e, const msbfs_graph *g, const uint32_
                             uint64_t out_len, void *stream, int sync);
 int msbfs_engine_check(msbfs_engine *e);
 
+/* No-allocation variant of msbfs_run_ms_bfs_device.  No counterpart in the
+ * reference: it is for callers that cannot allocate device or pinned memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results as msbfs_run_ms_bfs_device with
+ * sync = 1 (host sources, device output, synchronous), for source sets the
+ * deep family runs; every workspace lives in a static per-device array of
+ * MSBFS_NOALLOC_SCRATCH_BYTES, set up by msbfs_context_create.  Rejects
+ * with a message: more than MSBFS_NOALLOC_MAX_SOURCES sources
+ * (MSBFS_ERR_OUT_TOO_SMALL), graphs and source counts that route to the
+ * wide family (MSBFS_ERR_UNSUPPORTED), and a frontier level larger than the
+ * array's queue space (MSBFS_ERR_OUT_TOO_SMALL, after the traversal).  Calls
+ * on one GPU share its array: calls that overlap on one GPU race. */
+#define MSBFS_NOALLOC_SCRATCH_BYTES (64u << 20)
+#define MSBFS_NOALLOC_MAX_SOURCES 65536u
+int msbfs_run_ms_bfs_device_noalloc(msbfs_engine *e, const msbfs_graph *g,
+                                    const uint32_t *sources, uint64_t num_sources, int dist_type,
+                                    void *out_device, uint64_t out_len, void *stream);
+
 /* GpuBfsEngine::run_ms_reachability (crates/gpu_bfs/src/engine.rs:420-479):
  * vertex-major bitmask words[v*ceil(S/32) + s/32], bit s%32
  * (ReachabilityBitmask, crates/core/src/graph.rs:446-646). */

@@ -47,6 +47,11 @@ extern "C" int msbfs_context_create(int device, msbfs_context **out) {
     int coop = 0;
     (void)hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device);
     c->coop = coop;
+    int rc = noalloc_setup(c);  // this device's copy of the no-alloc scratch array
+    if (rc) {
+        context_release(c);
+        return rc;
+    }
     *out = c;
     return MSBFS_OK;
 }
@@ -1315,5 +1320,169 @@ extern "C" int msbfs_engine_autotune(msbfs_engine *e, const msbfs_graph *g,
     if (best) *best = e->tune;
     if (best_method) *best_method = win_m;
     if (best_ms) *best_ms = win;
+    return MSBFS_OK;
+}
+
+// ------------------------------------------------------ no-allocation variant
+//
+// msbfs_run_ms_bfs_device_noalloc: the deep family with every workspace in a
+// static per-device array, for callers that cannot allocate (code under HIP
+// graph capture, tight serving loops, harnesses that forbid hipMalloc).
+// Layout of the array (each part 256-byte aligned): the task counter and
+// flags, the graph record, the task table, the sources, the one-wave
+// kernel's per-workgroup sink words, and the frontier queues' spill space
+// (the rest).  Calls on one device share its array and leave their contents
+// in it; the counter and flags are cleared by a fill kernel at every call, the
+// rest is written before it is read.  Concurrent calls on one device race.
+
+// The array's name is a host-side stand-in: kernels get the per-device
+// address that hipGetSymbolAddress returns (msbfs_context::scratch).
+static __device__ __align__(256) unsigned char scratch[MSBFS_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+}  // namespace
+
+int msbfs::noalloc_setup(msbfs_context *c) {
+    // with c->device current (msbfs_context_create)
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return hip_fail(e == hipSuccess ? hipErrorInvalidSymbol : e,
+                                               "no-alloc scratch address");
+    c->scratch = p;
+    return MSBFS_OK;
+}
+
+extern "C" int msbfs_run_ms_bfs_device_noalloc(msbfs_engine *e, const msbfs_graph *g,
+                                               const uint32_t *sources, uint64_t S, int dtype,
+                                               void *out_device, uint64_t out_len, void *stream) {
+    clear_error();
+    if (!e || !g) return fail(MSBFS_ERR_INVALID_ARG, "null engine or graph");
+    if (dtype != MSBFS_DIST_U32 && dtype != MSBFS_DIST_U16 && dtype != MSBFS_DIST_U8)
+        return fail(MSBFS_ERR_INVALID_ARG, "unknown dist type %d", dtype);
+    if (S == 0) return MSBFS_OK;
+    if (!sources) return fail(MSBFS_ERR_INVALID_ARG, "null sources");
+    std::lock_guard<std::mutex> lk(e->mu);
+    int rc = check_sources(g, sources, S);
+    if (rc) return rc;
+    if (out_len < S * g->V)
+        return fail(MSBFS_ERR_OUT_TOO_SMALL, "output holds %llu elements, %llu needed",
+                    (unsigned long long)out_len, (unsigned long long)(S * g->V));
+    if (!out_device) return fail(MSBFS_ERR_INVALID_ARG, "null output");
+    msbfs_context *c = e->ctx;
+    if (!c->scratch) return fail(MSBFS_ERR_HIP, "no-alloc scratch was not set up");
+    if (S > MSBFS_NOALLOC_MAX_SOURCES)
+        return fail(MSBFS_ERR_OUT_TOO_SMALL,
+                    "no-alloc call: %llu sources, at most %u fit its scratch",
+                    (unsigned long long)S, (unsigned)MSBFS_NOALLOC_MAX_SOURCES);
+    const Plan plan = plan_for(e, g, S);
+    if (plan.family != MSBFS_FAMILY_DEEP)
+        return fail(MSBFS_ERR_UNSUPPORTED,
+                    "no-alloc call: this graph and source count route to the wide family, whose "
+                    "workspace (source words per vertex) does not fit a fixed scratch; use "
+                    "msbfs_run_ms_bfs_device");
+    HIP_TRY(hipSetDevice(c->device));
+    if ((rc = finish_run(e))) return rc;
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+
+    const int B = plan.bits;
+    const uint64_t V = g->V, ntasks = (S + B - 1) / B;
+    const int threads = plan.threads;
+    {
+        const hipError_t pe = deep_prepare(B, plan.ell, threads, plan.lds);
+        if (pe != hipSuccess)
+            return fail(MSBFS_ERR_HIP, "deep kernel set-up: %s", hipGetErrorString(pe));
+    }
+    int occ = 0;
+    HIP_TRY(deep_max_blocks_per_cu(B, plan.ell, threads, plan.lds, &occ));
+    if (occ < 1) return fail(MSBFS_ERR_HIP, "deep kernel cannot be resident");
+    const int blocks = (int)std::min<uint64_t>(ntasks, (uint64_t)c->cus * occ);
+    const size_t esz = B <= 8 ? 4 : 8;
+    // scratch layout
+    char *const base = (char *)c->scratch;
+    const size_t off_graph = 256;
+    const size_t off_tasks = align256(off_graph + sizeof(DeepGraph));
+    const size_t off_src = align256(off_tasks + ntasks * sizeof(DeepTask));
+    const size_t off_sink = align256(off_src + S * 4);
+    const size_t off_queues = align256(off_sink + (size_t)blocks * 64 * 4);
+    if (off_queues + (size_t)blocks * 2 * esz > MSBFS_NOALLOC_SCRATCH_BYTES)
+        return fail(MSBFS_ERR_OUT_TOO_SMALL, "no-alloc call: the task table does not fit its scratch");
+    // spill entries per queue: the worst case (every vertex in one level)
+    // when it fits, else what the scratch holds; a larger frontier is an error
+    const uint64_t worst = (uint64_t)B * V > plan.qcap ? (uint64_t)B * V - plan.qcap : 1;
+    const uint64_t room = (MSBFS_NOALLOC_SCRATCH_BYTES - off_queues) / ((size_t)blocks * 2 * esz);
+    const uint64_t spill = std::max<uint64_t>(1, std::min(worst, room));
+
+    // task table and graph record through pageable memory (synchronous copies)
+    std::vector<DeepTask> ht(ntasks);
+    for (uint64_t t = 0; t < ntasks; ++t) {
+        ht[t].graph = 0;
+        ht[t].src_off = t * B;
+        ht[t].nsrc = (uint32_t)std::min<uint64_t>(B, S - t * B);
+        ht[t].out_off = t * B * V;
+        ht[t].reach_bit = t * B;
+    }
+    DeepGraph hg{};
+    hg.ro = g->ro;
+    hg.ci = g->ci;
+    hg.ell = g->ell;
+    hg.V = (uint32_t)V;
+    hg.ell_width = (uint32_t)plan.ell;
+    LaunchOrder order(c, st, false);
+    if ((rc = order.begin())) return rc;
+    HIP_TRY(hipMemcpyAsync(base + off_tasks, ht.data(), ntasks * sizeof(DeepTask),
+                           hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(base + off_graph, &hg, sizeof(hg), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(base + off_src, sources, S * 4, hipMemcpyHostToDevice, st));
+    uint32_t *const work = (uint32_t *)base, *const flags = work + 8;
+    HIP_TRY(launch_zero_u32(work, 8 + FlagCount, st));
+
+    DeepParams p{};
+    p.tasks = (const DeepTask *)(base + off_tasks);
+    p.graphs = (const DeepGraph *)(base + off_graph);
+    p.sources = (const uint32_t *)(base + off_src);
+    p.out = out_device;
+    p.queues = base + off_queues;
+    p.cap = spill + plan.qcap;
+    p.lds_qcap = plan.qcap;
+    p.lds_vis_words = plan.vis_words;
+    p.work = work;
+    p.flags = flags;
+    p.ntasks = (uint32_t)ntasks;
+    p.out_kind = dtype == MSBFS_DIST_U16 ? 1u : dtype == MSBFS_DIST_U8 ? 2u : 0u;
+    p.log_bw = (dtype == MSBFS_DIST_U32) ? plan.log_bw : 0;
+    p.lds_bin_off = plan.bin_off;
+    p.lds_sink_off = plan.sink_off;
+    p.log_sink = (uint32_t *)(base + off_sink);
+    HIP_TRY(hipEventRecord(e->ev[1], st));
+    HIP_TRY(launch_deep(B, plan.ell, p, blocks, threads, plan.lds, st));
+    HIP_TRY(hipEventRecord(e->ev[2], st));
+    if ((rc = order.end())) return rc;
+    HIP_TRY(hipStreamSynchronize(st));
+    uint32_t f[FlagCount];
+    HIP_TRY(hipMemcpy(f, flags, sizeof(f), hipMemcpyDeviceToHost));
+    e->last = msbfs_run_stats{};
+    e->last.method = plan.method;
+    e->last.family = plan.family;
+    e->last.levels = f[FlagLevels];
+    e->last.instances = (uint32_t)ntasks;
+    e->last.bits_per_instance = (uint32_t)B;
+    e->last.grid_blocks = (uint32_t)blocks;
+    e->last.block_threads = (uint32_t)threads;
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, e->ev[1], e->ev[2]) == hipSuccess) e->last.kernel_ms = ms;
+    if (f[FlagQueueOverflow])
+        return fail(MSBFS_ERR_OUT_TOO_SMALL,
+                    "no-alloc call: a frontier level outgrew the scratch's queue space "
+                    "(%llu entries per task); use msbfs_run_ms_bfs_device",
+                    (unsigned long long)(spill + plan.qcap));
+    if (f[FlagU16Overflow] && dtype == MSBFS_DIST_U16)
+        return fail(MSBFS_ERR_DIAMETER_U16,
+                    "a distance >= 65535 cannot be stored as u16 (0xFFFF is the unreachable "
+                    "sentinel); use u32 output");
+    if (f[FlagU16Overflow] && dtype == MSBFS_DIST_U8)
+        return fail(MSBFS_ERR_DIAMETER_U8,
+                    "a distance >= 255 cannot be stored as u8 (0xFF is the unreachable "
+                    "sentinel); use u16 or u32 output");
     return MSBFS_OK;
 }

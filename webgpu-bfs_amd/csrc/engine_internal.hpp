@@ -110,6 +110,8 @@ struct msbfs_context {
     int cus = 0;
     size_t lds_max = 0;
     int coop = 0;
+    // device address of this GPU's copy of the no-alloc scratch array
+    void *scratch = nullptr;
 };
 
 struct msbfs_graph {
@@ -196,6 +198,9 @@ struct Plan {
 };
 
 void context_release(msbfs_context *c);
+// Looks up this device's address of the no-alloc scratch array (with the
+// context's device current) and keeps it in the context.
+int noalloc_setup(msbfs_context *c);
 // Brackets the enqueue of one traversal on stream `st` (coop = it contains a
 // cooperative launch).  The context's ordering lock is held from begin() to
 // the guard's end of life, so no other traversal is enqueued in between.

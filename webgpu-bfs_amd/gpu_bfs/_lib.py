@@ -181,6 +181,7 @@ SIGNATURES = {
     "msbfs_run_ms_bfs": (I32, [P, P, P, U64, I32, P, U64]),
     "msbfs_run_ms_bfs_device": (I32, [P, P, P, U64, I32, P, U64, P, I32]),
     "msbfs_engine_check": (I32, [P]),
+    "msbfs_run_ms_bfs_device_noalloc": (I32, [P, P, P, U64, I32, P, U64, P]),
     "msbfs_run_ms_reachability": (I32, [P, P, P, U64, P, U64]),
     "msbfs_batch_create": (I32, [P, U32, P, P, P, I32, PP]),
     "msbfs_batch_destroy": (I32, [P]),

@@ -307,6 +307,22 @@ class GpuBfsEngine:
             len(src), dt, ctypes.c_void_p(out_ptr), out_len,
             ctypes.c_void_p(stream) if stream else None, int(sync)))
 
+    def run_ms_bfs_device_noalloc(self, graph, sources, out_ptr: int, out_len: int,
+                                  dist_dtype=np.uint32, stream: int | None = None) -> None:
+        """msbfs_run_ms_bfs_device_noalloc: as run_ms_bfs_device(sync=True)
+        with every workspace in the library's static per-device scratch
+        array (no device or pinned allocation during the call)."""
+        src = _src_array(sources)
+        dt = L.dist_code(dist_dtype)
+        if len(src) and graph.num_vertices == 0:
+            raise L.SourceOutOfBoundsError(
+                L.ERR_SOURCE_OOB, f"Source vertex {int(src[0])} at index 0 exceeds graph vertex count 0")
+        dg = self._device_graph(graph)
+        L.check(L.lib().msbfs_run_ms_bfs_device_noalloc(
+            self._h, dg.handle, src.ctypes.data_as(ctypes.c_void_p) if len(src) else None,
+            len(src), dt, ctypes.c_void_p(out_ptr), out_len,
+            ctypes.c_void_p(stream) if stream else None))
+
     def check(self) -> None:
         L.check(L.lib().msbfs_engine_check(self._h))
 
