@@ -128,10 +128,13 @@ typedef struct msbfs_tuning {
     uint32_t direction_rule; /* 0: measured frontier degree sum and size (default);
                                 1: the reference's estimate n_f = ceil(bits/S), m_f = n_f*|E|/|V| */
     uint32_t trace_levels;   /* wide family: record up to this many levels (0 = off) */
-    uint32_t stepped;        /* wide family launch: 0 one cooperative kernel for all levels (grid
-                                barriers, the device to itself); 1 one launch per phase of each
-                                level, no co-residency needed, so other traversals may run beside
-                                it (used by the serving loop) */
+    uint32_t stepped;        /* wide family launch: 1 stepped, one launch per phase of each level
+                                (no co-resident grid needed, so other traversals may run beside
+                                it; the serving loop's mode); 2 one cooperative kernel for all
+                                levels (grid barriers, the device to itself); 0 auto: cooperative
+                                for graphs known to need more than 64 levels, or, before any run,
+                                graphs the reference's router calls road-like (max degree <= 8 and
+                                |E| <= 4|V|), stepped otherwise */
 } msbfs_tuning;
 
 /* One BFS level of the last wide-family run (recorded when trace_levels > 0). */
@@ -326,7 +329,9 @@ typedef struct msbfs_server_config {
     int32_t dist_type;                /* MSBFS_DIST_U32 or MSBFS_DIST_U16 */
     int32_t pack;                     /* 1: many queries per traversal; 0: one per traversal */
     uint32_t slots_per_lane;          /* traversals of one lane in flight at once; 0 = 4 */
-    uint32_t reserved;
+    uint32_t admit_wait_us;           /* beside a running traversal of the same lane, a launch
+                                         waits for half of max_sources_per_launch or until its
+                                         oldest query has waited this long; 0 = 2000 */
 } msbfs_server_config;
 
 typedef struct msbfs_query_info {

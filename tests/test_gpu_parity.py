@@ -427,7 +427,9 @@ def test_random_graphs_every_path(ctx):
         "er6000": gb.generate_erdos_renyi_np(6000, 3.0 / 5999, seed=13),
     }
     cases = [(M.Auto, {}), (M.PushPullAdaptive, {}), (M.BitParallel64Pull, {}),
-             (M.BitParallel64, {}), (M.WorkgroupPersistentBfs, {}),
+             (M.BitParallel64, {}), (M.PushPullAdaptive, {"stepped": 2}),
+             (M.BitParallel64Pull, {"stepped": 2}), (M.BitParallel64, {"stepped": 2}),
+             (M.WorkgroupPersistentBfs, {}),
              (M.WorkgroupPersistentBfs, {"deep_threads": 64}),
              (M.PushPullAdaptive, {"tile_words": 4}), (M.PushPullAdaptive, {"tile_words": 8})]
     for gname, g in graphs.items():

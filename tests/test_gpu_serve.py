@@ -132,7 +132,7 @@ def test_cooperative_lane_beside_deep_lanes(ctx, graphs):
     s.add_graph(road)                                     # deep
     s.add_graph(rmat, method=gb.GpuBfsMethod.PushPullAdaptive)  # wide, stepped
     s.add_graph(rmat)                                     # wide (Auto), cooperative below
-    s.set_tuning(2, stepped=0)
+    s.set_tuning(2, stepped=2)
     s.start()
     rng = np.random.default_rng(7)
     qs = []

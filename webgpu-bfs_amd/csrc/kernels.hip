@@ -2227,11 +2227,15 @@ __global__ void wide_ctl_kernel(WideParams p, uint32_t probe) {
         prev_bits = ld_dev_u64(&c->active_bits);
         prev_deg = ld_dev_u64(&c->deg_sum);
     }
+    // per-level trace, as wide_bfs_kernel records it: the new bits of the
+    // level just finished, and the start of the next (or the end)
+    if (p.trace && L > 0 && L - 1 < p.trace_cap) p.trace[4 * (L - 1) + 3] = prev_bits;
     if (n_front == 0 || L > V) {
         ct->done = 1;
         p.flags[FlagLevels] = L;
         p.flags[FlagPushLevels] = ct->push_levels;
         p.flags[FlagPullLevels] = ct->pull_levels;
+        if (p.trace && L < p.trace_cap) p.trace[4 * L] = wall_clock64();
         __threadfence();
         return;
     }
@@ -2267,6 +2271,10 @@ __global__ void wide_ctl_kernel(WideParams p, uint32_t probe) {
     ct->dir = dir;
     ct->n_front = n_front;
     ct->next = L + 1;
+    if (p.trace && L < p.trace_cap) {
+        p.trace[4 * L] = wall_clock64();
+        p.trace[4 * L + 1] = ((u64)dir << 32) | n_front;
+    }
     __threadfence();
 }
 
@@ -2313,6 +2321,8 @@ __global__ void __launch_bounds__(kWideThreads) __attribute__((amdgpu_waves_per_
     x.act_cur = p.act + (size_t)(L % 3) * W;
     x.direct = p.out != nullptr && nd >= D;
     u64 my_bits = 0, my_deg = 0;
+    if (PHASE == 1 && p.trace && L < p.trace_cap && blockIdx.x == 0 && threadIdx.x == 0)
+        p.trace[4 * L + 2] = wall_clock64();
     if (PHASE == 0) {
         if (dir == DirPush)
             wide_push_expand<TW>(p, x, cur, qcur, n_front, fb_cur, c, tick);

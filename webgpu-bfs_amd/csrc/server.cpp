@@ -146,12 +146,31 @@ size_t elem_bytes(const msbfs_server *s) { return s->cfg.dist_type == MSBFS_DIST
 // block of the pool; the block is carved into the queries' slices in order,
 // so query i's rows follow query i-1's and one packed traversal fills them
 // all.
+// Beside a running traversal of the same lane a new launch waits for half
+// a full pack, or until its oldest query has waited admit_wait_us: at low
+// load queries start at once, under saturation packs stay large (packed
+// sources share source words; concurrent small packs of one graph would
+// each pay for a whole traversal).
+bool worth_launching(const msbfs_server *s, const Lane &L, Clock::time_point now) {
+    bool busy = false;
+    for (const Slot &sl : L.slots) busy = busy || sl.inflight;
+    if (!busy || !s->cfg.pack) return true;
+    uint64_t nsrc = 0;
+    for (uint64_t id : L.pending) {
+        nsrc += s->queries.at(id).sources.size();
+        if (2 * nsrc >= s->cfg.max_sources_per_launch) return true;
+    }
+    const Query &head = s->queries.at(L.pending.front());
+    return now - head.t_submit >= std::chrono::microseconds(s->cfg.admit_wait_us);
+}
+
 void schedule(msbfs_server *s) {
     for (uint32_t li = 0; li < s->lanes.size(); ++li) {
         Lane &L = s->lanes[li];
         for (Slot &sl : L.slots) {
             if (L.pending.empty()) break;
             if (sl.inflight) continue;
+            if (!worth_launching(s, L, Clock::now())) break;
             std::vector<uint64_t> take;
             uint64_t nsrc = 0, total = 0;
             for (uint64_t id : L.pending) {
@@ -316,6 +335,7 @@ extern "C" int msbfs_server_create(msbfs_context *c, const msbfs_server_config *
     if (!s->cfg.max_sources_per_launch) s->cfg.max_sources_per_launch = 2048;
     if (!s->cfg.max_queries_per_launch) s->cfg.max_queries_per_launch = 512;
     if (!s->cfg.slots_per_lane) s->cfg.slots_per_lane = 4;
+    if (!s->cfg.admit_wait_us) s->cfg.admit_wait_us = 2000;
     if (s->cfg.slots_per_lane > 16) {
         msbfs_server_destroy(s);
         return fail(MSBFS_ERR_INVALID_ARG, "slots_per_lane must be <= 16");
@@ -394,6 +414,24 @@ extern "C" int msbfs_server_start(msbfs_server *s) {
     if (!s) return fail(MSBFS_ERR_INVALID_ARG, "null server");
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->running) return MSBFS_OK;
+    // Every slot runs one traversal of the widest launch it can get (the
+    // source cap, or what the output pool holds) before the first query, so
+    // its workspace is sized once: a workspace that grows while serving
+    // frees (a device-wide synchronisation) and allocates gigabytes under
+    // the scheduler's lock, stalling every lane.
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    for (Lane &L : s->lanes) {
+        if (L.g->V == 0) continue;
+        const uint64_t fit = s->pool.size() / (L.g->V * elem_bytes(s));
+        const uint64_t k = std::min<uint64_t>(s->cfg.max_sources_per_launch, fit);
+        if (k == 0) continue;
+        std::vector<uint32_t> warm(k, 0u);
+        for (Slot &sl : L.slots) {
+            int rc = msbfs_run_ms_bfs_device(sl.eng, L.g, warm.data(), k, s->cfg.dist_type,
+                                             s->pool_buf.p, k * L.g->V, sl.stream, 1);
+            if (rc) return rc;
+        }
+    }
     s->running = true;
     s->stopping = false;
     s->worker = std::thread(worker_main, s);

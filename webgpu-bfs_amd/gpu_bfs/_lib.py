@@ -119,7 +119,7 @@ class ServerConfig(ctypes.Structure):
     _fields_ = [("output_pool_bytes", ctypes.c_uint64), ("max_sources_per_launch", ctypes.c_uint32),
                 ("max_queries_per_launch", ctypes.c_uint32), ("dist_type", ctypes.c_int32),
                 ("pack", ctypes.c_int32), ("slots_per_lane", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32)]
+                ("admit_wait_us", ctypes.c_uint32)]
 
 
 class QueryInfo(ctypes.Structure):

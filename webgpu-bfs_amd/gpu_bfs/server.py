@@ -17,7 +17,8 @@ from .engine import DeviceGraph, GpuBfsMethod, GpuContext
 class Server:
     def __init__(self, ctx: GpuContext, pool_bytes: int = 16 << 30, dist_dtype=np.uint16,
                  pack: bool = True, max_sources_per_launch: int = 2048,
-                 max_queries_per_launch: int = 512, slots_per_lane: int = 4):
+                 max_queries_per_launch: int = 512, slots_per_lane: int = 4,
+                 admit_wait_us: int = 0):
         cfg = L.ServerConfig()
         cfg.output_pool_bytes = pool_bytes
         cfg.max_sources_per_launch = max_sources_per_launch
@@ -27,6 +28,7 @@ class Server:
         cfg.dist_type = self.dtype
         cfg.pack = int(pack)
         cfg.slots_per_lane = slots_per_lane
+        cfg.admit_wait_us = admit_wait_us
         h = ctypes.c_void_p()
         L.check(L.lib().msbfs_server_create(ctx.handle, ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h
