@@ -387,6 +387,16 @@ def measure(args, config, steps, warmup, with_cpu, with_host, with_trace=False):
 
 # ------------------------------------------------------------------ batch
 
+def sweep_summary(sweep):
+    """Autotune candidates as "knob=value,... : ms" strings (knobs left at
+    their defaults omitted)."""
+    res = []
+    for c in sweep:
+        knobs = ",".join(f"{k}={v}" for k, v in c.items() if k not in ("ms", "status") and v)
+        res.append(f"{knobs}: {c['ms']:.4f}" if c["ms"] is not None else f"{knobs}: error {c['status']}")
+    return res
+
+
 def bench_batch(args, steps, warmup):
     """512 G(n,p) graphs of uneven size, each with its own source set, in one
     persistent hipGraph-replayed launch, against one engine call per graph."""
@@ -460,7 +470,10 @@ def bench_batch(args, steps, warmup):
            "plain_gteps": round(te / (plain_ms / 1e3) / 1e9, 3),
            "speedup_vs_plain": round(plain_ms / ms_per_step, 2),
            "identical_to_plain": same,
-           "autotune": {"chosen": tuned, "ms": round(tuned_ms, 4)},
+           "autotune": {"chosen": {k: v for k, v in tuned.items() if v},
+                        "ms": round(tuned_ms, 4),
+                        "sweep": sweep_summary(b.sweep())},
+           "wide_graphs": int(sum(1 for f in b.families() if f == int(gb.Family.WIDE))),
            "config": {"workload": "batch512", "graphs": len(graphs),
                       "num_vertices": int(sum(g.num_vertices for g in graphs)),
                       "num_edges": int(sum(g.num_edges for g in graphs)),

@@ -135,6 +135,12 @@ typedef struct msbfs_tuning {
                                 for graphs known to need more than 64 levels, or, before any run,
                                 graphs the reference's router calls road-like (max degree <= 8 and
                                 |E| <= 4|V|), stepped otherwise */
+    uint32_t deep_rows;      /* deep family workgroup kernel, one source per task, u32/u16 output:
+                                1 staged, the row kept in LDS as a 4-bit code per vertex and written
+                                whole when the task ends (distances >= 14 stored as found); 2 every
+                                distance stored as found; 0 auto: staged when the graph's deep
+                                traversals are known to need <= 32 levels or none has run yet, and
+                                the codes fit in LDS */
 } msbfs_tuning;
 
 /* One BFS level of the last wide-family run (recorded when trace_levels > 0). */
@@ -290,7 +296,10 @@ int msbfs_run_ms_reachability(msbfs_engine *e, const msbfs_graph *g, const uint3
  * are concatenated graph by graph (source_counts[g] entries for graph g);
  * graph g's distances occupy the output elements
  * [offsets[g], offsets[g] + source_counts[g]*|V_g|) in the usual
- * dist[s*|V_g| + v] layout. */
+ * dist[s*|V_g| + v] layout.  A graph whose visited bits (one per vertex) do
+ * not fit one workgroup's LDS is traversed by the wide family instead, one
+ * engine run per such graph on the same stream after the persistent launch
+ * (outside a captured hipGraph: a replay launches them eagerly). */
 int msbfs_batch_create(msbfs_engine *e, uint32_t num_graphs, msbfs_graph *const *graphs,
                        const uint64_t *source_counts, const uint32_t *sources, int dist_type,
                        msbfs_batch **out);
@@ -305,10 +314,25 @@ int msbfs_batch_replay(msbfs_batch *b, void *stream, int sync);
 int msbfs_batch_check(msbfs_batch *b);
 int msbfs_batch_stats(msbfs_batch *b, msbfs_run_stats *st);
 int msbfs_batch_set_tuning(msbfs_batch *b, const msbfs_tuning *t);
-/* Autotuner: times every feasible (deep_bits, deep_threads) pair on the
- * batch and keeps the fastest; returns it and its time. */
+/* Family of each graph of the batch (msbfs_family), num_graphs entries. */
+int msbfs_batch_families(const msbfs_batch *b, int32_t *families);
+/* Autotuner: times on the batch itself every feasible deep_bits x
+ * deep_threads (64..1024) pair, staged and direct rows for one source per
+ * task, and, when some graph takes the wide family, its wave tile
+ * (tile_words), words per lane, push/pull thresholds (alpha, beta) and
+ * launch form, one knob at a time from the best so far; keeps the fastest
+ * and returns it and its time. */
 int msbfs_batch_autotune(msbfs_batch *b, void *out_device, uint64_t out_len, msbfs_tuning *best,
                          float *best_ms);
+/* One candidate of the last autotune. */
+typedef struct msbfs_sweep_entry {
+    msbfs_tuning tuning;
+    float ms;        /* mean of two timed runs after one warm-up; < 0 when it did not run */
+    int32_t status;  /* MSBFS_OK, or the error the candidate's plan or run returned */
+} msbfs_sweep_entry;
+/* Copies up to `cap` entries of the last autotune's sweep (in the order they
+ * ran) and sets *n to the number there are. */
+int msbfs_batch_sweep(const msbfs_batch *b, msbfs_sweep_entry *entries, uint32_t cap, uint32_t *n);
 
 /* ---- serving ----
  * Resident graphs ("lanes") and a scheduler thread that packs queries of

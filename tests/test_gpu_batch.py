@@ -93,12 +93,84 @@ def test_batch_set_sources_then_replay(setup):
 
 
 def test_batch_autotune_keeps_results(setup):
+    """The autotuner's sweep covers task widths, workgroup sizes from one
+    wave to 1024 threads and staged / direct rows; the winner and every
+    candidate that ran give the oracle's distances."""
+    import torch
+
     graphs, sources, eng, b, out = setup
     best, ms = b.autotune(out.data_ptr(), b.total)
     say("autotune", best, ms)
     assert best["deep_bits"] in (1, 2, 4, 8, 16, 32) and ms > 0
     b.run(out.data_ptr(), b.total)
-    check(b, out, expect_all(graphs, sources))
+    expect = expect_all(graphs, sources)
+    check(b, out, expect)
+    sweep = b.sweep()
+    ran = [c for c in sweep if c["status"] == 0]
+    assert ran and all(c["ms"] > 0 for c in ran)
+    assert {c["deep_threads"] for c in ran} >= {64, 128, 256, 512, 1024}
+    assert {c["deep_rows"] for c in ran if c["deep_bits"] == 1} == {1, 2}
+    assert min(c["ms"] for c in ran) == pytest.approx(ms, rel=1e-5)
+    knobs = ("deep_bits", "deep_threads", "deep_rows")
+    for c in ran:
+        say("candidate", {k: c[k] for k in knobs})
+        b.set_tuning(**{k: c[k] for k in knobs})
+        out.fill_(-3)
+        torch.cuda.synchronize()  # the batch runs on its own stream
+        b.run(out.data_ptr(), b.total)
+        check(b, out, expect)
+    b.set_tuning(**{k: best[k] for k in knobs})
+
+
+def test_batch_wide_fallback(ctx):
+    """A graph whose visited bits exceed one workgroup's LDS (1.4M vertices)
+    is traversed by the wide family inside the batch: eager runs, hipGraph
+    replays (the wide run launched eagerly after the captured launch), a
+    source update and the autotuner's wide knobs all give the oracle's
+    distances."""
+    import torch
+
+    rng = np.random.default_rng(12)
+    big = gb.generate_erdos_renyi_np(1_400_000, 3.0 / 1_399_999, seed=9)
+    graphs = [gb.generate_erdos_renyi_np(2000, 6 / 1999, seed=3), big, gb.generate_grid2d(40, 30),
+              gb.generate_erdos_renyi_np(500, 4 / 499, seed=4),
+              gb.generate_erdos_renyi_np(1_300_000, 2.0 / 1_299_999, seed=10)]
+    sources = [rng.integers(0, 2000, 50).astype(np.uint32),
+               rng.integers(0, big.num_vertices, 70).astype(np.uint32),
+               np.array([0, 1199, 5], np.uint32), np.zeros(0, np.uint32), np.zeros(0, np.uint32)]
+    eng = gb.GpuBfsEngine(ctx, gb.GpuBfsMethod.Auto)
+    b = gb.Batch(eng, graphs, sources)
+    D, W = int(gb.Family.DEEP), int(gb.Family.WIDE)
+    assert b.families() == [D, W, D, D, W]
+    out = torch.empty(b.total, dtype=torch.int32, device="cuda")
+    expect = expect_all(graphs, sources)
+    say("wide fallback eager", b.total)
+    out.fill_(-9)
+    torch.cuda.synchronize()  # the batch runs on its own stream
+    b.run(out.data_ptr(), b.total)
+    check(b, out, expect)
+    b.capture(out.data_ptr(), b.total)
+    for r in range(2):
+        out.fill_(-7)
+        torch.cuda.synchronize()
+        say("wide fallback replay", r)
+        b.replay()
+        check(b, out, expect)
+    new = [rng.integers(0, g.num_vertices, size=len(s)).astype(np.uint32)
+           for g, s in zip(graphs, sources)]
+    b.set_sources(new)
+    b.replay()
+    check(b, out, expect_all(graphs, new))
+    b.set_sources(sources)
+    best, ms = b.autotune(out.data_ptr(), b.total)
+    say("wide fallback autotune", best, ms)
+    sweep = b.sweep()
+    for knob in ("tile_words", "words_per_lane", "alpha", "beta", "stepped"):
+        assert any(c[knob] != 0 for c in sweep), knob
+    out.fill_(-5)
+    torch.cuda.synchronize()
+    b.run(out.data_ptr(), b.total)
+    check(b, out, expect)
 
 
 def test_batch_u16_and_errors(ctx):

@@ -281,6 +281,7 @@ def test_device_output_on_torch_stream(engines, road):
     s = torch.cuda.Stream()
     for m in (M.Auto, M.PushPullAdaptive):
         out.fill_(7)
+        torch.cuda.synchronize()  # the run goes on stream s
         engines[m].run_ms_bfs_device(road, src, out.data_ptr(), out.numel(),
                                      stream=s.cuda_stream, sync=False)
         engines[m].check()
@@ -564,6 +565,47 @@ def test_logged_u32_rows(ctx, road):
             st = e.last_stats()
             assert st["block_threads"] == 64 and st["bits_per_instance"] == 1
             assert st["instances"] >= 4 * ctx.info()["compute_units"]
+
+
+def test_staged_rows_match_oracle(ctx):
+    """One source per task on the workgroup kernel keeps each row in LDS as
+    4-bit level codes (distance + 1 below 14, 15 for distances stored as
+    found) and writes it whole when the task ends.  Every row against the
+    oracle, staged and direct, u32 and u16, output buffers starting with
+    garbage, for: a sparse G(n,p) graph with unreachable vertices, a path
+    whose distances run far past the codes (escapes from level 14 on), a grid
+    with a neighbour table, |V| of 1, 7, 8 and 9 (partial code words), and
+    duplicated sources."""
+    import torch
+
+    rng = np.random.default_rng(8)
+    path = gb.CsrGraph.from_edge_list(45, [(i, i + 1) for i in range(44)])
+    cases = [(gb.generate_erdos_renyi_np(3000, 2.2 / 2999, seed=5), None),
+             (path, np.array([0, 44, 22, 13, 14, 15, 0], np.uint32)),
+             (gb.generate_grid2d(30, 31), None),
+             (gb.generate_rmat(12, 8, directed=True), None)]
+    for nv in (1, 7, 8, 9):
+        cases.append((gb.CsrGraph.from_edge_list(nv, [(i, (i * 3 + 1) % nv) for i in range(nv - 1)]),
+                      None))
+    for g, src in cases:
+        if src is None:
+            src = rng.integers(0, g.num_vertices, 300).astype(np.uint32)
+        expect = oracle(g, src)
+        n = len(src) * g.num_vertices
+        for rows in (1, 2):
+            for threads in (256, 1024):
+                e = gb.GpuBfsEngine(ctx, M.WorkgroupPersistentBfs)
+                e.set_tuning(deep_bits=1, deep_threads=threads, deep_rows=rows)
+                say("staged rows=%d threads=%d V=%d S=%d" % (rows, threads, g.num_vertices, len(src)))
+                got = _device_u32(e, g, src, 0x33333333)
+                assert np.array_equal(got, expect), (g.num_vertices, rows, threads,
+                                                     int(np.count_nonzero(got != expect)))
+                st = e.last_stats()
+                assert st["block_threads"] == threads and st["bits_per_instance"] == 1
+                out16 = torch.full((max(n, 1),), 0x1234, dtype=torch.int16, device="cuda")
+                e.run_ms_bfs_device(g, src, out16.data_ptr(), n, dist_dtype=np.uint16)
+                got16 = out16[:n].cpu().numpy().view(np.uint16)
+                assert np.array_equal(got16, to_u16_expect(expect)), (g.num_vertices, rows, "u16")
 
 
 def test_every_method_enum_matches_oracle(ctx):

@@ -5,9 +5,12 @@
 // all graphs go into one table, largest graphs first, and the persistent
 // workgroups pull them from a global counter.  One workgroup runs one task
 // at a time with the task's visited bits in LDS, so graphs of any size mix
-// freely inside the launch and no device-wide barrier is involved.  A run
-// can be recorded into a hipGraph once and replayed; an autotuner times the
-// candidate task widths and workgroup sizes on the batch itself.
+// freely inside the launch and no device-wide barrier is involved.  A graph
+// whose visited bits do not fit one workgroup's LDS goes to the wide family
+// instead: one engine (workspace) per such graph, run on the same stream
+// after the persistent launch.  A run can be recorded into a hipGraph once
+// and replayed (the wide runs stay eager: their level count is only known on
+// the device); an autotuner times the candidate plans on the batch itself.
 //
 // The reference has no batched path (one engine call per graph and source
 // set, crates/gpu_bfs/src/engine.rs:219-297).
@@ -40,9 +43,21 @@ struct msbfs_batch {
     msbfs_run_stats last{};
     bool pending = false;
     bool planned = false;
+    std::vector<char> is_wide;               // per graph: traversed by the wide family
+    std::vector<uint32_t> wide;              // those graphs
+    std::vector<msbfs_engine *> wide_eng;    // one engine per wide graph
+    std::vector<msbfs_sweep_entry> sweep;    // the last autotune's candidates
 };
 
 namespace {
+
+// A graph runs in the persistent launch when one source per task fits:
+// its visited bits, two minimal queues and the kernel's static LDS.
+bool deep_fits(const msbfs_context *c, uint64_t V) {
+    return ((V + 31) / 32) * 4 + 2 * 512 * 4 + kDeepStaticLds <= c->lds_max;
+}
+
+size_t dist_bytes(int dtype) { return dtype == MSBFS_DIST_U16 ? 2 : 4; }
 
 void drop_graph(msbfs_batch *b) {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
@@ -62,7 +77,8 @@ int batch_bits(const msbfs_batch *b) {
     for (int B : {1, 2, 4, 8, 16, 32}) {
         if (!fits(B)) break;
         uint64_t tasks = 0;
-        for (uint32_t g = 0; g < b->n; ++g) tasks += (b->S[g] + B - 1) / B;
+        for (uint32_t g = 0; g < b->n; ++g)
+            if (!b->is_wide[g]) tasks += (b->S[g] + B - 1) / B;
         if (best == 0 || tasks >= 2 * (uint64_t)c->cus) best = B;
         if (B == 8) break;  // wider tasks only when the autotuner asks for them
     }
@@ -79,13 +95,18 @@ int batch_plan(msbfs_batch *b) {
     p.method = MSBFS_METHOD_WORKGROUP_PERSISTENT_BFS;
     p.bits = batch_bits(b);
     if (p.bits == 0)
-        return fail(MSBFS_ERR_UNSUPPORTED,
-                    "batch: the largest graph (%llu vertices) does not fit one workgroup's LDS",
-                    (unsigned long long)b->max_v);
+        return fail(MSBFS_ERR_INVALID_ARG,
+                    "batch: %u sources per task of the largest graph (%llu vertices) do not fit "
+                    "one workgroup's LDS",
+                    b->tune.deep_bits, (unsigned long long)b->max_v);
+    for (msbfs_engine *we : b->wide_eng) {
+        int rc = msbfs_engine_set_tuning(we, &b->tune);
+        if (rc) return rc;
+    }
     const int B = p.bits;
     std::vector<DeepTask> tasks;
     for (uint32_t g = 0; g < b->n; ++g)
-        for (uint64_t k = 0; k * B < b->S[g]; ++k) {
+        for (uint64_t k = 0; !b->is_wide[g] && k * B < b->S[g]; ++k) {
             DeepTask t{};
             t.graph = g;
             t.src_off = b->src_off[g] + k * B;
@@ -141,6 +162,7 @@ DeepParams batch_params(msbfs_batch *b, void *out) {
     p.lds_qcap = b->plan.qcap;
     p.lds_vis_words = b->plan.vis_words;
     p.lds_sink_off = b->plan.sink_off;
+    p.staged = b->plan.staged ? 1u : 0u;  // batch outputs are u32 or u16
     p.work = b->work.as<uint32_t>();
     p.flags = b->flags.as<uint32_t>();
     p.reach = nullptr;
@@ -164,17 +186,49 @@ int batch_enqueue(msbfs_batch *b, void *out, hipStream_t st) {
     return MSBFS_OK;
 }
 
+// The wide-family graphs, one engine run each on stream `st` (not waited
+// for: batch_finish checks them).
+int batch_enqueue_wide(msbfs_batch *b, void *out, hipStream_t st) {
+    for (size_t i = 0; i < b->wide.size(); ++i) {
+        const uint32_t g = b->wide[i];
+        if (b->S[g] == 0) continue;
+        const uint64_t n = b->S[g] * b->graphs[g]->V;
+        int rc = msbfs_run_ms_bfs_device(b->wide_eng[i], b->graphs[g],
+                                         b->h_sources.data() + b->src_off[g], b->S[g], b->dtype,
+                                         (char *)out + b->out_off[g] * dist_bytes(b->dtype), n,
+                                         st, 0);
+        if (rc) return rc;
+    }
+    return MSBFS_OK;
+}
+
 int batch_finish(msbfs_batch *b) {
     if (!b->pending) return MSBFS_OK;
     b->pending = false;
+    // every wide run is finished (and its errors taken) before the timing
+    // event is read
+    int wide_rc = MSBFS_OK;
+    uint32_t wide_levels = 0;
+    for (size_t i = 0; i < b->wide.size(); ++i) {
+        if (b->S[b->wide[i]] == 0) continue;
+        int rc = msbfs_engine_check(b->wide_eng[i]);
+        if (rc) {
+            if (!wide_rc) wide_rc = rc;
+            continue;
+        }
+        msbfs_run_stats ws{};
+        if (msbfs_engine_last_stats(b->wide_eng[i], &ws) == MSBFS_OK)
+            wide_levels = std::max(wide_levels, ws.levels);
+    }
     HIP_TRY(hipEventSynchronize(b->ev[1]));
     float ms = 0;
     if (hipEventElapsedTime(&ms, b->ev[0], b->ev[1]) == hipSuccess) {
         b->last.kernel_ms = ms;
         b->last.total_ms = ms;
     }
+    if (wide_rc) return wide_rc;
     const uint32_t *f = (const uint32_t *)b->h_flags.p;
-    b->last.levels = f[FlagLevels];
+    b->last.levels = std::max(f[FlagLevels], wide_levels);
     if (f[FlagQueueOverflow]) return fail(MSBFS_ERR_HIP, "batch: internal frontier queue overflow");
     if (f[FlagU16Overflow] && b->dtype == MSBFS_DIST_U16)
         return fail(MSBFS_ERR_DIAMETER_U16, "batch: a distance >= 65535 cannot be stored as u16");
@@ -225,13 +279,33 @@ extern "C" int msbfs_batch_create(msbfs_engine *e, uint32_t n, msbfs_graph *cons
         b->out_off[g] = b->total_out;
         b->total_sources += counts[g];
         b->total_out += counts[g] * G->V;
-        b->max_v = std::max<uint64_t>(b->max_v, G->V);
+        const bool wide = !deep_fits(e->ctx, G->V);
+        b->is_wide.push_back(wide ? 1 : 0);
+        if (wide)
+            b->wide.push_back(g);
+        else
+            b->max_v = std::max<uint64_t>(b->max_v, G->V);
     }
     // One neighbour-table width for the whole launch: used only when every
-    // graph has a table of the same width.
-    b->ell = n ? b->graphs[0]->ell_width : 0;
-    for (uint32_t g = 0; g < n; ++g)
-        if (b->graphs[g]->ell_width != b->ell) b->ell = 0;
+    // graph in it has a table of the same width.
+    int ell = -1;
+    for (uint32_t g = 0; g < n; ++g) {
+        if (b->is_wide[g]) continue;
+        if (ell < 0)
+            ell = b->graphs[g]->ell_width;
+        else if (ell != b->graphs[g]->ell_width)
+            ell = 0;
+    }
+    b->ell = std::max(ell, 0);
+    for (size_t i = 0; i < b->wide.size(); ++i) {
+        msbfs_engine *we = nullptr;
+        int rc = msbfs_engine_create(e->ctx, MSBFS_METHOD_PUSH_PULL_ADAPTIVE, &we);
+        if (rc) {
+            msbfs_batch_destroy(b);
+            return rc;
+        }
+        b->wide_eng.push_back(we);
+    }
     if (b->total_sources && !sources) {
         msbfs_batch_destroy(b);
         return fail(MSBFS_ERR_INVALID_ARG, "null sources");
@@ -263,7 +337,7 @@ extern "C" int msbfs_batch_create(msbfs_engine *e, uint32_t n, msbfs_graph *cons
     for (uint32_t g = 0; g < n; ++g) {
         tab[g].ro = b->graphs[g]->ro;
         tab[g].ci = b->graphs[g]->ci;
-        tab[g].ell = b->ell ? b->graphs[g]->ell : nullptr;
+        tab[g].ell = b->ell && !b->is_wide[g] ? b->graphs[g]->ell : nullptr;
         tab[g].V = (uint32_t)b->graphs[g]->V;
         tab[g].ell_width = (uint32_t)b->ell;
     }
@@ -297,6 +371,7 @@ extern "C" int msbfs_batch_destroy(msbfs_batch *b) {
     if (!b) return MSBFS_OK;
     (void)hipSetDevice(b->eng->ctx->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (msbfs_engine *we : b->wide_eng) msbfs_engine_destroy(we);
     drop_graph(b);
     for (auto &ev : b->ev)
         if (ev) (void)hipEventDestroy(ev);
@@ -345,12 +420,16 @@ extern "C" int msbfs_batch_run(msbfs_batch *b, void *out, uint64_t out_len, void
     HIP_TRY(hipSetDevice(b->eng->ctx->device));
     if ((rc = batch_finish(b))) return rc;
     hipStream_t st = stream ? (hipStream_t)stream : b->stream;
-    LaunchOrder order(b->eng->ctx, st, false);
-    if ((rc = order.begin())) return rc;
-    HIP_TRY(hipEventRecord(b->ev[0], st));
-    if ((rc = batch_enqueue(b, out, st))) return rc;
+    {
+        LaunchOrder order(b->eng->ctx, st, false);
+        if ((rc = order.begin())) return rc;
+        HIP_TRY(hipEventRecord(b->ev[0], st));
+        if ((rc = batch_enqueue(b, out, st))) return rc;
+        if ((rc = order.end())) return rc;
+    }
+    // (each wide run takes the context's launch ordering itself)
+    if ((rc = batch_enqueue_wide(b, out, st))) return rc;
     HIP_TRY(hipEventRecord(b->ev[1], st));
-    if ((rc = order.end())) return rc;
     b->pending = true;
     return sync ? batch_finish(b) : MSBFS_OK;
 }
@@ -391,12 +470,15 @@ extern "C" int msbfs_batch_replay(msbfs_batch *b, void *stream, int sync) {
     int rc = batch_finish(b);
     if (rc) return rc;
     hipStream_t st = stream ? (hipStream_t)stream : b->stream;
-    LaunchOrder order(b->eng->ctx, st, false);
-    if ((rc = order.begin())) return rc;
-    HIP_TRY(hipEventRecord(b->ev[0], st));
-    HIP_TRY(hipGraphLaunch(b->exec, st));
+    {
+        LaunchOrder order(b->eng->ctx, st, false);
+        if ((rc = order.begin())) return rc;
+        HIP_TRY(hipEventRecord(b->ev[0], st));
+        HIP_TRY(hipGraphLaunch(b->exec, st));
+        if ((rc = order.end())) return rc;
+    }
+    if ((rc = batch_enqueue_wide(b, b->graph_out, st))) return rc;
     HIP_TRY(hipEventRecord(b->ev[1], st));
-    if ((rc = order.end())) return rc;
     b->pending = true;
     return sync ? batch_finish(b) : MSBFS_OK;
 }
@@ -423,36 +505,104 @@ extern "C" int msbfs_batch_set_tuning(msbfs_batch *b, const msbfs_tuning *t) {
     return batch_plan(b);
 }
 
-// Autotuner: every feasible (sources per task, workgroup size) pair is run
-// once to warm up and twice timed on the batch itself; the fastest stays.
+extern "C" int msbfs_batch_families(const msbfs_batch *b, int32_t *families) {
+    if (!b || (b->n && !families)) return fail(MSBFS_ERR_INVALID_ARG, "null argument");
+    for (uint32_t g = 0; g < b->n; ++g)
+        families[g] = b->is_wide[g] ? MSBFS_FAMILY_WIDE : MSBFS_FAMILY_DEEP;
+    return MSBFS_OK;
+}
+
+namespace {
+
+// Plans and times one candidate: a warm-up run, then the mean of two.
+msbfs_sweep_entry try_tuning(msbfs_batch *b, const msbfs_tuning &t, void *out, uint64_t out_len) {
+    msbfs_sweep_entry r{};
+    r.tuning = t;
+    r.ms = -1.0f;
+    b->tune = t;
+    r.status = batch_plan(b);
+    float ms = 0;
+    for (int it = 0; it < 3 && r.status == MSBFS_OK; ++it) {
+        r.status = msbfs_batch_run(b, out, out_len, nullptr, 1);
+        if (it > 0) ms += b->last.kernel_ms;
+    }
+    if (r.status == MSBFS_OK) r.ms = ms / 2;
+    b->sweep.push_back(r);
+    return r;
+}
+
+}  // namespace
+
+// Autotuner.  Deep part: every feasible (sources per task, workgroup size)
+// pair, and for one source per task staged and direct rows.  Wide part (only
+// when some graph takes the wide family): from the best deep plan, one knob
+// at a time -- wave tile, words per lane, push/pull thresholds, launch form --
+// each value kept when it is faster.
 extern "C" int msbfs_batch_autotune(msbfs_batch *b, void *out, uint64_t out_len,
                                     msbfs_tuning *best, float *best_ms) {
     clear_error();
     if (!b) return fail(MSBFS_ERR_INVALID_ARG, "null batch");
     int rc = batch_check_out(b, out, out_len);
     if (rc) return rc;
+    if ((rc = batch_finish(b))) return rc;
+    b->sweep.clear();
     const msbfs_tuning base = b->tune;
     msbfs_tuning winner = base;
     float win_ms = 1e30f;
+    auto consider = [&](const msbfs_tuning &t) {
+        const msbfs_sweep_entry r = try_tuning(b, t, out, out_len);
+        if (r.status == MSBFS_OK && r.ms < win_ms) {
+            win_ms = r.ms;
+            winner = t;
+        }
+    };
+    const bool any_deep = b->wide.size() < b->n;
     for (uint32_t bits : {1u, 2u, 4u, 8u, 16u, 32u}) {
+        if (!any_deep) break;
         if (((b->max_v * bits + 31) / 32) * 4 + kDeepStaticLds > b->eng->ctx->lds_max) break;
-        for (uint32_t threads : {256u, 512u, 1024u}) {
-            msbfs_tuning t = base;
-            t.deep_bits = bits;
-            t.deep_threads = threads;
-            b->tune = t;
-            if ((rc = batch_plan(b))) continue;
-            float ms = 0;
-            bool ok = true;
-            for (int it = 0; it < 3 && ok; ++it) {
-                if (msbfs_batch_run(b, out, out_len, nullptr, 1) != MSBFS_OK) ok = false;
-                if (it > 0) ms += b->last.kernel_ms;
+        for (uint32_t threads : {64u, 128u, 256u, 512u, 1024u})
+            for (uint32_t rows : {1u, 2u}) {
+                // staged rows: one source per task, workgroup kernel
+                if ((bits > 1 || (threads == 64 && b->ell > 0)) && rows == 1) continue;
+                msbfs_tuning t = base;
+                t.deep_bits = bits;
+                t.deep_threads = threads;
+                t.deep_rows = bits == 1 ? rows : 0;
+                consider(t);
             }
-            if (ok && ms / 2 < win_ms) {
-                win_ms = ms / 2;
-                winner = t;
+    }
+    if (!b->wide.empty()) {
+        if (win_ms >= 1e30f) consider(winner);
+        auto knob = [&](uint32_t msbfs_tuning::*field, std::initializer_list<uint32_t> vals) {
+            // from the best so far, each value of one knob
+            const msbfs_tuning from = winner;
+            for (uint32_t v : vals) {
+                if (from.*field == v) continue;
+                msbfs_tuning t = from;
+                t.*field = v;
+                consider(t);
+            }
+        };
+        // tiles up to twice the widest wide graph's default (lanes past the
+        // graph's source words stay idle)
+        uint64_t wmax = 1;
+        for (uint32_t g : b->wide) wmax = std::max<uint64_t>(wmax, (b->S[g] + 63) / 64);
+        uint32_t tw0 = 1;
+        while (tw0 < 64 && tw0 < wmax) tw0 *= 2;
+        std::vector<uint32_t> tws;
+        for (uint32_t tw = 1; tw <= std::min(64u, 2 * tw0); tw *= 2) tws.push_back(tw);
+        {
+            const msbfs_tuning from = winner;
+            for (uint32_t v : tws) {
+                msbfs_tuning t = from;
+                t.tile_words = v;
+                consider(t);
             }
         }
+        knob(&msbfs_tuning::words_per_lane, {1u, 2u});
+        knob(&msbfs_tuning::alpha, {4u, 8u, 28u});
+        knob(&msbfs_tuning::beta, {12u, 48u});
+        knob(&msbfs_tuning::stepped, {1u, 2u});
     }
     b->tune = winner;
     if ((rc = batch_plan(b))) return rc;
@@ -460,5 +610,13 @@ extern "C" int msbfs_batch_autotune(msbfs_batch *b, void *out, uint64_t out_len,
     if (best) *best = winner;
     if (best_ms) *best_ms = win_ms;
     clear_error();
+    return MSBFS_OK;
+}
+
+extern "C" int msbfs_batch_sweep(const msbfs_batch *b, msbfs_sweep_entry *entries, uint32_t cap,
+                                 uint32_t *n) {
+    if (!b || !n || (cap && !entries)) return fail(MSBFS_ERR_INVALID_ARG, "null argument");
+    *n = (uint32_t)b->sweep.size();
+    for (uint32_t i = 0; i < cap && i < *n; ++i) entries[i] = b->sweep[i];
     return MSBFS_OK;
 }

@@ -350,6 +350,7 @@ extern "C" int msbfs_tuning_validate(const msbfs_tuning *t) {
     if (t->direction_rule > 1)
         return fail(MSBFS_ERR_INVALID_ARG, "direction_rule must be 0 or 1");
     if (t->stepped > 2) return fail(MSBFS_ERR_INVALID_ARG, "stepped must be 0, 1 or 2");
+    if (t->deep_rows > 2) return fail(MSBFS_ERR_INVALID_ARG, "deep_rows must be 0, 1 or 2");
     return MSBFS_OK;
 }
 
@@ -410,29 +411,46 @@ int deep_bits_for(const msbfs_context *c, const msbfs_tuning &t, uint64_t V, uin
     return best;
 }
 
-// LDS split of a deep-family workgroup: visited bits first, the rest as two
-// frontier queues.  Enough workgroups per CU are planned to hold every task
-// at once when LDS and threads allow.
+// Staged rows (level codes in LDS, one source per task) pay off when most
+// distances fit their 4-bit codes (below 14); a graph known to need more
+// levels stores as it goes.
+constexpr uint32_t kStagedMaxLevels = 32;
+
+// LDS split of a deep-family workgroup: visited bits (and staged level
+// codes) first, the rest as two frontier queues.  Enough workgroups per CU
+// are planned to hold every task at once when LDS and threads allow.
+
 void deep_layout_v(const msbfs_context *c, const msbfs_tuning &t, uint64_t max_v, int ell,
                    uint64_t ntasks, Plan &p, uint32_t levels_seen) {
     const int B = p.bits;
     p.ell = ell;
+    const size_t esz = B <= 8 ? 4 : 8;
+    const size_t min_q = 2 * 512 * esz;
+    // each resident workgroup also holds the kernel's static LDS
+    auto dyn_budget = [&](int k) { return c->lds_max / k - kDeepStaticLds; };
+    const uint64_t vis_plain = (max_v * B + 31) / 32;
+    const uint64_t nib_words = (max_v + 7) / 8;
+    // One source per task and a frontier that fits one wave (|V| / levels,
+    // from an earlier traversal of this graph): the one-wave kernel
+    // (road-like |V| = 114,599, 3,080 levels, ms: S=2048 3.11 vs 3.44,
+    // S=256 2.07 vs 2.21; Grid 256x256 (128 per level) S=512 0.82 vs 0.57
+    // and sparse G(n,p) S=2048 11.5 vs 7.7 stay on the workgroup kernel).
+    const bool wave_rule = B == 1 && ell > 0 && levels_seen > 0 && max_v <= 64ull * levels_seen;
+    const bool wave = ell > 0 && (t.deep_threads ? t.deep_threads == 64 : wave_rule);
+    p.staged = false;
+    if (B == 1 && !wave && t.deep_rows != 2 &&
+        (vis_plain + nib_words) * 4 + min_q <= dyn_budget(1))
+        p.staged = t.deep_rows == 1 || levels_seen == 0 || levels_seen <= kStagedMaxLevels;
+    const uint64_t vis_words = vis_plain + (p.staged ? nib_words : 0);
     // Workgroup size: the 2048 threads of a CU shared by the tasks it holds
     // at once (capped by LDS), between 128 and 1024.
     if (t.deep_threads) {
         p.threads = (int)t.deep_threads;
-    } else if (B == 1 && ell > 0 && levels_seen > 0 && max_v <= 64ull * levels_seen) {
-        // One source per task and a frontier that fits one wave (|V| /
-        // levels, from an earlier traversal of this graph): the one-wave
-        // kernel (road-like |V| = 114,599, 3,080 levels, ms: S=2048 3.11 vs
-        // 3.44, S=256 2.07 vs 2.21; Grid 256x256 (128 per level) S=512 0.82
-        // vs 0.57 and sparse G(n,p) S=2048 11.5 vs 7.7 stay on the
-        // workgroup kernel).
+    } else if (wave_rule) {
         p.threads = 64;
     } else {
-        const uint64_t vis = ((max_v * B + 31) / 32) * 4;
-        const uint64_t fit = std::max<uint64_t>(
-            1, c->lds_max / (vis + 2 * 512 * (B <= 8 ? 4 : 8) + kDeepStaticLds));
+        const uint64_t fit =
+            std::max<uint64_t>(1, c->lds_max / (vis_words * 4 + min_q + kDeepStaticLds));
         const uint64_t k = std::min<uint64_t>((ntasks + c->cus - 1) / c->cus, fit);
         // (road-like |V| = 114,599, S = 2048, B = 1, eight tasks per CU, ms:
         // 64 threads 3.57, 128 3.39, 256 3.48; one task per CU: 256 best)
@@ -441,14 +459,10 @@ void deep_layout_v(const msbfs_context *c, const msbfs_tuning &t, uint64_t max_v
         else
             p.threads = k < 4 ? 512 : (k < 8 ? 256 : 128);
     }
-    p.vis_words = (uint32_t)((max_v * B + 31) / 32);
-    const size_t esz = B <= 8 ? 4 : 8;
+    p.vis_words = (uint32_t)vis_words;
     const int max_by_threads = std::max(1, 2048 / p.threads);
     int per_cu = (int)std::min<uint64_t>((ntasks + c->cus - 1) / c->cus, max_by_threads);
     per_cu = std::max(per_cu, 1);
-    const size_t min_q = 2 * 512 * esz;
-    // each resident workgroup also holds the kernel's static LDS
-    auto dyn_budget = [&](int k) { return c->lds_max / k - kDeepStaticLds; };
     while (per_cu > 1 && p.vis_words * 4 + min_q > dyn_budget(per_cu)) --per_cu;
     const size_t per_wg = dyn_budget(per_cu);
     // Logged u32 output (one-wave kernel, one source per task): one LDS
@@ -846,6 +860,7 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
         p.log_bw = (out && dtype == MSBFS_DIST_U32) ? plan.log_bw : 0;
         p.lds_bin_off = plan.bin_off;
         p.lds_sink_off = plan.sink_off;
+        p.staged = (plan.staged && (dtype == MSBFS_DIST_U32 || dtype == MSBFS_DIST_U16)) ? 1u : 0u;
         if (p.log_bw) {
             if ((rc = ensure_plain(e->log_sink, (size_t)blocks * 64 * 4))) return rc;
             p.log_sink = e->log_sink.as<uint32_t>();
@@ -1469,6 +1484,7 @@ extern "C" int msbfs_run_ms_bfs_device_noalloc(msbfs_engine *e, const msbfs_grap
     p.lds_bin_off = plan.bin_off;
     p.lds_sink_off = plan.sink_off;
     p.log_sink = (uint32_t *)(base + off_sink);
+    p.staged = (plan.staged && (dtype == MSBFS_DIST_U32 || dtype == MSBFS_DIST_U16)) ? 1u : 0u;
     HIP_TRY(hipEventRecord(e->ev[1], st));
     HIP_TRY(launch_deep(B, plan.ell, p, blocks, threads, plan.lds, st));
     HIP_TRY(hipEventRecord(e->ev[2], st));

@@ -194,6 +194,7 @@ struct Plan {
     uint32_t log_bw = 0;   // deep: vertex bin bits of the logged u32 output (0: none)
     uint32_t bin_off = 0;  // deep: LDS word offset of the log's bin counters
     uint32_t sink_off = 0; // deep one-wave kernel: LDS word offset of its append sink
+    bool staged = false;   // deep: u32/u16 rows staged in LDS as level codes
     size_t lds = 0;     // deep: dynamic LDS bytes per workgroup
 };
 

@@ -250,13 +250,28 @@ __device__ __forceinline__ void deep_write_bits(void *out_task, uint32_t V, uint
 // neighbour table (graphs of degree <= ELL) a queue entry costs one vector
 // load of its neighbour slots, its claims are independent LDS atomics, and
 // the wave appends all its new entries with one LDS atomic.
+//
+// DT = 5 / 6 (u32 / u16 rows, one source per task) stages the task's row in
+// LDS: a 4-bit code per vertex, distance + 1 for distances below 14 and 15
+// for the rest, which are stored directly as found.  The code is set with
+// one LDS atomic when the entry is expanded, in place of a scattered store
+// into the row, and the whole row is written with coalesced stores when the
+// task ends (sentinels included; codes of 15 are skipped).  Scattered
+// stores leave lines partly written in L2 over several levels and load the
+// memory system the neighbour-list loads wait on (512 G(n,p) graphs of
+// 1K-64K vertices: 12.70 ms with them, 9.12 ms with none at all).
 template <int B, int ELL, int DT>
 __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
-    // DT: 0 = no distances (reachability only), 1 = u16, 2 = u32, 3 = u8.
+    // DT: 0 = no distances (reachability only), 1 = u16, 2 = u32, 3 = u8,
+    // 5 = u32 staged, 6 = u16 staged.
     constexpr bool OUT = DT != 0;
-    constexpr uint32_t KIND = DT == 1 ? 1u : DT == 3 ? 2u : 0u;  // output element kind
-    constexpr uint32_t ESZ = DT == 1 ? 2u : DT == 3 ? 1u : 4u;
-    constexpr uint32_t DMAX = DT == 1 ? 0xFFFEu : DT == 3 ? 0xFEu : 0xFFFFFFFFu;
+    constexpr bool STG = DT == 5 || DT == 6;
+    static_assert(!STG || B == 1, "staged rows hold one source per task");
+    constexpr bool U16 = DT == 1 || DT == 6;
+    constexpr uint32_t KIND = U16 ? 1u : DT == 3 ? 2u : 0u;  // output element kind
+    constexpr uint32_t ESZ = U16 ? 2u : DT == 3 ? 1u : 4u;
+    constexpr uint32_t DMAX = U16 ? 0xFFFEu : DT == 3 ? 0xFEu : 0xFFFFFFFFu;
+    constexpr uint32_t kDirect = 14;  // staged rows: levels from here on are stored as found
     typedef typename DeepEntry<B>::T Entry;
     constexpr uint32_t VPW = 32 / B;  // vertices per LDS word
     extern __shared__ uint32_t lds[];
@@ -282,7 +297,11 @@ __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
         const DeepGraph g = p.graphs[task.graph];
         const uint32_t V = g.V;
         const uint32_t nwords = (V + VPW - 1) / VPW;
-        for (uint32_t i = tid; i < nwords; i += nth) vis[i] = 0;
+        // staged rows: 8 codes per word after this task's visited bits
+        __attribute__((address_space(3))) uint32_t *const nib =
+            (__attribute__((address_space(3))) uint32_t *)(vis + nwords);
+        const uint32_t nclear = nwords + (STG ? (V + 7) / 8 : 0);
+        for (uint32_t i = tid; i < nclear; i += nth) vis[i] = 0;
         if (tid == 0) {
             s_cnt[0] = task.nsrc;
             s_cnt[1] = 0;
@@ -326,6 +345,7 @@ __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
                 if (OUT && level > DMAX) atomicOr(&p.flags[FlagU16Overflow], 1u);
             }
             const uint32_t dval = min(level, DMAX);
+            const uint32_t code = min(level + 1, 15u);  // staged rows
             for (uint32_t base = 0; base < n; base += nth) {
                 const uint32_t i = base + tid;
                 uint32_t u = 0, f = 0;
@@ -363,7 +383,11 @@ __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
                         // This entry's lowest source bit, one straight-line
                         // store per lane; a lane without an entry repeats lane
                         // 0's store (entries are contiguous, so lane 0 has one).
-                        if constexpr (OUT) {
+                        if constexpr (STG)
+                            if (act)
+                                __hip_atomic_fetch_or(nib + u / 8, code << ((u % 8) * 4),
+                                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        if (OUT && (!STG || level >= kDirect)) {
                             const uint32_t b0 = __builtin_ctz(f | 0x80000000u);
                             const uint32_t u_l0 = (uint32_t)__builtin_amdgcn_readlane((int)u, 0);
                             const uint32_t b_l0 = (uint32_t)__builtin_amdgcn_readlane((int)b0, 0);
@@ -462,7 +486,13 @@ __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
                             else
                                 x = ((__attribute__((address_space(3))) Entry *)qa.lds)[i];
                             unpack_entry(x, u, f);
-                            if (OUT) deep_write_bits<B, KIND>(out_task, V, u, f, dval);
+                            if constexpr (STG) {
+                                __hip_atomic_fetch_or(nib + u / 8, code << ((u % 8) * 4),
+                                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                                if (level >= kDirect) deep_write_bits<B, KIND>(out_task, V, u, f, dval);
+                            } else if (OUT) {
+                                deep_write_bits<B, KIND>(out_task, V, u, f, dval);
+                            }
                         }
                         uint32_t e = 0, eend = 0;
                         if (act) {
@@ -545,11 +575,23 @@ __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
         }
         if (tid == 0) atomicMax(&p.flags[FlagLevels], level);
 
+        // Staged rows: the whole row from its codes, consecutive threads on
+        // consecutive vertices (every visited vertex has a code: it was
+        // expanded the level after it was claimed).
+        if constexpr (STG) {
+            for (uint32_t v = tid; v < V; v += nth) {
+                const uint32_t c = (nib[v / 8] >> ((v % 8) * 4)) & 15u;
+                if (c == 0)
+                    store_sentinel(out_task, KIND, v);
+                else if (c != 15u)
+                    store_elem<false>(out_task, v, (typename std::conditional<U16, uint16_t, uint32_t>::type)(c - 1));
+            }
+        }
         // Unreached (source, vertex) pairs get the sentinel; rows are written
         // by consecutive threads, so the stores coalesce.  A wave first tests
         // 64 visited words at once and skips their vertices when every bit is
         // set (a connected graph has nothing to write).
-        if constexpr (OUT) {
+        if constexpr (OUT && !STG) {
             const uint32_t lane = lane_id(), wv = tid / 64, nw = max(1u, nth / 64);
             for (uint32_t i0 = wv * 64; i0 < nwords; i0 += nw * 64) {
                 const uint32_t i = i0 + lane;
@@ -868,7 +910,10 @@ __global__ void __launch_bounds__(64) deep_wave_kernel(DeepParams p) {
 // source per task; it is null otherwise.
 template <int B, int ELL, int DT>
 static const void *deep_kernel_for(int threads) {
-    if constexpr (DT == 4) {
+    if constexpr (DT == 5 || DT == 6) {
+        if constexpr (B == 1) return (const void *)deep_bfs_kernel<1, ELL, DT>;
+        return nullptr;
+    } else if constexpr (DT == 4) {
         if constexpr (ELL > 0 && B == 1)
             if (threads == 64) return (const void *)deep_wave_kernel<B, ELL, 4>;
         return nullptr;
@@ -882,11 +927,18 @@ static const void *deep_kernel_for(int threads) {
 template <int B, int ELL>
 static hipError_t deep_launch_t(const DeepParams &p, int blocks, int threads, size_t lds,
                                 hipStream_t st) {
-    const int dt = !p.out ? 0 : p.out_kind == 1 ? 1 : p.out_kind == 2 ? 3 : p.log_bw ? 4 : 2;
+    const int dt = !p.out               ? 0
+                   : p.out_kind == 1    ? (p.staged ? 6 : 1)
+                   : p.out_kind == 2    ? 3
+                   : p.log_bw           ? 4
+                   : p.staged           ? 5
+                                        : 2;
     const void *k = dt == 0   ? deep_kernel_for<B, ELL, 0>(threads)
                     : dt == 1 ? deep_kernel_for<B, ELL, 1>(threads)
                     : dt == 3 ? deep_kernel_for<B, ELL, 3>(threads)
                     : dt == 4 ? deep_kernel_for<B, ELL, 4>(threads)
+                    : dt == 5 ? deep_kernel_for<B, ELL, 5>(threads)
+                    : dt == 6 ? deep_kernel_for<B, ELL, 6>(threads)
                               : deep_kernel_for<B, ELL, 2>(threads);
     if (!k) return hipErrorInvalidConfiguration;
     void *args[] = {(void *)&p};
@@ -917,20 +969,25 @@ static hipError_t deep_prepare_t(int threads, size_t lds) {
     if (e == hipSuccess) e = deep_prepare_dt<B, ELL, 2>(threads, lds);
     if (e == hipSuccess) e = deep_prepare_dt<B, ELL, 3>(threads, lds);
     if (e == hipSuccess) e = deep_prepare_dt<B, ELL, 4>(threads, lds);
+    if (e == hipSuccess) e = deep_prepare_dt<B, ELL, 5>(threads, lds);
+    if (e == hipSuccess) e = deep_prepare_dt<B, ELL, 6>(threads, lds);
     return e;
 }
 
 // Resident workgroups per CU: the least over the output variants.
 template <int B, int ELL>
 static hipError_t deep_occ_t(int threads, size_t lds, int *out) {
-    int o[5] = {1 << 30, 1 << 30, 1 << 30, 1 << 30, 1 << 30};
-    const void *k[5] = {deep_kernel_for<B, ELL, 0>(threads), deep_kernel_for<B, ELL, 1>(threads),
-                        deep_kernel_for<B, ELL, 2>(threads), deep_kernel_for<B, ELL, 3>(threads),
-                        deep_kernel_for<B, ELL, 4>(threads)};
+    constexpr int NK = 7;
+    int o[NK];
+    for (int i = 0; i < NK; ++i) o[i] = 1 << 30;
+    const void *k[NK] = {deep_kernel_for<B, ELL, 0>(threads), deep_kernel_for<B, ELL, 1>(threads),
+                         deep_kernel_for<B, ELL, 2>(threads), deep_kernel_for<B, ELL, 3>(threads),
+                         deep_kernel_for<B, ELL, 4>(threads), deep_kernel_for<B, ELL, 5>(threads),
+                         deep_kernel_for<B, ELL, 6>(threads)};
     hipError_t e = hipSuccess;
-    for (int i = 0; i < 5 && e == hipSuccess; ++i)
+    for (int i = 0; i < NK && e == hipSuccess; ++i)
         if (k[i]) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&o[i], k[i], threads, lds);
-    *out = *std::min_element(o, o + 5);
+    *out = *std::min_element(o, o + NK);
     return e;
 }
 

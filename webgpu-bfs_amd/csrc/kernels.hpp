@@ -50,6 +50,9 @@ struct DeepParams {
     uint32_t lds_bin_off;   // LDS word offset of the per-bin log counters
     uint32_t *log_sink;     // 64 words per workgroup: stores of lanes without a log entry
     uint32_t lds_sink_off;  // one-wave kernel: LDS word offset of its 64 sink entries
+    // u32 / u16 rows staged in LDS as 4-bit level codes after the visited
+    // bits (workgroup kernel, one source per task); lds_vis_words covers both
+    uint32_t staged;
 };
 // Vertex bins of the logged output: 4,096 vertices (16 KiB of u32 in LDS).
 constexpr uint32_t kDeepLogBits = 12;

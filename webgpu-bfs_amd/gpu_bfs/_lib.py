@@ -105,7 +105,11 @@ class GraphInfo(ctypes.Structure):
 class Tuning(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint32) for n in (
         "alpha", "beta", "tile_words", "words_per_lane", "deep_bits", "deep_threads",
-        "wide_blocks_per_cu", "direction_rule", "trace_levels", "stepped")]
+        "wide_blocks_per_cu", "direction_rule", "trace_levels", "stepped", "deep_rows")]
+
+
+class SweepEntry(ctypes.Structure):
+    _fields_ = [("tuning", Tuning), ("ms", ctypes.c_float), ("status", ctypes.c_int32)]
 
 
 class LevelRecord(ctypes.Structure):
@@ -194,6 +198,8 @@ SIGNATURES = {
     "msbfs_batch_stats": (I32, [P, ctypes.POINTER(RunStats)]),
     "msbfs_batch_set_tuning": (I32, [P, ctypes.POINTER(Tuning)]),
     "msbfs_batch_autotune": (I32, [P, P, U64, ctypes.POINTER(Tuning), ctypes.POINTER(ctypes.c_float)]),
+    "msbfs_batch_families": (I32, [P, P]),
+    "msbfs_batch_sweep": (I32, [P, ctypes.POINTER(SweepEntry), U32, ctypes.POINTER(U32)]),
     "msbfs_server_create": (I32, [P, ctypes.POINTER(ServerConfig), PP]),
     "msbfs_server_add_graph": (I32, [P, P, I32, ctypes.POINTER(U32)]),
     "msbfs_server_set_tuning": (I32, [P, U32, ctypes.POINTER(Tuning)]),

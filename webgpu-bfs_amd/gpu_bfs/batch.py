@@ -81,6 +81,31 @@ class Batch:
                                              ctypes.byref(t), ctypes.byref(ms)))
         return {k: getattr(t, k) for k, _ in t._fields_}, ms.value
 
+    def sweep(self) -> list:
+        """The last autotune's candidates in the order they ran: dicts with
+        the tuning knobs, "ms" (None when the candidate did not run) and
+        "status"."""
+        n = ctypes.c_uint32()
+        L.check(L.lib().msbfs_batch_sweep(self._h, None, 0, ctypes.byref(n)))
+        arr = (L.SweepEntry * max(n.value, 1))()
+        L.check(L.lib().msbfs_batch_sweep(self._h, arr, n.value, ctypes.byref(n)))
+        res = []
+        for e in arr[:n.value]:
+            d = {k: getattr(e.tuning, k) for k, _ in e.tuning._fields_}
+            d["ms"] = float(e.ms) if e.ms >= 0 else None
+            d["status"] = e.status
+            res.append(d)
+        return res
+
+    def families(self) -> list:
+        """Family (gpu_bfs.Family value) that traverses each graph: DEEP in
+        the persistent launch, WIDE for graphs whose visited bits exceed one
+        workgroup's LDS."""
+        n = len(self.dgraphs)
+        arr = (ctypes.c_int32 * max(n, 1))()
+        L.check(L.lib().msbfs_batch_families(self._h, arr))
+        return list(arr[:n])
+
     def split(self, out_host: np.ndarray) -> list:
         """Per-graph [S_g, V_g] views of a host copy of the batch output."""
         res = []
