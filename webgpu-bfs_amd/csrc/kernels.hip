@@ -1224,7 +1224,31 @@ struct WaveCtx {
                          // bitmap words from the block's first, [4..68) the vertices
     bool direct;         // this level's new distances are stored now (not deferred)
     uint32_t *wq;        // this wave's 64 LDS slots for WaveAppender
+    uint32_t *par;       // this wave's 64 LDS slots: each lane group's frontier parents of a step
 };
+
+// A lane group's frontier parents of one scan step, listed in LDS: the lane
+// that loaded in-edge parent u (frontier bit set) writes u at its rank
+// among the group's such lanes, so a gather round reads eight parents with
+// two 16-byte LDS reads instead of finding the ballot's next bit and
+// broadcasting it from its lane eight times.  `gl` = the group's TW slots;
+// returns the group's count.  (A wave's LDS operations execute in order,
+// so the list is read after it is written without a wait.)
+__device__ __forceinline__ uint32_t list_parents(uint32_t *gl, bool inf, uint32_t u, u64 gmask) {
+    const u64 m = __ballot(inf) & gmask;
+    if (inf) gl[mbcnt64(m, 0)] = u;
+    return (uint32_t)__popcll(m);
+}
+
+// Eight parents of a list from LDS (slots r .. r+7; r a multiple of 8).
+__device__ __forceinline__ void read_parents(const uint32_t *gl, uint32_t r, uint32_t u[8]) {
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const __attribute__((address_space(3))) u32x4 *q =
+        (const __attribute__((address_space(3))) u32x4 *)(gl + r);
+    const u32x4 a = q[0], b = q[1];
+    u[0] = a.x; u[1] = a.y; u[2] = a.z; u[3] = a.w;
+    u[4] = b.x; u[5] = b.y; u[6] = b.z; u[7] = b.w;
+}
 
 // Appends to a global queue through the wave's LDS slots: one atomic on the
 // queue counter per 64 entries instead of one per call (the counters are
@@ -1471,25 +1495,17 @@ __device__ __forceinline__ WT scan_gather(const WideParams &p, const WaveCtx &x,
         uu = ((g32)p.ri)[e];
         inf = (((g32)fb)[uu >> 5] >> (uu & 31)) & 1u;
     }
-    u64 m = __ballot(inf) & gmask;
+    uint32_t *const gl = x.par + x.vi * TW;
+    uint32_t cnt = list_parents(gl, inf, uu, gmask);
     WT acc = wconst<WT>(0);
     const uint64_t W = p.W;
-    while (__any(m != 0)) {
-        uint32_t j[8];
-        bool ok[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            ok[k] = m != 0;
-            j[k] = ok[k] ? (uint32_t)__builtin_ctzll(m) : x.lane;
-            m &= m - 1;
-        }
+    for (uint32_t r = 0; __any(r < cnt); r += 8) {
         uint32_t u[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) u[k] = (uint32_t)__shfl((int)uu, (int)j[k]);
+        read_parents(gl, r, u);
 #pragma unroll
         for (int k = 0; k < 8; ++k)
-            acc |= *(gw)((ok[k] && wok) ? cur + (uint64_t)u[k] * W + w : p.zrow + (w & 63));
-        if (!(__ballot(nz(((have | acc) & need) ^ need)) & gmask)) m = 0;
+            acc |= *(gw)((r + k < cnt && wok) ? cur + (uint64_t)u[k] * W + w : p.zrow + (w & 63));
+        if (!(__ballot(nz(((have | acc) & need) ^ need)) & gmask)) cnt = 0;
     }
     return acc;
 }
@@ -1618,30 +1634,22 @@ __device__ __forceinline__ WT heavy_gather(const WideParams &p, const WaveCtx &x
     uint32_t uu = e < e1 ? ((g32)p.ri)[e] : 0u;
     uint32_t fw = e < e1 ? ((g32)fb)[uu >> 5] : 0u;
     uint32_t un = e + 64 < e1 ? ((g32)p.ri)[e + 64] : 0u;
+    uint32_t *const gl = x.par + x.vi * TW;
     WT acc = wconst<WT>(0), got = wconst<WT>(0);
     for (uint32_t base = e0; base < e1; base += 64) {
         const bool inf = e < e1 && ((fw >> (uu & 31)) & 1u);
-        u64 m = __ballot(inf) & gmask;
+        const uint32_t cnt = list_parents(gl, inf, uu, gmask);
         const uint32_t en = e + 64;
         const uint32_t fwn = en < e1 ? ((g32)fb)[un >> 5] : 0u;
         const uint32_t unn = en + 64 < e1 ? ((g32)p.ri)[en + 64] : 0u;
         // a lane whose word already has every needed bit reads the zero row
         const bool lok = wok && nz(need & ~got);
-        while (__any(m != 0)) {
-            uint32_t j[8];
-            bool ok[8];
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                ok[k] = m != 0;
-                j[k] = ok[k] ? (uint32_t)__builtin_ctzll(m) : x.lane;
-                m &= m - 1;
-            }
+        for (uint32_t r = 0; __any(r < cnt); r += 8) {
             uint32_t u[8];
-#pragma unroll
-            for (int k = 0; k < 8; ++k) u[k] = (uint32_t)__shfl((int)uu, (int)j[k]);
+            read_parents(gl, r, u);
 #pragma unroll
             for (int k = 0; k < 8; ++k)
-                acc |= *(gw)((ok[k] && lok) ? cur + (uint64_t)u[k] * W + w : p.zrow + (w & 63));
+                acc |= *(gw)((r + k < cnt && lok) ? cur + (uint64_t)u[k] * W + w : p.zrow + (w & 63));
         }
         got = group_or<TV>(acc);
         if (__all(!nz((got & need) ^ need))) break;
@@ -2116,10 +2124,12 @@ __global__ void __launch_bounds__(kWideThreads) __attribute__((amdgpu_waves_per_
     for (uint32_t w = threadIdx.x; w < W; w += blockDim.x) x.act_lds[w] = 0;
     __shared__ uint32_t s_blk[4 + 64];
     __shared__ uint32_t s_wq[kWideThreads];
+    __shared__ __align__(16) uint32_t s_par[kWideThreads];
     __shared__ u64 s_sum[2];
     if (threadIdx.x < 2) s_sum[threadIdx.x] = 0;
     x.blk = s_blk;
     x.wq = s_wq + (threadIdx.x / 64) * 64;
+    x.par = s_par + (threadIdx.x / 64) * 64;
     if (threadIdx.x < 4) s_blk[threadIdx.x] = 0;
     __syncthreads();
 
@@ -2359,10 +2369,12 @@ __global__ void __launch_bounds__(kWideThreads) __attribute__((amdgpu_waves_per_
     for (uint32_t w = threadIdx.x; w < W; w += blockDim.x) x.act_lds[w] = 0;
     __shared__ uint32_t s_blk[4 + 64];
     __shared__ uint32_t s_wq[kWideThreads];
+    __shared__ __align__(16) uint32_t s_par[kWideThreads];
     __shared__ u64 s_sum[2];
     if (threadIdx.x < 2) s_sum[threadIdx.x] = 0;
     x.blk = s_blk;
     x.wq = s_wq + (threadIdx.x / 64) * 64;
+    x.par = s_par + (threadIdx.x / 64) * 64;
     if (threadIdx.x < 4) s_blk[threadIdx.x] = 0;
     __syncthreads();
 
