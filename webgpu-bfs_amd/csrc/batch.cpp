@@ -163,6 +163,7 @@ DeepParams batch_params(msbfs_batch *b, void *out) {
     p.lds_vis_words = b->plan.vis_words;
     p.lds_sink_off = b->plan.sink_off;
     p.staged = b->plan.staged ? 1u : 0u;  // batch outputs are u32 or u16
+    p.log_defer_after = (uint32_t)b->ntasks;  // (no logged rows in a batch)
     p.work = b->work.as<uint32_t>();
     p.flags = b->flags.as<uint32_t>();
     p.reach = nullptr;

@@ -411,6 +411,12 @@ int deep_bits_for(const msbfs_context *c, const msbfs_tuning &t, uint64_t V, uin
     return best;
 }
 
+// Logged rows: the tasks that finish in the last part of a launch leave
+// their rows' rewrite to a separate pass over the whole device (1/kLogDefer
+// of the tasks).
+constexpr uint64_t kLogDefer = 4;
+uint64_t log_deferred_tasks(uint64_t ntasks) { return ntasks / kLogDefer; }
+
 // Staged rows (level codes in LDS, one source per task) pay off when most
 // distances fit their 4-bit codes (below 14); a graph known to need more
 // levels stores as it goes.
@@ -861,9 +867,17 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
         p.lds_bin_off = plan.bin_off;
         p.lds_sink_off = plan.sink_off;
         p.staged = (plan.staged && (dtype == MSBFS_DIST_U32 || dtype == MSBFS_DIST_U16)) ? 1u : 0u;
+        p.log_defer_after = (uint32_t)ntasks;
+        uint32_t log_nbins = 0;
         if (p.log_bw) {
             if ((rc = ensure_plain(e->log_sink, (size_t)blocks * 64 * 4))) return rc;
             p.log_sink = e->log_sink.as<uint32_t>();
+            log_nbins = (uint32_t)((V + (1u << p.log_bw) - 1) >> p.log_bw);
+            const uint64_t nlist = log_deferred_tasks(ntasks);
+            if ((rc = ensure_plain(e->log_defer, std::max<size_t>(1, nlist * (1 + log_nbins) * 4))))
+                return rc;
+            p.log_defer = e->log_defer.as<uint32_t>();
+            p.log_defer_after = (uint32_t)(ntasks - nlist);
         }
         p.work = e->work.as<uint32_t>();
         p.flags = e->flags.as<uint32_t>();
@@ -879,6 +893,7 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
         p.out_kind = dist_kind(dtype);
         HIP_TRY(hipEventRecord(e->ev[1], st));
         HIP_TRY(launch_deep(B, plan.ell, p, blocks, threads, plan.lds, st));
+        if (p.log_bw) HIP_TRY(launch_log_rewrite(p, log_nbins, st));
         if (reach)
             HIP_TRY(launch_reach_transpose(e->reach_t.as<uint32_t>(), vw_n, S, (uint32_t)V, reach,
                                            reach_wpv, st));
@@ -1434,7 +1449,12 @@ extern "C" int msbfs_run_ms_bfs_device_noalloc(msbfs_engine *e, const msbfs_grap
     const size_t off_tasks = align256(off_graph + sizeof(DeepGraph));
     const size_t off_src = align256(off_tasks + ntasks * sizeof(DeepTask));
     const size_t off_sink = align256(off_src + S * 4);
-    const size_t off_queues = align256(off_sink + (size_t)blocks * 64 * 4);
+    // logged rows: the deferred tasks' list and bin counts
+    const bool logged = dtype == MSBFS_DIST_U32 && plan.log_bw;
+    const uint64_t log_nbins = logged ? (V + (1u << plan.log_bw) - 1) >> plan.log_bw : 0;
+    const uint64_t log_nlist = logged ? log_deferred_tasks(ntasks) : 0;
+    const size_t off_defer = align256(off_sink + (size_t)blocks * 64 * 4);
+    const size_t off_queues = align256(off_defer + log_nlist * (1 + log_nbins) * 4);
     if (off_queues + (size_t)blocks * 2 * esz > MSBFS_NOALLOC_SCRATCH_BYTES)
         return fail(MSBFS_ERR_OUT_TOO_SMALL, "no-alloc call: the task table does not fit its scratch");
     // spill entries per queue: the worst case (every vertex in one level)
@@ -1484,9 +1504,12 @@ extern "C" int msbfs_run_ms_bfs_device_noalloc(msbfs_engine *e, const msbfs_grap
     p.lds_bin_off = plan.bin_off;
     p.lds_sink_off = plan.sink_off;
     p.log_sink = (uint32_t *)(base + off_sink);
+    p.log_defer = (uint32_t *)(base + off_defer);
+    p.log_defer_after = (uint32_t)(ntasks - log_nlist);
     p.staged = (plan.staged && (dtype == MSBFS_DIST_U32 || dtype == MSBFS_DIST_U16)) ? 1u : 0u;
     HIP_TRY(hipEventRecord(e->ev[1], st));
     HIP_TRY(launch_deep(B, plan.ell, p, blocks, threads, plan.lds, st));
+    if (p.log_bw) HIP_TRY(launch_log_rewrite(p, (uint32_t)log_nbins, st));
     HIP_TRY(hipEventRecord(e->ev[2], st));
     if ((rc = order.end())) return rc;
     HIP_TRY(hipStreamSynchronize(st));

@@ -844,7 +844,27 @@ __global__ void __launch_bounds__(64) deep_wave_kernel(DeepParams p) {
                 }
             }
         }
+        // The tasks that finish last leave their rows to a separate pass, so
+        // that their rewrites run on the whole device rather than one wave
+        // each after the traversal's longest levels (road-like |V| = 114,599,
+        // 2,048 sources: the in-kernel rewrites were 0.54 ms of 2.72).
+        bool defer = false;
         if constexpr (LOG) {
+            uint32_t done = 0;
+            if (lane == 0) done = atomicAdd(p.work + 1, 1u);
+            done = (uint32_t)__builtin_amdgcn_readlane((int)done, 0);
+            if (done >= p.log_defer_after) {
+                defer = true;
+                uint32_t k = 0;
+                if (lane == 0) k = atomicAdd(p.work + 2, 1u);
+                k = (uint32_t)__builtin_amdgcn_readlane((int)k, 0);
+                const uint32_t nlist = p.ntasks - p.log_defer_after;
+                if (lane == 0) p.log_defer[k] = t;
+                uint32_t *const counts = p.log_defer + nlist + (uint64_t)k * nbins;
+                for (uint32_t c = lane; c < nbins; c += 64) counts[c] = lbin[c];
+            }
+        }
+        if constexpr (LOG) if (!defer) {
             // The log entries were stored by this wave; make them visible
             // to its loads (the vector L1 is not kept coherent with them).
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
@@ -886,8 +906,8 @@ __global__ void __launch_bounds__(64) deep_wave_kernel(DeepParams p) {
                     rewrite_bin(c + 1, nxt);
                 }
             }
-        } else
-        if constexpr (OUT) {
+        }
+        if constexpr (!LOG && OUT) {
             // sentinels for unreached pairs, skipping runs of 64 full visited words
             for (uint32_t i0 = 0; i0 < nwords; i0 += 64) {
                 const uint32_t i = i0 + lane;
@@ -902,6 +922,50 @@ __global__ void __launch_bounds__(64) deep_wave_kernel(DeepParams p) {
             }
         }
     }
+}
+
+// Rewrite of the logged rows listed by deep_wave_kernel: one workgroup per
+// (listed task, bin).  The bin's log (at most 2^log_bw entries of (vertex
+// in bin, level), the span's own first elements) is loaded, the span's
+// distances are built in LDS over a sentinel fill, and the span is stored
+// whole.  Workgroups beyond the listed tasks return at once.
+constexpr uint32_t kLogRewriteThreads = 256;
+__global__ void __launch_bounds__(kLogRewriteThreads) deep_log_rewrite_kernel(DeepParams p,
+                                                                              uint32_t nbins) {
+    __shared__ uint32_t chunk[1u << kDeepLogBits];
+    constexpr uint32_t PER = (1u << kDeepLogBits) / kLogRewriteThreads;
+    const uint32_t k = blockIdx.x / nbins, c = blockIdx.x % nbins;
+    if (k >= p.work[2]) return;  // uniform (written by the traversal's launch)
+    const uint32_t nlist = p.ntasks - p.log_defer_after;
+    const uint32_t t = p.log_defer[k];
+    const uint32_t cnt = p.log_defer[nlist + (uint64_t)k * nbins + c];
+    const DeepTask task = p.tasks[t];
+    const uint32_t V = p.graphs[task.graph].V;
+    const uint32_t lbw = p.log_bw, lmask = (1u << lbw) - 1u;
+    const uint32_t c0 = c << lbw, len = min(lmask + 1u, V - c0);
+    uint32_t *const span = (uint32_t *)p.out + task.out_off + c0;
+    uint32_t e[PER];
+#pragma unroll
+    for (uint32_t q = 0; q < PER; ++q) {
+        const uint32_t j = q * kLogRewriteThreads + threadIdx.x;
+        e[q] = j < cnt ? span[j] : 0xFFFFFFFFu;
+    }
+    for (uint32_t x = threadIdx.x; x < len; x += kLogRewriteThreads) chunk[x] = 0xFFFFFFFFu;
+    __syncthreads();
+#pragma unroll
+    for (uint32_t q = 0; q < PER; ++q)
+        if (e[q] != 0xFFFFFFFFu) chunk[e[q] & lmask] = e[q] >> lbw;
+    __syncthreads();
+    for (uint32_t x = threadIdx.x; x < len; x += kLogRewriteThreads)
+        __builtin_nontemporal_store(chunk[x], span + x);
+}
+
+hipError_t launch_log_rewrite(const DeepParams &p, uint32_t nbins, hipStream_t st) {
+    const uint64_t blocks = (uint64_t)(p.ntasks - p.log_defer_after) * nbins;
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(deep_log_rewrite_kernel, dim3((unsigned)blocks), dim3(kLogRewriteThreads), 0,
+                       st, p, nbins);
+    return hipGetLastError();
 }
 
 // The deep kernel for a launch: the one-wave kernel for 64-thread

@@ -53,6 +53,13 @@ struct DeepParams {
     // u32 / u16 rows staged in LDS as 4-bit level codes after the visited
     // bits (workgroup kernel, one source per task); lds_vis_words covers both
     uint32_t staged;
+    // Logged rows of the tasks that finish last are rewritten by a separate
+    // pass: the (log_defer_after + 1)-th task to finish and later ones list
+    // themselves in log_defer[0 .. ntasks - log_defer_after) and their bins'
+    // entry counts after that list (nbins per listed task); work[1] counts
+    // finished tasks, work[2] listed ones.
+    uint32_t log_defer_after;
+    uint32_t *log_defer;
 };
 // Vertex bins of the logged output: 4,096 vertices (16 KiB of u32 in LDS).
 constexpr uint32_t kDeepLogBits = 12;
@@ -150,6 +157,8 @@ enum : int {
     FlagCount = 8
 };
 
+// The rewrite of the listed tasks' logged rows (after launch_deep).
+hipError_t launch_log_rewrite(const DeepParams &p, uint32_t nbins, hipStream_t st);
 hipError_t launch_deep(int bits, int ell, const DeepParams &p, int blocks, int threads,
                        size_t lds_bytes, hipStream_t st);
 hipError_t deep_max_blocks_per_cu(int bits, int ell, int threads, size_t lds_bytes, int *out);
