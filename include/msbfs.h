@@ -139,8 +139,9 @@ typedef struct msbfs_tuning {
                                 1 staged, the row kept in LDS as a 4-bit code per vertex and written
                                 whole when the task ends (distances >= 14 stored as found); 2 every
                                 distance stored as found; 0 auto: staged when the graph's deep
-                                traversals are known to need <= 32 levels or none has run yet, and
-                                the codes fit in LDS */
+                                traversals are known to need <= 32 levels, or, before the first
+                                one, when it is not road-like (max degree <= 8 and |E| <= 4|V|),
+                                and the codes fit in LDS */
 } msbfs_tuning;
 
 /* One BFS level of the last wide-family run (recorded when trace_levels > 0). */

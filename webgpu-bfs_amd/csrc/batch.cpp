@@ -120,7 +120,11 @@ int batch_plan(msbfs_batch *b) {
         return b->graphs[x.graph]->E > b->graphs[y.graph]->E;
     });
     b->ntasks = tasks.size();
-    deep_layout_v(c, b->tune, b->max_v, b->ell, b->ntasks, p, 0);
+    bool road_like = true;
+    for (uint32_t g = 0; g < b->n; ++g)
+        if (!b->is_wide[g])
+            road_like = road_like && b->graphs[g]->max_degree <= 8 && b->graphs[g]->E <= 4 * b->graphs[g]->V;
+    deep_layout_v(c, b->tune, b->max_v, b->ell, b->ntasks, p, 0, road_like);
     HIP_TRY(deep_prepare(B, p.ell, p.threads, p.lds));
     int occ = 0;
     HIP_TRY(deep_max_blocks_per_cu(B, p.ell, p.threads, p.lds, &occ));

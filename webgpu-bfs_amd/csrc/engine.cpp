@@ -419,7 +419,11 @@ uint64_t log_deferred_tasks(uint64_t ntasks) { return ntasks / kLogDefer; }
 
 // Staged rows (level codes in LDS, one source per task) pay off when most
 // distances fit their 4-bit codes (below 14); a graph known to need more
-// levels stores as it goes.
+// levels stores as it goes, and so does one that looks road-like before its
+// first traversal: its codes would take LDS that holds more tasks at once,
+// and such a traversal is bound by its levels (road-like |V| = 114,599, 2,048
+// sources, first traversal: 3.5 ms direct, 9.6 ms staged at a quarter of
+// the tasks per CU).
 constexpr uint32_t kStagedMaxLevels = 32;
 
 // LDS split of a deep-family workgroup: visited bits (and staged level
@@ -427,7 +431,7 @@ constexpr uint32_t kStagedMaxLevels = 32;
 // are planned to hold every task at once when LDS and threads allow.
 
 void deep_layout_v(const msbfs_context *c, const msbfs_tuning &t, uint64_t max_v, int ell,
-                   uint64_t ntasks, Plan &p, uint32_t levels_seen) {
+                   uint64_t ntasks, Plan &p, uint32_t levels_seen, bool road_like) {
     const int B = p.bits;
     p.ell = ell;
     const size_t esz = B <= 8 ? 4 : 8;
@@ -446,7 +450,8 @@ void deep_layout_v(const msbfs_context *c, const msbfs_tuning &t, uint64_t max_v
     p.staged = false;
     if (B == 1 && !wave && t.deep_rows != 2 &&
         (vis_plain + nib_words) * 4 + min_q <= dyn_budget(1))
-        p.staged = t.deep_rows == 1 || levels_seen == 0 || levels_seen <= kStagedMaxLevels;
+        p.staged = t.deep_rows == 1 ||
+                   (levels_seen ? levels_seen <= kStagedMaxLevels : !road_like);
     const uint64_t vis_words = vis_plain + (p.staged ? nib_words : 0);
     // Workgroup size: the 2048 threads of a CU shared by the tasks it holds
     // at once (capped by LDS), between 128 and 1024.
@@ -570,7 +575,8 @@ Plan plan_for(const msbfs_engine *e, const msbfs_graph *g, uint64_t S) {
             p.policy = MSBFS_DIR_ADAPTIVE;
         } else {
             deep_layout_v(e->ctx, t, g->V, g->ell_width, (S + p.bits - 1) / p.bits, p,
-                          g->deep_levels.load(std::memory_order_relaxed));
+                          g->deep_levels.load(std::memory_order_relaxed),
+                          g->max_degree <= 8 && g->E <= 4 * g->V);
         }
     }
     return p;

@@ -223,9 +223,11 @@ void engine_release(msbfs_engine *e);
 int ensure_zeroed(DevBuf &b, size_t bytes, bool force, hipStream_t st);
 int ensure_plain(DevBuf &b, size_t bytes);
 // LDS split of a deep-family workgroup for graphs of up to max_v vertices.
+// levels_seen: the graphs' deep level count from an earlier traversal (0:
+// none yet); road_like: every graph passes the reference's road-like test
+// (max degree <= 8 and |E| <= 4|V|), taken as a sign of many levels.
 void deep_layout_v(const msbfs_context *c, const msbfs_tuning &t, uint64_t max_v, int ell,
-                   uint64_t ntasks, Plan &p,
-                   uint32_t levels_seen);
+                   uint64_t ntasks, Plan &p, uint32_t levels_seen, bool road_like);
 int deep_bits_for(const msbfs_context *c, const msbfs_tuning &t, uint64_t V, uint64_t S);
 
 }  // namespace msbfs
