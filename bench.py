@@ -240,10 +240,13 @@ def measure(args, config, steps, warmup, with_cpu, with_host, with_trace=False):
     model_bytes, te, maxl, per_level = level_bytes(dist2d, S, V, E, deg, b_d)
     md = int(deg.max()) if V else 0
     ell = 4 if md <= 4 else (8 if md <= 8 else 0)
+    lat = int(stats["lattice_bits"])
     # Compulsory HBM bytes of one launch: every distance written once, the
-    # graph (CSR, plus the neighbour table of the deep family's one-wave
+    # graph in the form the kernel reads (the lattice kernel: its adjacency
+    # bits; otherwise the CSR, plus the neighbour table of the one-wave
     # kernel when the maximum degree is <= 8) and the sources read once.
-    compulsory = b_d * S * V + 4 * (V + 1) + 4 * E + 4 * ell * V + 4 * S
+    graph_bytes = (V * lat + 7) // 8 if lat else 4 * (V + 1) + 4 * E + 4 * ell * V
+    compulsory = b_d * S * V + graph_bytes + 4 * S
     deep = stats["family"] == 1
     # The deep family keeps all per-source state in LDS, so only the
     # compulsory bytes reach HBM; the wide family keeps source words in HBM
@@ -253,7 +256,8 @@ def measure(args, config, steps, warmup, with_cpu, with_host, with_trace=False):
     achieved = alg_bytes / (kmean / 1e3) / 1e9
     traffic, traffic_src = traffic_of(config)
     if deep:
-        kernel = ("deep_wave_kernel" if stats["block_threads"] == 64 else "deep_bfs_kernel")
+        kernel = ("deep_wave_kernel (lattice adjacency in LDS)" if lat
+                  else "deep_wave_kernel" if stats["block_threads"] == 64 else "deep_bfs_kernel")
     else:
         kernel = "wide_bfs_kernel + wide_final_kernel"
     res = {
@@ -270,6 +274,7 @@ def measure(args, config, steps, warmup, with_cpu, with_host, with_trace=False):
                    "push_levels": int(stats["push_levels"]),
                    "pull_levels": int(stats["pull_levels"]),
                    "bits_per_instance": int(stats["bits_per_instance"]),
+                   "lattice_bits": lat,
                    "grid_blocks": int(stats["grid_blocks"]),
                    "block_threads": int(stats["block_threads"]), "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,

@@ -141,7 +141,17 @@ typedef struct msbfs_tuning {
                                 distance stored as found; 0 auto: staged when the graph's deep
                                 traversals are known to need <= 32 levels, or, before the first
                                 one, when it is not road-like (max degree <= 8 and |E| <= 4|V|),
-                                and the codes fit in LDS */
+                                and the codes fit in LDS.  One-wave kernel, u32 output: 2 stores
+                                every distance as found instead of through the binned log (used
+                                with four or more tasks per CU).  Lattice kernel, u32 output: 1
+                                writes through the binned log, else every distance is stored as
+                                found */
+    uint32_t deep_lattice;   /* deep family, one source per task, graphs whose every edge joins ids
+                                that differ by one of at most four offsets (grids, grid-derived
+                                road models): the lattice kernel keeps the whole adjacency in LDS
+                                (2 or 4 bits per vertex) shared by a workgroup's waves, one task per
+                                wave.  0 auto: in place of the one-wave kernel; 1 for every
+                                one-source-per-task run on such a graph; 2 never */
 } msbfs_tuning;
 
 /* One BFS level of the last wide-family run (recorded when trace_levels > 0). */
@@ -167,6 +177,8 @@ typedef struct msbfs_run_stats {
     uint32_t grid_blocks;
     uint32_t block_threads;  /* workgroup size of the traversal kernel (deep family: 64 is the
                                 one-wave kernel) */
+    uint32_t lattice_bits;   /* deep family: bits per vertex of the LDS lattice adjacency the run
+                                used (0: none) */
     float kernel_ms;         /* traversal kernel(s) device time, hipEvents */
     float total_ms;          /* whole call device time, hipEvents */
     uint64_t workspace_bytes;
@@ -421,6 +433,13 @@ int msbfs_csr_to_csc(uint64_t num_vertices, uint64_t num_edges, const uint32_t *
 /* CsrGraph::is_symmetric (graph.rs:129-132): 1 / 0, or a negative status. */
 int msbfs_csr_is_symmetric(uint64_t num_vertices, uint64_t num_edges,
                            const uint32_t *row_offsets, const uint32_t *col_indices);
+/* Lattice test of the deep family's LDS adjacency (no counterpart in the reference): for a
+ * symmetric graph whose every edge joins ids that differ by one of at most four offsets, writes
+ * them ascending to offsets[0..k) (the rest 0) and returns k >= 1; returns 0 for any other graph
+ * (offsets all 0), or a negative status. */
+int msbfs_csr_lattice_offsets(uint64_t num_vertices, uint64_t num_edges,
+                              const uint32_t *row_offsets, const uint32_t *col_indices,
+                              uint32_t *offsets);
 /* CsrGraph::validate (graph.rs:226-278): MSBFS_OK or MSBFS_ERR_INVALID_GRAPH. */
 int msbfs_csr_validate(uint64_t num_vertices, uint64_t num_edges, const uint32_t *row_offsets,
                        const uint32_t *col_indices);

@@ -131,9 +131,12 @@ def test_road_like_deep_tile_variants(ctx, road):
 
 def test_deep_one_wave_routing(ctx, road):
     """The first deep traversal of a graph records its level count; later
-    ones take the one-wave kernel when |V| / levels <= 64 (road-like: ~37
-    vertices per level) and keep the workgroup kernel otherwise (Grid
-    256x256: ~128).  Every run matches the oracle."""
+    ones take a one-task-per-wave kernel when |V| / levels <= 64 (road-like:
+    ~37 vertices per level) and keep the workgroup kernel otherwise (Grid
+    256x256: ~128).  Both graphs are lattices (ids joined by offsets 1 and
+    the row length), so the one-task-per-wave kernel is the lattice kernel
+    (adjacency in LDS); with the lattice turned off it is the one-wave
+    kernel.  Every run matches the oracle."""
     for g, wave in ((road, True), (gb.generate_grid2d(256, 256), False)):
         src = wl.sources(g, 512)
         expect = oracle(g, src)
@@ -144,7 +147,13 @@ def test_deep_one_wave_routing(ctx, road):
             assert np.array_equal(e.run_ms_bfs(dg, src), expect), (g.num_vertices, run)
             st = e.last_stats()
             assert st["bits_per_instance"] == 1
-            assert (st["block_threads"] == 64) == (wave and run > 0), (g.num_vertices, run, st)
+            assert (st["lattice_bits"] == 2) == (wave and run > 0), (g.num_vertices, run, st)
+            assert st["lattice_bits"] or st["block_threads"] != 64
+        e.set_tuning(deep_lattice=2)
+        say("one-wave routing V=%d, lattice off" % g.num_vertices)
+        assert np.array_equal(e.run_ms_bfs(dg, src), expect), g.num_vertices
+        st = e.last_stats()
+        assert st["lattice_bits"] == 0 and (st["block_threads"] == 64) == wave, (g.num_vertices, st)
 
 
 @pytest.fixture(scope="module")
@@ -565,6 +574,67 @@ def test_logged_u32_rows(ctx, road):
             st = e.last_stats()
             assert st["block_threads"] == 64 and st["bits_per_instance"] == 1
             assert st["instances"] >= 4 * ctx.info()["compute_units"]
+
+
+def _king_grid(w, h):
+    edges = []
+    for y in range(h):
+        for x in range(w):
+            for dx, dy in ((1, 0), (-1, 1), (0, 1), (1, 1)):
+                if 0 <= x + dx < w and y + dy < h:
+                    edges.append((y * w + x, (y + dy) * w + x + dx))
+    return gb.CsrGraph.from_edge_list(w * h, edges).to_undirected()
+
+
+def test_lattice_kernel_matches_oracle(ctx, road):
+    """The lattice kernel (one source per wave, the graph's adjacency in LDS
+    as 2 or 4 bits per vertex shared by the workgroup's waves) against the
+    oracle on every row: road-like graphs (offsets 1 and the row length,
+    missing grid edges), a 4-neighbour grid whose row ends have no +1 edge,
+    an 8-neighbour grid (four offsets: 4 bits per vertex), a path (one
+    offset), a three-offset triangular lattice; u32 stored directly (the
+    default) and through the binned log (deep_rows=1, four or more tasks per
+    CU), u16, u8, reachability, host output, duplicated sources, output
+    buffers starting with garbage, one to eight waves per workgroup, and a
+    second run on the same engine."""
+    rng = np.random.default_rng(11)
+    small = gb.generate_road_like(20_000, 160, 900, seed=3)
+    path = gb.CsrGraph.from_edge_list(3000, [(i, i + 1) for i in range(2999)]).to_undirected()
+    w = 37
+    tri = gb.CsrGraph.from_edge_list(
+        w * 40, [(u, u + d) for u in range(w * 40) for d in (1, w, w + 1)
+                 if u + d < w * 40 and (d == w or u % w != w - 1)]).to_undirected()
+    cases = [(road, [1, 339], 2048), (small, None, 1100), (small, None, 300),
+             (gb.generate_grid2d(17, 241), [1, 17], 1100), (_king_grid(45, 52), [1, 44, 45, 46], 600),
+             (path, [1], 70), (tri, [1, w, w + 1], 1500), (gb.generate_grid2d(64, 64), [1, 64], 5)]
+    for g, offs, S in cases:
+        if offs is not None:
+            assert g.lattice_offsets() == offs
+        lat = 2 if len(g.lattice_offsets()) <= 2 else 4
+        src = rng.integers(0, g.num_vertices, S).astype(np.uint32)
+        src[: min(S, 3)] = src[0]  # duplicates
+        expect = oracle(g, src)
+        dg = gb.DeviceGraph(ctx, g)
+        e = gb.GpuBfsEngine(ctx, M.WorkgroupPersistentBfs)
+        for rows, rep in ((0, 0), (0, 1), (1, 0), (1, 1)):
+            e.set_tuning(deep_bits=1, deep_lattice=1, deep_rows=rows)
+            say("lattice V=%d S=%d u32 device rows %d rep %d" % (g.num_vertices, S, rows, rep))
+            got = _device_u32(e, dg, src, 0x12345678 if rep else 0xFFFFFFFF)
+            assert np.array_equal(got, expect), (g.num_vertices, S, rows, rep,
+                                                 int(np.count_nonzero(got != expect)))
+            st = e.last_stats()
+            assert st["lattice_bits"] == lat and st["bits_per_instance"] == 1, st
+            assert st["block_threads"] % 64 == 0 and 64 <= st["block_threads"] <= 512, st
+        e.set_tuning(deep_rows=0)
+        say("lattice V=%d S=%d host u32 / u16 / reachability" % (g.num_vertices, S))
+        assert np.array_equal(e.run_ms_bfs(dg, src), expect)
+        assert np.array_equal(e.run_ms_bfs(dg, src, np.uint16), to_u16_expect(expect))
+        bm = e.run_ms_reachability(dg, src)
+        assert np.array_equal(bm.words, orc.reachability_from_dist(g.num_vertices, S, expect))
+        if expect[expect != INF32].max(initial=0) < 255:
+            say("lattice V=%d S=%d u8" % (g.num_vertices, S))
+            got8 = e.run_ms_bfs(dg, src, np.uint8)
+            assert np.array_equal(got8, np.where(expect == INF32, 0xFF, expect).astype(np.uint8))
 
 
 def test_staged_rows_match_oracle(ctx):

@@ -227,6 +227,34 @@ def test_context_without_device_raises():
     assert out.stdout.strip() in ("device", "nodevice"), out.stderr
 
 
+def test_lattice_offsets():
+    """The deep family keeps a lattice graph's adjacency in LDS: a graph
+    qualifies when it is symmetric and every edge joins ids that differ by
+    one of at most four offsets.  Grids have offsets {1, width} by
+    construction, the 8-neighbour grid {1, w-1, w, w+1}, a path {1}; five
+    offsets, a directed graph, an edgeless graph and R-MAT do not qualify."""
+    assert gb.generate_grid2d(5, 7).lattice_offsets() == [1, 5]
+    assert gb.generate_grid2d(256, 256).lattice_offsets() == [1, 256]
+    assert gb.generate_road_like().lattice_offsets() == [1, 339]
+    assert gb.CsrGraph.from_edge_list(10, [(i, i + 1) for i in range(9)]).to_undirected() \
+        .lattice_offsets() == [1]
+    w, h = 9, 6
+    king = []
+    for y in range(h):
+        for x in range(w):
+            for dx, dy in ((1, 0), (-1, 1), (0, 1), (1, 1)):
+                if 0 <= x + dx < w and y + dy < h:
+                    king.append((y * w + x, (y + dy) * w + x + dx))
+    kg = gb.CsrGraph.from_edge_list(w * h, king).to_undirected()
+    assert kg.lattice_offsets() == [1, w - 1, w, w + 1]
+    five = [(0, 1), (0, 2), (0, 3), (0, 4), (0, 5)]
+    assert gb.CsrGraph.from_edge_list(6, five).to_undirected().lattice_offsets() == []
+    assert gb.CsrGraph.from_edge_list(6, five[:4]).to_undirected().lattice_offsets() == [1, 2, 3, 4]
+    assert gb.CsrGraph.from_edge_list(3, [(0, 1), (1, 2)]).lattice_offsets() == []  # directed
+    assert gb.CsrGraph.from_edge_list(4, []).lattice_offsets() == []
+    assert gb.generate_rmat(10, 8).lattice_offsets() == []
+
+
 def test_tuning_validation_rejects_bad_fields():
     """Every set_tuning entry point (engine, batch, server lane) shares
     msbfs_tuning_validate; a workgroup narrower than one wave would make the
@@ -246,7 +274,8 @@ def test_tuning_validation_rejects_bad_fields():
     for bad in (dict(deep_threads=32), dict(deep_threads=1), dict(deep_threads=96),
                 dict(deep_threads=2048), dict(deep_bits=3), dict(deep_bits=64),
                 dict(tile_words=3), dict(tile_words=128), dict(words_per_lane=3),
-                dict(direction_rule=2), dict(wide_blocks_per_cu=17)):
+                dict(direction_rule=2), dict(wide_blocks_per_cu=17), dict(deep_rows=3),
+                dict(deep_lattice=3)):
         assert rc(**bad) == _lib.ERR_INVALID_ARG, bad
         assert _lib.last_error()
     assert lib.msbfs_tuning_validate(None) == _lib.ERR_INVALID_ARG

@@ -74,6 +74,10 @@ void csr_from_pairs(uint64_t V, const std::vector<uint64_t> &pairs, HostCsr &out
 void csr_transpose(uint64_t V, const uint32_t *ro, const uint32_t *ci, uint32_t *co,
                    uint32_t *ri);
 bool csr_is_symmetric(uint64_t V, uint64_t E, const uint32_t *ro, const uint32_t *ci);
+// Lattice offsets: when every edge (u, v) with v > u has v - u among at most
+// four distinct values, writes them ascending to d[0..k) (the rest 0) and
+// returns k (0 for a graph with no such edge); returns -1 otherwise.
+int csr_lattice_offsets(uint64_t V, const uint32_t *ro, const uint32_t *ci, uint32_t d[4]);
 int host_threads();
 // Hands a built CSR to the C API (msbfs_host_graph_get / _free).
 msbfs_host_graph *wrap_host_graph(HostCsr &&c);

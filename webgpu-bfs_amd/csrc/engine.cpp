@@ -174,6 +174,37 @@ extern "C" int msbfs_graph_create(msbfs_context *c, uint64_t V, uint64_t E, cons
         }
         e = upload(&g->ell, ell.data(), ell.size());
     }
+    // Lattice adjacency: when every edge of a symmetric graph joins ids that
+    // differ by one of at most four offsets (grids and grid-derived road
+    // models numbered row by row), a few bits per vertex hold the whole
+    // graph, small enough for LDS (road-like 114,599 vertices on a
+    // 339-column grid: offsets 1 and 339, 28.6 KB).  Self-loops reach
+    // nothing new and are left out.
+    if (e == hipSuccess && g->symmetric && V > 1 && E > 0) {
+        uint32_t d[4];
+        const int K = csr_lattice_offsets(V, ro, ci, d);
+        if (K > 0) {
+            const int lat = K <= 2 ? 2 : 4;
+            const uint32_t fpw = 32 / lat;
+            // padded to whole 16-byte groups: the kernel copies it to LDS in
+            // the size it reserves
+            std::vector<uint32_t> words(((V + fpw - 1) / fpw + 3) / 4 * 4, 0u);
+            for (uint64_t u = 0; u < V; ++u)
+                for (uint32_t j = ro[u]; j < ro[u + 1]; ++j) {
+                    if (ci[j] <= u) continue;
+                    const uint32_t dd = ci[j] - (uint32_t)u;
+                    int k = 0;
+                    while (d[k] != dd) ++k;
+                    words[u / fpw] |= 1u << ((u % fpw) * lat + k);
+                }
+            e = upload(&g->lat, words.data(), words.size());
+            if (e == hipSuccess) {
+                g->lat_bits = lat;
+                g->lat_words = (uint32_t)words.size();
+                std::memcpy(g->lat_d, d, sizeof(d));
+            }
+        }
+    }
     // Vertices whose edge lists the wide family hands to a whole wave.
     if (e == hipSuccess) {
         std::vector<uint32_t> hout, hin;
@@ -264,6 +295,7 @@ void msbfs::graph_release(msbfs_graph *g) {
     if (g->ro) (void)hipFree(g->ro);
     if (g->ci) (void)hipFree(g->ci);
     if (g->ell) (void)hipFree(g->ell);
+    if (g->lat) (void)hipFree(g->lat);
     if (g->heavy_in && g->heavy_in != g->heavy_out) (void)hipFree(g->heavy_in);
     if (g->heavy_out) (void)hipFree(g->heavy_out);
     if (g->hchunk_in) (void)hipFree(g->hchunk_in);
@@ -351,6 +383,7 @@ extern "C" int msbfs_tuning_validate(const msbfs_tuning *t) {
         return fail(MSBFS_ERR_INVALID_ARG, "direction_rule must be 0 or 1");
     if (t->stepped > 2) return fail(MSBFS_ERR_INVALID_ARG, "stepped must be 0, 1 or 2");
     if (t->deep_rows > 2) return fail(MSBFS_ERR_INVALID_ARG, "deep_rows must be 0, 1 or 2");
+    if (t->deep_lattice > 2) return fail(MSBFS_ERR_INVALID_ARG, "deep_lattice must be 0, 1 or 2");
     return MSBFS_OK;
 }
 
@@ -485,7 +518,7 @@ void deep_layout_v(const msbfs_context *c, const msbfs_tuning &t, uint64_t max_v
     // about 7 us per bin when the task ends, hidden behind other tasks when
     // there are many (road-like, 114,599 vertices, ms logged / direct:
     // S=2048 (8 per CU) 2.81 / 2.95, S=256 2.14 / 1.89, S=1 1.71 / 1.48).
-    const bool logged = p.threads == 64 && B == 1 && ell > 0 && max_v > 0 &&
+    const bool logged = p.threads == 64 && B == 1 && ell > 0 && max_v > 0 && t.deep_rows != 2 &&
                         max_v < (1ull << (32 - kDeepLogBits)) &&
                         ntasks >= 4 * (uint64_t)std::max(1, c->cus);
     const uint64_t nbins = logged ? (max_v + (1u << kDeepLogBits) - 1) >> kDeepLogBits : 0;
@@ -506,6 +539,52 @@ void deep_layout_v(const msbfs_context *c, const msbfs_tuning &t, uint64_t max_v
     // the one-wave kernel's 64 sink entries come out of the static reservation
     p.sink_off = (uint32_t)((p.lds + 7) / 8 * 2);
     if (p.threads == 64 && ell > 0) p.lds = p.sink_off * 4 + 64 * esz;
+}
+
+// Lattice kernel: the adjacency words first (shared by the workgroup's
+// waves), then one region per wave laid out as the one-wave kernel's LDS:
+// visited bits, two queues, the log's bin counters (logged u32 output),
+// the 64-entry append sink.  As many waves per workgroup (tasks per CU) as
+// the tasks fill, up to kLatticeMaxWaves, each wave's queues holding at
+// least 64 entries (a level that outgrows them spills to global memory).
+bool lattice_layout(const msbfs_context *c, const msbfs_graph *g, uint64_t ntasks, bool logged_ok,
+                    Plan &p) {
+    if (!g->lat || g->lat_bits == 0) return false;
+    const uint64_t V = g->V;
+    const uint64_t adj = (g->lat_words + 3ull) / 4 * 4;  // 16-byte aligned regions
+    if (adj * 4 >= c->lds_max) return false;
+    const uint64_t budget = c->lds_max / 4 - adj;  // words for the regions
+    const uint64_t vis = (V + 31) / 32;
+    const uint64_t cus = (uint64_t)std::max(1, c->cus);
+    const uint64_t want = std::min<uint64_t>(kLatticeMaxWaves, std::max<uint64_t>(1, (ntasks + cus - 1) / cus));
+    // logged rows as for the one-wave kernel (four or more tasks per CU); a
+    // task's end rewrites each bin through its visited bits and queues
+    const bool logged = logged_ok && V < (1ull << (32 - kDeepLogBits)) && ntasks >= 4 * cus;
+    for (int lg = logged ? 1 : 0; lg >= 0; --lg) {
+        const uint64_t nbins = lg ? (V + (1u << kDeepLogBits) - 1) >> kDeepLogBits : 0;
+        for (uint64_t w = want; w >= 1; --w) {
+            const uint64_t region = budget / w / 4 * 4;
+            const uint64_t fixed = vis + nbins + 64;
+            if (region < fixed + 2 * 64) continue;
+            uint64_t qcap = std::min<uint64_t>((region - fixed) / 2, std::max<uint64_t>(V, 1));
+            if (lg && vis + 2 * qcap < std::min<uint64_t>(V, 1u << kDeepLogBits)) continue;
+            p.lat = g->lat_bits;
+            p.lat_waves = (uint32_t)w;
+            p.lat_words = (uint32_t)adj;
+            p.threads = (int)(64 * w);
+            p.ell = 2 * g->lat_bits;
+            p.vis_words = (uint32_t)vis;
+            p.qcap = (uint32_t)qcap;
+            p.staged = false;
+            p.log_bw = lg ? kDeepLogBits : 0;
+            p.bin_off = (uint32_t)(vis + 2 * qcap);
+            p.sink_off = (uint32_t)(p.bin_off + nbins);
+            p.lat_region = (uint32_t)((p.sink_off + 64 + 3) / 4 * 4);
+            p.lds = (size_t)(adj + w * p.lat_region) * 4;
+            return true;
+        }
+    }
+    return false;
 }
 
 int ensure_zeroed(DevBuf &b, size_t bytes, bool force, hipStream_t st) {
@@ -533,7 +612,9 @@ size_t dist_bytes(int dtype) {
 }
 
 
-Plan plan_for(const msbfs_engine *e, const msbfs_graph *g, uint64_t S) {
+// lattice: whether the deep family may take the lattice kernel (the
+// no-allocation entry point keeps to the one-wave and workgroup kernels).
+Plan plan_for(const msbfs_engine *e, const msbfs_graph *g, uint64_t S, bool lattice = true) {
     Plan p;
     p.method = e->method;
     if (p.method == MSBFS_METHOD_AUTO) {
@@ -577,6 +658,20 @@ Plan plan_for(const msbfs_engine *e, const msbfs_graph *g, uint64_t S) {
             deep_layout_v(e->ctx, t, g->V, g->ell_width, (S + p.bits - 1) / p.bits, p,
                           g->deep_levels.load(std::memory_order_relaxed),
                           g->max_degree <= 8 && g->E <= 4 * g->V);
+            // The lattice kernel takes the place the one-wave kernel would
+            // take by default (or, when asked, of any one-source-per-task
+            // run): same tasks, the adjacency read from LDS instead of the
+            // neighbour table.
+            if (lattice && g->lat && p.bits == 1 &&
+                (t.deep_lattice == 1 || (t.deep_lattice == 0 && t.deep_threads == 0 &&
+                                         p.threads == 64 && g->ell_width > 0))) {
+                Plan q = p;
+                // direct stores by default: with no global load in the
+                // level chain, scattered stores do not hold it up, and the
+                // binned log's appends and rewrite cost more (road-like
+                // |V| = 114,599, 2,048 sources: 2.03 ms direct, 2.26 logged)
+                if (lattice_layout(e->ctx, g, S, t.deep_rows == 1, q)) p = q;
+            }
         }
     }
     return p;
@@ -809,22 +904,29 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
         const int B = plan.bits;
         const uint64_t ntasks = (S + B - 1) / B;
         const int threads = plan.threads;
+        // the lattice kernel runs lat_waves tasks per workgroup, each wave
+        // with its own spill queues and log sink
+        const uint64_t waves = plan.lat ? plan.lat_waves : 1;
         {
-            const hipError_t pe = deep_prepare(B, plan.ell, threads, plan.lds);
+            const hipError_t pe = plan.lat ? deep_lattice_prepare(plan.lat, (int)waves, plan.lds)
+                                           : deep_prepare(B, plan.ell, threads, plan.lds);
             if (pe != hipSuccess)
                 return fail(pe == hipErrorOutOfMemory ? MSBFS_ERR_OOM : MSBFS_ERR_HIP,
                             "deep kernel set-up (%d sources per task, %d threads, %zu bytes of "
                             "LDS): %s", B, plan.threads, plan.lds, hipGetErrorString(pe));
         }
         int occ = 0;
-        HIP_TRY(deep_max_blocks_per_cu(B, plan.ell, threads, plan.lds, &occ));
+        if (plan.lat)
+            HIP_TRY(deep_lattice_blocks_per_cu(plan.lat, (int)waves, plan.lds, &occ));
+        else
+            HIP_TRY(deep_max_blocks_per_cu(B, plan.ell, threads, plan.lds, &occ));
         if (occ < 1) return fail(MSBFS_ERR_HIP, "deep kernel cannot be resident (lds %zu)", plan.lds);
         const uint64_t resident = (uint64_t)c->cus * occ;
-        const int blocks = (int)std::min<uint64_t>(ntasks, resident);
+        const int blocks = (int)std::min<uint64_t>((ntasks + waves - 1) / waves, resident);
         const uint64_t cap = std::max<uint64_t>((uint64_t)B * V, (uint64_t)B);
         const uint64_t spill = cap > plan.qcap ? cap - plan.qcap : 1;
         const size_t esz = B <= 8 ? 4 : 8;
-        if ((rc = ensure_plain(e->queues, (size_t)blocks * 2 * spill * esz))) return rc;
+        if ((rc = ensure_plain(e->queues, (size_t)blocks * waves * 2 * spill * esz))) return rc;
         if ((rc = ensure_plain(e->tasks, ntasks * sizeof(DeepTask)))) return rc;
         if ((rc = ensure_plain(e->graphs, sizeof(DeepGraph)))) return rc;
         if ((rc = ensure_plain(e->work, 16))) return rc;
@@ -853,6 +955,8 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
         hg->ell = g->ell;
         hg->V = (uint32_t)V;
         hg->ell_width = (uint32_t)plan.ell;
+        hg->lat = g->lat;
+        std::memcpy(hg->lat_d, g->lat_d, sizeof(hg->lat_d));
         HIP_TRY(hipMemcpyAsync(e->tasks.p, ht, ntasks * sizeof(DeepTask), hipMemcpyHostToDevice, st));
         HIP_TRY(hipMemcpyAsync(e->graphs.p, hg, sizeof(DeepGraph), hipMemcpyHostToDevice, st));
         std::memcpy(e->tasks_key, key, sizeof(key));
@@ -875,8 +979,11 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
         p.staged = (plan.staged && (dtype == MSBFS_DIST_U32 || dtype == MSBFS_DIST_U16)) ? 1u : 0u;
         p.log_defer_after = (uint32_t)ntasks;
         uint32_t log_nbins = 0;
+        p.lat_words = plan.lat_words;
+        p.lat_region = plan.lat_region;
+        p.lat_waves = (uint32_t)waves;
         if (p.log_bw) {
-            if ((rc = ensure_plain(e->log_sink, (size_t)blocks * 64 * 4))) return rc;
+            if ((rc = ensure_plain(e->log_sink, (size_t)blocks * waves * 64 * 4))) return rc;
             p.log_sink = e->log_sink.as<uint32_t>();
             log_nbins = (uint32_t)((V + (1u << p.log_bw) - 1) >> p.log_bw);
             const uint64_t nlist = log_deferred_tasks(ntasks);
@@ -898,7 +1005,10 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
         p.ntasks = (uint32_t)ntasks;
         p.out_kind = dist_kind(dtype);
         HIP_TRY(hipEventRecord(e->ev[1], st));
-        HIP_TRY(launch_deep(B, plan.ell, p, blocks, threads, plan.lds, st));
+        if (plan.lat)
+            HIP_TRY(launch_deep_lattice(plan.lat, p, blocks, plan.lds, st));
+        else
+            HIP_TRY(launch_deep(B, plan.ell, p, blocks, threads, plan.lds, st));
         if (p.log_bw) HIP_TRY(launch_log_rewrite(p, log_nbins, st));
         if (reach)
             HIP_TRY(launch_reach_transpose(e->reach_t.as<uint32_t>(), vw_n, S, (uint32_t)V, reach,
@@ -908,6 +1018,7 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
         e->last.bits_per_instance = (uint32_t)B;
         e->last.grid_blocks = (uint32_t)blocks;
         e->last.block_threads = (uint32_t)threads;
+        e->last.lattice_bits = (uint32_t)plan.lat;
         e->last.workspace_bytes = e->queues.bytes;
     }
     HIP_TRY(hipMemcpyAsync(e->h_flags.p, e->flags.p, FlagCount * 4, hipMemcpyDeviceToHost, st));
@@ -1426,7 +1537,7 @@ extern "C" int msbfs_run_ms_bfs_device_noalloc(msbfs_engine *e, const msbfs_grap
         return fail(MSBFS_ERR_OUT_TOO_SMALL,
                     "no-alloc call: %llu sources, at most %u fit its scratch",
                     (unsigned long long)S, (unsigned)MSBFS_NOALLOC_MAX_SOURCES);
-    const Plan plan = plan_for(e, g, S);
+    const Plan plan = plan_for(e, g, S, false);
     if (plan.family != MSBFS_FAMILY_DEEP)
         return fail(MSBFS_ERR_UNSUPPORTED,
                     "no-alloc call: this graph and source count route to the wide family, whose "

@@ -123,6 +123,12 @@ struct msbfs_graph {
     uint32_t *ro = nullptr, *ci = nullptr, *co = nullptr, *ri = nullptr;
     uint32_t *ell = nullptr;  // V x ell_width neighbour slots, 0xFFFFFFFF padded
     int ell_width = 0;        // 4 or 8 when max degree allows, else 0
+    // lattice adjacency (DeepGraph::lat): lat_bits bits per vertex (2 or 4;
+    // 0 when the graph is not a lattice), lat_words u32 words on the device
+    uint32_t *lat = nullptr;
+    int lat_bits = 0;
+    uint32_t lat_words = 0;
+    uint32_t lat_d[4] = {0, 0, 0, 0};
     uint32_t *heavy_out = nullptr, *heavy_in = nullptr;  // degree > kHeavyDegree
     uint32_t n_heavy_out = 0, n_heavy_in = 0;
     uint32_t *hchunk_out = nullptr, *hchunk_in = nullptr;  // uint4 (v, e0, e1, 0) items
@@ -196,6 +202,10 @@ struct Plan {
     uint32_t sink_off = 0; // deep one-wave kernel: LDS word offset of its append sink
     bool staged = false;   // deep: u32/u16 rows staged in LDS as level codes
     size_t lds = 0;     // deep: dynamic LDS bytes per workgroup
+    int lat = 0;           // deep lattice kernel: adjacency bits per vertex (0: not used)
+    uint32_t lat_waves = 0;   // deep lattice kernel: waves (tasks at once) per workgroup
+    uint32_t lat_region = 0;  // deep lattice kernel: LDS words per wave
+    uint32_t lat_words = 0;   // deep lattice kernel: LDS words of the adjacency
 };
 
 void context_release(msbfs_context *c);
@@ -229,5 +239,9 @@ int ensure_plain(DevBuf &b, size_t bytes);
 void deep_layout_v(const msbfs_context *c, const msbfs_tuning &t, uint64_t max_v, int ell,
                    uint64_t ntasks, Plan &p, uint32_t levels_seen, bool road_like);
 int deep_bits_for(const msbfs_context *c, const msbfs_tuning &t, uint64_t V, uint64_t S);
+// LDS split of the lattice kernel for one graph (adjacency, then one region
+// per wave); false when it does not fit.
+bool lattice_layout(const msbfs_context *c, const msbfs_graph *g, uint64_t ntasks, bool logged_ok,
+                    Plan &p);
 
 }  // namespace msbfs

@@ -97,6 +97,23 @@ void csr_transpose(uint64_t V, const uint32_t *ro, const uint32_t *ci, uint32_t 
         for (uint32_t e = ro[u]; e < ro[u + 1]; ++e) ri[pos[ci[e]]++] = (uint32_t)u;
 }
 
+int csr_lattice_offsets(uint64_t V, const uint32_t *ro, const uint32_t *ci, uint32_t d[4]) {
+    for (int k = 0; k < 4; ++k) d[k] = 0;
+    int K = 0;
+    for (uint64_t u = 0; u < V; ++u)
+        for (uint32_t j = ro[u]; j < ro[u + 1]; ++j) {
+            if (ci[j] <= u) continue;  // self-loops reach nothing; lower ids are mirrored
+            const uint32_t dd = ci[j] - (uint32_t)u;
+            int k = 0;
+            while (k < K && d[k] != dd) ++k;
+            if (k < K) continue;
+            if (K == 4) return -1;
+            d[K++] = dd;
+        }
+    std::sort(d, d + K);
+    return K;
+}
+
 bool csr_is_symmetric(uint64_t V, uint64_t E, const uint32_t *ro, const uint32_t *ci) {
     std::vector<uint32_t> co(V + 1), ri(E);
     csr_transpose(V, ro, ci, co.data(), ri.data());
@@ -209,6 +226,20 @@ int msbfs_csr_is_symmetric(uint64_t V, uint64_t E, const uint32_t *ro, const uin
     int rc = msbfs_csr_validate(V, E, ro, ci);
     if (rc) return -rc;
     return csr_is_symmetric(V, E, ro, ci) ? 1 : 0;
+}
+
+int msbfs_csr_lattice_offsets(uint64_t V, uint64_t E, const uint32_t *ro, const uint32_t *ci,
+                              uint32_t *offsets) {
+    if (!offsets) return -fail(MSBFS_ERR_INVALID_ARG, "null offsets");
+    int rc = msbfs_csr_validate(V, E, ro, ci);
+    if (rc) return -rc;
+    if (!csr_is_symmetric(V, E, ro, ci)) {
+        for (int k = 0; k < 4; ++k) offsets[k] = 0;
+        return 0;
+    }
+    const int k = csr_lattice_offsets(V, ro, ci, offsets);
+    if (k < 0) for (int i = 0; i < 4; ++i) offsets[i] = 0;
+    return k < 0 ? 0 : k;
 }
 
 // Grid2D::new (crates/core/src/generators.rs:28-80): row-major ids, each

@@ -640,9 +640,40 @@ __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
 // sequential streams that L2 merges into whole lines.  When the task ends,
 // each bin's log is scattered into LDS (sentinels first) and the span is
 // rewritten in order with coalesced stores.
-template <int B, int ELL, int DT>
-__global__ void __launch_bounds__(64) deep_wave_kernel(DeepParams p) {
+//
+// LAT > 0: the lattice kernel.  The graph's lattice adjacency (LAT bits per
+// vertex, DeepGraph::lat) is copied into LDS once per workgroup and shared
+// by its waves, each running its own tasks in its own LDS region, so a
+// level's chain is LDS operations only: entry read, adjacency fields of the
+// vertex and of its lower neighbours, claims, appends.  (Road-like graph on
+// a 339-column grid, 114,599 vertices: 2 bits per vertex, 28.6 KB.)
+template <int LAT>
+__device__ __forceinline__ void lattice_neighbours(
+    const __attribute__((address_space(3))) uint32_t *adj, const uint32_t (&d)[4], uint32_t u,
+    uint32_t (&nb)[2 * LAT]) {
+    constexpr uint32_t FPW = 32 / LAT;  // vertex fields per word
+    const uint32_t fu = adj[u / FPW] >> ((u % FPW) * LAT);
+    uint32_t fm[LAT];
+#pragma unroll
+    for (int k = 0; k < LAT; ++k) {
+        // the lower neighbour's field (the vertex's own when there is none:
+        // an unused offset is 0 and its bit is never set)
+        const uint32_t w = u >= d[k] ? u - d[k] : u;
+        fm[k] = adj[w / FPW] >> ((w % FPW) * LAT);
+    }
+#pragma unroll
+    for (int k = 0; k < LAT; ++k) {
+        // a missing edge leaves the vertex itself, whose claim finds nothing
+        nb[k] = ((fu >> k) & 1u) ? u + d[k] : u;
+        nb[LAT + k] = (u >= d[k] && ((fm[k] >> k) & 1u)) ? u - d[k] : u;
+    }
+}
+
+template <int B, int ELL, int DT, int LAT = 0>
+__global__ void __launch_bounds__(LAT ? 64 * kLatticeMaxWaves : 64) deep_wave_kernel(DeepParams p) {
     static_assert(ELL == 4 || ELL == 8, "the wave kernel reads neighbour tables");
+    static_assert(LAT == 0 || (ELL == 2 * LAT && B == 1),
+                  "a lattice vertex has 2 x LAT neighbour slots; one source per task");
     static_assert(DT != 4 || B == 1, "logged output holds one source per task");
     constexpr bool LOG = DT == 4;
     constexpr bool OUT = DT != 0;
@@ -653,25 +684,37 @@ __global__ void __launch_bounds__(64) deep_wave_kernel(DeepParams p) {
     typedef __attribute__((address_space(3))) Entry *lq_t;
     typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
     constexpr uint32_t VPW = 32 / B;
-    // No static LDS: the visited bits start at address 0, so claim
-    // addresses need no base added.
+    // No static LDS: without a lattice the visited bits start at address 0,
+    // so claim addresses need no base added.  With one, the adjacency
+    // words come first and each wave's region follows.
     extern __shared__ uint32_t lds[];
-    Entry *const s_sink = (Entry *)(lds + p.lds_sink_off);  // appends a lane does not make
+    const uint32_t wave = LAT ? threadIdx.x / 64u : 0u;
+    uint32_t *const wl = LAT ? lds + p.lat_words + wave * p.lat_region : lds;
+    const uint32_t gwave = LAT ? blockIdx.x * p.lat_waves + wave : blockIdx.x;
+    Entry *const s_sink = (Entry *)(wl + p.lds_sink_off);  // appends a lane does not make
 
-    uint32_t *const vis = lds;
+    uint32_t *const vis = wl;
     __attribute__((address_space(3))) uint32_t *const lvis =
         (__attribute__((address_space(3))) uint32_t *)vis;
     const uint32_t lane = lane_id();
-    Entry *const lq0 = (Entry *)(lds + p.lds_vis_words);
+    Entry *const lq0 = (Entry *)(wl + p.lds_vis_words);
     Entry *const lq1 = lq0 + p.lds_qcap;
     const uint64_t gcap = p.cap - p.lds_qcap;
-    Entry *const gq0 = (Entry *)p.queues + (uint64_t)blockIdx.x * 2 * gcap;
+    Entry *const gq0 = (Entry *)p.queues + (uint64_t)gwave * 2 * gcap;
     Entry *const gq1 = gq0 + gcap;
     typedef __attribute__((address_space(1))) uint32_t *g32_t;
     typedef __attribute__((address_space(3))) uint32_t *l32_t;
     const uint32_t lbw = LOG ? p.log_bw : 0u, lmask = (1u << lbw) - 1u;
-    const l32_t lbin = (l32_t)(lds + p.lds_bin_off);
-    const g32_t sink = LOG ? (g32_t)p.log_sink + blockIdx.x * 64 + lane : nullptr;
+    const l32_t lbin = (l32_t)(wl + p.lds_bin_off);
+    const g32_t sink = LOG ? (g32_t)p.log_sink + gwave * 64 + lane : nullptr;
+    const __attribute__((address_space(3))) uint32_t *const ladj =
+        (const __attribute__((address_space(3))) uint32_t *)lds;
+    if constexpr (LAT > 0) {
+        // one graph per lattice launch
+        const uint32_t *const ga = p.graphs[0].lat;
+        for (uint32_t i = threadIdx.x; i < p.lat_words; i += blockDim.x) lds[i] = ga[i];
+        __syncthreads();
+    }
 
     for (;;) {
         uint32_t t = 0;
@@ -686,6 +729,10 @@ __global__ void __launch_bounds__(64) deep_wave_kernel(DeepParams p) {
         const uint32_t *const ell_s = (const uint32_t *)(
             ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(ell_a >> 32)) << 32) |
             (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)ell_a));
+        uint32_t ld[4] = {0, 0, 0, 0};  // lattice offsets (scalar registers)
+        if constexpr (LAT > 0)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) ld[k] = (uint32_t)__builtin_amdgcn_readfirstlane((int)g.lat_d[k]);
         const uint32_t nwords = (V + VPW - 1) / VPW;
         for (uint32_t i = lane; i < nwords; i += 64) vis[i] = 0;
         const uint32_t nbins = LOG ? (V + lmask) >> lbw : 0u;
@@ -706,7 +753,7 @@ __global__ void __launch_bounds__(64) deep_wave_kernel(DeepParams p) {
         // pass's row load is in flight (its address waits on an LDS atomic,
         // and a store issued late in a pass would hold up the next pass's
         // first wait on the memory counter)
-        uint32_t pend_at = 0xFFFFFFFFu, pend_val = 0;
+        uint32_t pend_at = 0xFFFFFFFFu, pend_pos = 0, pend_val = 0;
         while (n) {
             if (OUT && level > DMAX && lane == 0) atomicOr(&p.flags[FlagU16Overflow], 1u);
             const uint32_t dval = min(level, DMAX);
@@ -728,7 +775,9 @@ __global__ void __launch_bounds__(64) deep_wave_kernel(DeepParams p) {
                     }
                     constexpr bool ALL = !decltype(spill)::value;
                     uint32_t nb[ELL];
-                    if (ALL || act) {
+                    if constexpr (LAT > 0) {
+                        lattice_neighbours<LAT>(ladj, ld, u, nb);
+                    } else if (ALL || act) {
                         const __attribute__((address_space(1))) u32x4 *row =
                             (const __attribute__((address_space(1))) u32x4 *)(ell_s + (uint64_t)u * ELL);
 #pragma unroll
@@ -741,19 +790,14 @@ __global__ void __launch_bounds__(64) deep_wave_kernel(DeepParams p) {
                         }
                     }
                     if constexpr (LOG) {
-                        // append (vertex in bin, level) to the bin's log
-                        // one unconditional store per lane (lanes without an entry
-                        // write their sink word), so the wait for this pass's row
-                        // load may leave it in flight
-                        *(pend_at != 0xFFFFFFFFu ? (g32_t)out_task + pend_at : sink) = pend_val;
+                        // the previous pass's log entry: one unconditional
+                        // store per lane (lanes without an entry write their
+                        // sink word), so the wait for this pass's row load
+                        // may leave it in flight; its bin position (an LDS
+                        // atomic issued late in that pass) is added only
+                        // here, so no wait for it ends that pass
+                        *(pend_at != 0xFFFFFFFFu ? (g32_t)out_task + pend_at + pend_pos : sink) = pend_val;
                         pend_at = 0xFFFFFFFFu;
-                        if (act) {
-                            const uint32_t bin = u >> lbw;
-                            const uint32_t pos = __hip_atomic_fetch_add(
-                                lbin + bin, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                            pend_at = (bin << lbw) + pos;
-                            pend_val = (u & lmask) | (level << lbw);
-                        }
                     } else if constexpr (OUT) {
                         // the entry's lowest source bit, one straight-line
                         // store per lane (a lane without an entry repeats
@@ -813,6 +857,17 @@ __global__ void __launch_bounds__(64) deep_wave_kernel(DeepParams p) {
                         }
                     }
                     nn += tot;
+                    if constexpr (LOG) {
+                        // append (vertex in bin, level) to the bin's log: the
+                        // position is reserved after the claims and appends
+                        if (act) {
+                            const uint32_t bin = u >> lbw;
+                            pend_pos = __hip_atomic_fetch_add(lbin + bin, 1u, __ATOMIC_RELAXED,
+                                                              __HIP_MEMORY_SCOPE_WORKGROUP);
+                            pend_at = bin << lbw;
+                            pend_val = (u & lmask) | (level << lbw);
+                        }
+                    }
                     // the entry's other source bits (sources whose searches
                     // reach the vertex at the same level)
                     if constexpr (OUT && B > 1)
@@ -830,7 +885,7 @@ __global__ void __launch_bounds__(64) deep_wave_kernel(DeepParams p) {
             ++level;
         }
         if constexpr (LOG)
-            if (pend_at != 0xFFFFFFFFu) ((g32_t)out_task)[pend_at] = pend_val;
+            if (pend_at != 0xFFFFFFFFu) ((g32_t)out_task)[pend_at + pend_pos] = pend_val;
         if (lane == 0) atomicMax(&p.flags[FlagLevels], level);
         if (p.reach_t) {
             // as in the workgroup kernel: one ballot per 64 vertices
@@ -868,7 +923,7 @@ __global__ void __launch_bounds__(64) deep_wave_kernel(DeepParams p) {
             // The log entries were stored by this wave; make them visible
             // to its loads (the vector L1 is not kept coherent with them).
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
-            const l32_t chunk = (l32_t)lds;  // visited bits and queues are free now
+            const l32_t chunk = (l32_t)wl;  // visited bits and queues are free now
             // A bin holds at most 4,096 entries: a lane loads its 64 at once,
             // and the next bin's loads are issued before this bin is
             // scattered, so the rewrite costs about one memory round trip
@@ -1090,6 +1145,61 @@ hipError_t deep_prepare(int bits, int ell, int threads, size_t lds) {
 
 hipError_t deep_max_blocks_per_cu(int bits, int ell, int threads, size_t lds, int *out) {
     DEEP_SWITCH(deep_occ_t, threads, lds, out)
+}
+
+// Lattice kernel variants by output: 0 none, 1 u16, 2 u32, 3 u8, 4 logged u32.
+template <int LAT>
+static const void *deep_lattice_kernel(int dt) {
+    switch (dt) {
+    case 0: return (const void *)deep_wave_kernel<1, 2 * LAT, 0, LAT>;
+    case 1: return (const void *)deep_wave_kernel<1, 2 * LAT, 1, LAT>;
+    case 3: return (const void *)deep_wave_kernel<1, 2 * LAT, 3, LAT>;
+    case 4: return (const void *)deep_wave_kernel<1, 2 * LAT, 4, LAT>;
+    default: return (const void *)deep_wave_kernel<1, 2 * LAT, 2, LAT>;
+    }
+}
+
+static const void *deep_lattice_kernel_any(int lat, int dt) {
+    return lat == 2 ? deep_lattice_kernel<2>(dt) : lat == 4 ? deep_lattice_kernel<4>(dt) : nullptr;
+}
+
+hipError_t launch_deep_lattice(int lat, const DeepParams &p, int blocks, size_t lds,
+                               hipStream_t st) {
+    const int dt = !p.out ? 0 : p.out_kind == 1 ? 1 : p.out_kind == 2 ? 3 : p.log_bw ? 4 : 2;
+    const void *k = deep_lattice_kernel_any(lat, dt);
+    if (!k || p.lat_waves < 1 || p.lat_waves > (uint32_t)kLatticeMaxWaves)
+        return hipErrorInvalidConfiguration;
+    void *args[] = {(void *)&p};
+    return hipLaunchKernel(k, dim3(blocks), dim3(64 * p.lat_waves), args, lds, st);
+}
+
+hipError_t deep_lattice_prepare(int lat, int waves, size_t lds) {
+    (void)waves;
+    for (int dt = 0; dt <= 4; ++dt) {
+        const void *k = deep_lattice_kernel_any(lat, dt);
+        if (!k) return hipErrorInvalidValue;
+        hipFuncAttributes a;
+        hipError_t e = hipFuncGetAttributes(&a, k);
+        if (e != hipSuccess) return e;
+        if (a.sharedSizeBytes != 0) return hipErrorInvalidConfiguration;
+        e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t deep_lattice_blocks_per_cu(int lat, int waves, size_t lds, int *out) {
+    int best = 1 << 30;
+    for (int dt = 0; dt <= 4; ++dt) {
+        const void *k = deep_lattice_kernel_any(lat, dt);
+        if (!k) return hipErrorInvalidValue;
+        int o = 0;
+        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k, 64 * waves, lds);
+        if (e != hipSuccess) return e;
+        best = std::min(best, o);
+    }
+    *out = best;
+    return hipSuccess;
 }
 
 // ---------------------------------------------------------------- wide family

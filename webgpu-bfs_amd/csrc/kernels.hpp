@@ -25,6 +25,13 @@ struct DeepGraph {
     const uint32_t *ell;   // V * ell_width neighbour slots (0xFFFFFFFF = empty) or null
     uint32_t V;
     uint32_t ell_width;    // 0 (use CSR), 4 or 8
+    // Lattice adjacency (graphs whose every edge joins vertices whose ids
+    // differ by one of at most four offsets): LAT bits per vertex, bit k of
+    // u set when the edge (u, u + lat_d[k]) exists; the edge (u, u - lat_d[k])
+    // is bit k of vertex u - lat_d[k].  Unused offsets are 0.  Null when the
+    // graph is not such a lattice.
+    const uint32_t *lat;
+    uint32_t lat_d[4];
 };
 
 struct DeepParams {
@@ -60,6 +67,13 @@ struct DeepParams {
     // finished tasks, work[2] listed ones.
     uint32_t log_defer_after;
     uint32_t *log_defer;
+    // Lattice kernel (deep_wave_kernel with the adjacency in LDS): the
+    // adjacency words come first in LDS, shared by the workgroup's waves,
+    // then one region of lat_region words per wave, laid out as the one-wave
+    // kernel's LDS (the offsets above are relative to it).
+    uint32_t lat_words;
+    uint32_t lat_region;
+    uint32_t lat_waves;
 };
 // Vertex bins of the logged output: 4,096 vertices (16 KiB of u32 in LDS).
 constexpr uint32_t kDeepLogBits = 12;
@@ -162,6 +176,14 @@ hipError_t launch_log_rewrite(const DeepParams &p, uint32_t nbins, hipStream_t s
 hipError_t launch_deep(int bits, int ell, const DeepParams &p, int blocks, int threads,
                        size_t lds_bytes, hipStream_t st);
 hipError_t deep_max_blocks_per_cu(int bits, int ell, int threads, size_t lds_bytes, int *out);
+// The lattice kernel (one source per task, the graph's lattice adjacency
+// in LDS): lat = bits per vertex of the adjacency (2 or 4), a workgroup of
+// p.lat_waves waves, each running its own tasks.
+constexpr int kLatticeMaxWaves = 8;
+hipError_t launch_deep_lattice(int lat, const DeepParams &p, int blocks, size_t lds_bytes,
+                               hipStream_t st);
+hipError_t deep_lattice_prepare(int lat, int waves, size_t lds_bytes);
+hipError_t deep_lattice_blocks_per_cu(int lat, int waves, size_t lds_bytes, int *out);
 // LDS the deep kernel declares statically (task slot, level counters, claim
 // sink), reserved out of the per-workgroup budget; deep_prepare fails if the
 // compiled kernel needs more.

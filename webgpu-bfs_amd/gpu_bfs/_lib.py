@@ -105,7 +105,8 @@ class GraphInfo(ctypes.Structure):
 class Tuning(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint32) for n in (
         "alpha", "beta", "tile_words", "words_per_lane", "deep_bits", "deep_threads",
-        "wide_blocks_per_cu", "direction_rule", "trace_levels", "stepped", "deep_rows")]
+        "wide_blocks_per_cu", "direction_rule", "trace_levels", "stepped", "deep_rows",
+        "deep_lattice")]
 
 
 class SweepEntry(ctypes.Structure):
@@ -145,7 +146,7 @@ class RunStats(ctypes.Structure):
                 ("levels", ctypes.c_uint32), ("push_levels", ctypes.c_uint32),
                 ("pull_levels", ctypes.c_uint32), ("instances", ctypes.c_uint32),
                 ("bits_per_instance", ctypes.c_uint32), ("grid_blocks", ctypes.c_uint32),
-                ("block_threads", ctypes.c_uint32),
+                ("block_threads", ctypes.c_uint32), ("lattice_bits", ctypes.c_uint32),
                 ("kernel_ms", ctypes.c_float), ("total_ms", ctypes.c_float),
                 ("workspace_bytes", ctypes.c_uint64)]
 
@@ -220,6 +221,7 @@ SIGNATURES = {
     "msbfs_csr_from_undirected_pairs": (I32, [U64, U64, P, P, PP]),
     "msbfs_csr_to_csc": (I32, [U64, U64, P, P, P, P]),
     "msbfs_csr_is_symmetric": (I32, [U64, U64, P, P]),
+    "msbfs_csr_lattice_offsets": (I32, [U64, U64, P, P, P]),
     "msbfs_csr_validate": (I32, [U64, U64, P, P]),
     "msbfs_gen_grid2d": (I32, [U64, U64, PP]),
     "msbfs_gen_road_like": (I32, [U64, U64, U64, U64, PP]),

@@ -90,6 +90,19 @@ class CsrGraph:
             L.check(-r)
         return r == 1
 
+    def lattice_offsets(self) -> list:
+        """Id offsets of a lattice graph (symmetric, every edge joining ids
+        that differ by one of at most four offsets), ascending; [] for any
+        other graph.  The deep family keeps such a graph's adjacency in LDS
+        (a few bits per vertex)."""
+        d = np.zeros(4, np.uint32)
+        r = L.lib().msbfs_csr_lattice_offsets(self.num_vertices, self.num_edges,
+                                              _ptr(self.row_offsets), _ptr(self.col_indices),
+                                              _ptr(d))
+        if r < 0:
+            L.check(-r)
+        return [int(x) for x in d[:r]]
+
     def to_undirected(self) -> "CsrGraph":
         """CsrGraph::to_undirected (graph.rs:136-150)."""
         if self.is_symmetric():
