@@ -186,20 +186,25 @@ extern "C" int msbfs_graph_create(msbfs_context *c, uint64_t V, uint64_t E, cons
         if (K > 0) {
             const int lat = K <= 2 ? 2 : 4;
             const uint32_t fpw = 32 / lat;
-            // padded to whole 16-byte groups: the kernel copies it to LDS in
-            // the size it reserves
-            std::vector<uint32_t> words(((V + fpw - 1) / fpw + 3) / 4 * 4, 0u);
+            // zero fields below vertex 0 (whole words, at least the largest
+            // offset), so the kernel looks up a lower neighbour's field
+            // without a bounds test; the total padded to whole 16-byte
+            // groups: the kernel copies it to LDS in the size it reserves
+            const uint64_t pad = (d[K - 1] + fpw - 1) / fpw * fpw;
+            std::vector<uint32_t> words(((pad + V + fpw - 1) / fpw + 3) / 4 * 4, 0u);
             for (uint64_t u = 0; u < V; ++u)
                 for (uint32_t j = ro[u]; j < ro[u + 1]; ++j) {
                     if (ci[j] <= u) continue;
                     const uint32_t dd = ci[j] - (uint32_t)u;
                     int k = 0;
                     while (d[k] != dd) ++k;
-                    words[u / fpw] |= 1u << ((u % fpw) * lat + k);
+                    const uint64_t x = pad + u;
+                    words[x / fpw] |= 1u << ((x % fpw) * lat + k);
                 }
             e = upload(&g->lat, words.data(), words.size());
             if (e == hipSuccess) {
                 g->lat_bits = lat;
+                g->lat_pad = (uint32_t)pad;
                 g->lat_words = (uint32_t)words.size();
                 std::memcpy(g->lat_d, d, sizeof(d));
             }
@@ -956,6 +961,7 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
         hg->V = (uint32_t)V;
         hg->ell_width = (uint32_t)plan.ell;
         hg->lat = g->lat;
+        hg->lat_pad = g->lat_pad;
         std::memcpy(hg->lat_d, g->lat_d, sizeof(hg->lat_d));
         HIP_TRY(hipMemcpyAsync(e->tasks.p, ht, ntasks * sizeof(DeepTask), hipMemcpyHostToDevice, st));
         HIP_TRY(hipMemcpyAsync(e->graphs.p, hg, sizeof(DeepGraph), hipMemcpyHostToDevice, st));

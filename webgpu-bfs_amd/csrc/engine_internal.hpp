@@ -129,6 +129,7 @@ struct msbfs_graph {
     int lat_bits = 0;
     uint32_t lat_words = 0;
     uint32_t lat_d[4] = {0, 0, 0, 0};
+    uint32_t lat_pad = 0;     // zero fields before vertex 0's (>= the largest offset)
     uint32_t *heavy_out = nullptr, *heavy_in = nullptr;  // degree > kHeavyDegree
     uint32_t n_heavy_out = 0, n_heavy_in = 0;
     uint32_t *hchunk_out = nullptr, *hchunk_in = nullptr;  // uint4 (v, e0, e1, 0) items

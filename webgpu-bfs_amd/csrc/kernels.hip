@@ -647,25 +647,27 @@ __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
 // level's chain is LDS operations only: entry read, adjacency fields of the
 // vertex and of its lower neighbours, claims, appends.  (Road-like graph on
 // a 339-column grid, 114,599 vertices: 2 bits per vertex, 28.6 KB.)
+// adj: the fields after `pad` zero fields (pad >= every offset).
 template <int LAT>
 __device__ __forceinline__ void lattice_neighbours(
-    const __attribute__((address_space(3))) uint32_t *adj, const uint32_t (&d)[4], uint32_t u,
-    uint32_t (&nb)[2 * LAT]) {
+    const __attribute__((address_space(3))) uint32_t *adj, const uint32_t (&d)[4], uint32_t pad,
+    uint32_t u, uint32_t (&nb)[2 * LAT]) {
     constexpr uint32_t FPW = 32 / LAT;  // vertex fields per word
-    const uint32_t fu = adj[u / FPW] >> ((u % FPW) * LAT);
+    const uint32_t x = u + pad;
+    const uint32_t fu = adj[x / FPW] >> ((x % FPW) * LAT);
     uint32_t fm[LAT];
 #pragma unroll
     for (int k = 0; k < LAT; ++k) {
-        // the lower neighbour's field (the vertex's own when there is none:
-        // an unused offset is 0 and its bit is never set)
-        const uint32_t w = u >= d[k] ? u - d[k] : u;
-        fm[k] = adj[w / FPW] >> ((w % FPW) * LAT);
+        // the lower neighbour's field (a padding field below vertex 0, or
+        // the vertex's own for an unused offset 0: its bit k is never set)
+        const uint32_t xm = x - d[k];
+        fm[k] = adj[xm / FPW] >> ((xm % FPW) * LAT);
     }
 #pragma unroll
     for (int k = 0; k < LAT; ++k) {
         // a missing edge leaves the vertex itself, whose claim finds nothing
         nb[k] = ((fu >> k) & 1u) ? u + d[k] : u;
-        nb[LAT + k] = (u >= d[k] && ((fm[k] >> k) & 1u)) ? u - d[k] : u;
+        nb[LAT + k] = ((fm[k] >> k) & 1u) ? u - d[k] : u;
     }
 }
 
@@ -709,6 +711,20 @@ __global__ void __launch_bounds__(LAT ? 64 * kLatticeMaxWaves : 64) deep_wave_ke
     const g32_t sink = LOG ? (g32_t)p.log_sink + gwave * 64 + lane : nullptr;
     const __attribute__((address_space(3))) uint32_t *const ladj =
         (const __attribute__((address_space(3))) uint32_t *)lds;
+    // Queue entries: (vertex, source bits) packed; the lattice kernel runs
+    // one source per task, so its entries are bare vertex ids.
+    auto pk = [](uint32_t v, uint32_t m) -> Entry {
+        if constexpr (LAT > 0) return (Entry)v;
+        else return pack_entry(v, m, (Entry *)nullptr);
+    };
+    auto upk = [](Entry x, uint32_t &v, uint32_t &m) {
+        if constexpr (LAT > 0) {
+            v = (uint32_t)x;
+            m = 1u;
+        } else {
+            unpack_entry(x, v, m);
+        }
+    };
     if constexpr (LAT > 0) {
         // one graph per lattice launch
         const uint32_t *const ga = p.graphs[0].lat;
@@ -729,10 +745,13 @@ __global__ void __launch_bounds__(LAT ? 64 * kLatticeMaxWaves : 64) deep_wave_ke
         const uint32_t *const ell_s = (const uint32_t *)(
             ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(ell_a >> 32)) << 32) |
             (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)ell_a));
-        uint32_t ld[4] = {0, 0, 0, 0};  // lattice offsets (scalar registers)
-        if constexpr (LAT > 0)
+        uint32_t ld[4] = {0, 0, 0, 0};  // lattice offsets and padding (scalar registers)
+        uint32_t lpad = 0;
+        if constexpr (LAT > 0) {
 #pragma unroll
             for (int k = 0; k < 4; ++k) ld[k] = (uint32_t)__builtin_amdgcn_readfirstlane((int)g.lat_d[k]);
+            lpad = (uint32_t)__builtin_amdgcn_readfirstlane((int)g.lat_pad);
+        }
         const uint32_t nwords = (V + VPW - 1) / VPW;
         for (uint32_t i = lane; i < nwords; i += 64) vis[i] = 0;
         const uint32_t nbins = LOG ? (V + lmask) >> lbw : 0u;
@@ -744,7 +763,7 @@ __global__ void __launch_bounds__(LAT ? 64 * kLatticeMaxWaves : 64) deep_wave_ke
             const uint32_t s = p.sources[task.src_off + lane];
             const uint32_t bit = 1u << lane;
             atomicOr(&vis[s / VPW], bit << ((s % VPW) * B));
-            qa.put(lane, pack_entry(s, bit, (Entry *)nullptr));
+            qa.put(lane, pk(s, bit));
         }
         void *const out_task = OUT ? (void *)((char *)p.out + task.out_off * ESZ) : nullptr;
         uint32_t n = task.nsrc;  // entries of this level (wave-uniform)
@@ -764,19 +783,19 @@ __global__ void __launch_bounds__(LAT ? 64 * kLatticeMaxWaves : 64) deep_wave_ke
                 auto pass = [&](auto spill) {
                     uint32_t u = 0, f = 0;
                     if constexpr (decltype(spill)::value) {
-                        if (act) unpack_entry(qa.get(i), u, f);
+                        if (act) upk(qa.get(i), u, f);
                     } else {
                         // all lanes read (i < lcap); a lane without an entry
                         // takes vertex 0 and no source bits, so its row load
                         // is valid and it appends nothing
-                        unpack_entry(((lq_t)qa.lds)[i], u, f);
+                        upk(((lq_t)qa.lds)[i], u, f);
                         u = act ? u : 0u;
                         f = act ? f : 0u;
                     }
                     constexpr bool ALL = !decltype(spill)::value;
                     uint32_t nb[ELL];
                     if constexpr (LAT > 0) {
-                        lattice_neighbours<LAT>(ladj, ld, u, nb);
+                        lattice_neighbours<LAT>(ladj, ld, lpad, u, nb);
                     } else if (ALL || act) {
                         const __attribute__((address_space(1))) u32x4 *row =
                             (const __attribute__((address_space(1))) u32x4 *)(ell_s + (uint64_t)u * ELL);
@@ -817,6 +836,7 @@ __global__ void __launch_bounds__(LAT ? 64 * kLatticeMaxWaves : 64) deep_wave_ke
                     uint32_t newly[ELL];
                     if (act) {  // (lanes without an entry would all hit vertex 0's words)
                         uint32_t old[ELL];
+                        if constexpr (LAT > 0) f = 1u;  // one source per task
 #pragma unroll
                         for (int k = 0; k < ELL; ++k)
                             old[k] = __hip_atomic_fetch_or(lvis + nb[k] / VPW, f << ((nb[k] % VPW) * B),
@@ -843,15 +863,22 @@ __global__ void __launch_bounds__(LAT ? 64 * kLatticeMaxWaves : 64) deep_wave_ke
                         lq_t const q = (lq_t)qb.lds;
                         lq_t const sk = (lq_t)s_sink + lane;
 #pragma unroll
-                        for (int k = 0; k < ELL; ++k)
-                            *(newly[k] ? q + mbcnt64(bk[k], base_k[k]) : sk) =
-                                pack_entry(nb[k], newly[k], (Entry *)nullptr);
+                        for (int k = 0; k < ELL; ++k) {
+                            if constexpr (LAT > 0) {
+                                // straight-line: position for every lane, then a select
+                                // (a masked branch per slot costs more instructions)
+                                const uint32_t pos = mbcnt64(bk[k], base_k[k]);
+                                const uint32_t ix = newly[k] ? pos : (uint32_t)(sk - q);
+                                q[ix] = pk(nb[k], newly[k]);
+                            } else {
+                                *(newly[k] ? q + mbcnt64(bk[k], base_k[k]) : sk) = pk(nb[k], newly[k]);
+                            }
+                        }
                     } else {
 #pragma unroll
                         for (int k = 0; k < ELL; ++k) {
                             if (newly[k]) {
-                                if (!qb.put(mbcnt64(bk[k], base_k[k]),
-                                            pack_entry(nb[k], newly[k], (Entry *)nullptr)))
+                                if (!qb.put(mbcnt64(bk[k], base_k[k]), pk(nb[k], newly[k])))
                                     atomicOr(&p.flags[FlagQueueOverflow], 1u);
                             }
                         }

@@ -30,8 +30,12 @@ struct DeepGraph {
     // u set when the edge (u, u + lat_d[k]) exists; the edge (u, u - lat_d[k])
     // is bit k of vertex u - lat_d[k].  Unused offsets are 0.  Null when the
     // graph is not such a lattice.
+    // The fields are stored after lat_pad zero fields (lat_pad >= every
+    // offset), so vertex u's field is field lat_pad + u and a lower
+    // neighbour's lookup needs no bounds test.
     const uint32_t *lat;
     uint32_t lat_d[4];
+    uint32_t lat_pad;
 };
 
 struct DeepParams {
