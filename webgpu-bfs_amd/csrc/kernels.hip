@@ -817,7 +817,7 @@ __global__ void __launch_bounds__(LAT ? 64 * kLatticeMaxWaves : 64) deep_wave_ke
                         // here, so no wait for it ends that pass
                         *(pend_at != 0xFFFFFFFFu ? (g32_t)out_task + pend_at + pend_pos : sink) = pend_val;
                         pend_at = 0xFFFFFFFFu;
-                    } else if constexpr (OUT) {
+                    } else if constexpr (OUT && LAT == 0) {
                         // the entry's lowest source bit, one straight-line
                         // store per lane (a lane without an entry repeats
                         // lane 0's)
@@ -841,6 +841,16 @@ __global__ void __launch_bounds__(LAT ? 64 * kLatticeMaxWaves : 64) deep_wave_ke
                         for (int k = 0; k < ELL; ++k)
                             old[k] = __hip_atomic_fetch_or(lvis + nb[k] / VPW, f << ((nb[k] % VPW) * B),
                                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        if constexpr (LAT > 0 && OUT && !LOG) {
+                            // the lattice kernel has no global load to keep
+                            // ahead of, so its store goes in this masked block
+                            if constexpr (KIND == 1)
+                                ((__attribute__((address_space(1))) uint16_t *)out_task)[u] = (uint16_t)dval;
+                            else if constexpr (KIND == 2)
+                                ((__attribute__((address_space(1))) uint8_t *)out_task)[u] = (uint8_t)dval;
+                            else
+                                ((__attribute__((address_space(1))) uint32_t *)out_task)[u] = dval;
+                        }
 #pragma unroll
                         for (int k = 0; k < ELL; ++k) newly[k] = f & ~(old[k] >> ((nb[k] % VPW) * B));
                     } else {
