@@ -403,8 +403,11 @@ def sweep_summary(sweep):
 
 
 def bench_batch(args, steps, warmup):
-    """512 G(n,p) graphs of uneven size, each with its own source set, in one
-    persistent hipGraph-replayed launch, against one engine call per graph."""
+    """512 G(n,p) graphs of uneven size, each with its own source set, in the
+    batch mode the autotuner picks (union traversals: the graphs joined into
+    disjoint unions by source words and traversed bit-parallel; or one
+    persistent hipGraph-replayed launch over per-source tasks), against one
+    engine call per graph."""
     import numpy as np
     import torch
 
@@ -483,9 +486,14 @@ def bench_batch(args, steps, warmup):
                       "num_vertices": int(sum(g.num_vertices for g in graphs)),
                       "num_edges": int(sum(g.num_edges for g in graphs)),
                       "num_sources": int(sum(len(s) for s in sources)),
-                      "tasks": int(st["instances"]), "sources_per_task": int(st["bits_per_instance"]),
+                      "mode": {1: "union traversals", 2: "persistent deep launch"}.get(
+                          tuned["batch_union"],
+                          "persistent deep launch (hipGraph) beside union traversals of the graphs "
+                          f"of >= {tuned['union_words'] or 2} source words"),
+                      "instances": int(st["instances"]),
+                      "source_bits_per_instance": int(st["bits_per_instance"]),
                       "threads": int(st["block_threads"]), "grid_blocks": int(st["grid_blocks"]),
-                      "hipgraph": True}}
+                      "hipgraph": tuned["batch_union"] != 1}}
     del out, plain
     torch.cuda.empty_cache()
     return res

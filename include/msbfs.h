@@ -152,6 +152,19 @@ typedef struct msbfs_tuning {
                                 (2 or 4 bits per vertex) shared by a workgroup's waves, one task per
                                 wave.  0 auto: in place of the one-wave kernel; 1 for every
                                 one-source-per-task run on such a graph; 2 never */
+    uint32_t batch_union;    /* batches: union traversals -- graphs grouped by source words per
+                                vertex (the next power of two of ceil(S/64)), each group's graphs
+                                joined into one disjoint union whose sources share the source bits
+                                (bit i is source i of every graph: bits never cross graphs), each
+                                union traversed by the wide family, bit-parallel -- against one
+                                persistent deep-family launch over per-source tasks.  1 every graph
+                                in a union; 2 every graph in the deep launch; 0 graphs of at least
+                                union_words source words in unions, the rest in the deep launch */
+    uint32_t union_words;    /* with batch_union 0: the fewest source words per vertex (1, 2, 4, ...)
+                                that puts a graph into a union; 0 auto: 2 (512 G(n,p) graphs of
+                                1K-64K vertices, ms per traversal of the graphs of 1, 2 and 4 words:
+                                unions 5.9, 1.8 and 2.0 against about 2.5, 2.1 and 3.6 in the deep
+                                launch) */
 } msbfs_tuning;
 
 /* One BFS level of the last wide-family run (recorded when trace_levels > 0). */
@@ -327,6 +340,8 @@ int msbfs_batch_replay(msbfs_batch *b, void *stream, int sync);
 int msbfs_batch_check(msbfs_batch *b);
 int msbfs_batch_stats(msbfs_batch *b, msbfs_run_stats *st);
 int msbfs_batch_set_tuning(msbfs_batch *b, const msbfs_tuning *t);
+/* The batch's current tuning (after msbfs_batch_autotune: the winner's). */
+int msbfs_batch_get_tuning(const msbfs_batch *b, msbfs_tuning *t);
 /* Family of each graph of the batch (msbfs_family), num_graphs entries. */
 int msbfs_batch_families(const msbfs_batch *b, int32_t *families);
 /* Autotuner: times on the batch itself every feasible deep_bits x

@@ -112,6 +112,15 @@ def test_batch512_full_workload(ctx):
         eng.run_ms_bfs_device(dg, sources[gi], plain.data_ptr() + 4 * int(b.offsets[gi]),
                               len(sources[gi]) * graphs[gi].num_vertices)
     assert torch.equal(plain, out)
+    # both batch modes: the default union traversals checked above, and the
+    # persistent deep launch over per-source tasks
+    mode = b.get_tuning()["batch_union"]
+    b.set_tuning(batch_union=1 if mode == 2 else 2)
+    say("batch512 other mode (batch_union=%d)" % b.get_tuning()["batch_union"])
+    out.fill_(-3)
+    torch.cuda.synchronize()
+    b.run(out.data_ptr(), b.total)
+    assert torch.equal(plain, out)
 
 
 def test_serve_workload_road_and_rmat22(ctx):

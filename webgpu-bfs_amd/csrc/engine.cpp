@@ -123,6 +123,13 @@ extern "C" int msbfs_context_synchronize(msbfs_context *c) {
 extern "C" int msbfs_graph_create(msbfs_context *c, uint64_t V, uint64_t E, const uint32_t *ro,
                                   const uint32_t *ci, msbfs_graph **out) {
     clear_error();
+    return graph_create_impl(c, V, E, ro, ci, nullptr, out);
+}
+
+// vbound (batch union graphs, else null): per vertex, the source count of
+// its graph, kept in the pull's light and heavy in-list items.
+int msbfs::graph_create_impl(msbfs_context *c, uint64_t V, uint64_t E, const uint32_t *ro,
+                             const uint32_t *ci, const uint32_t *vbound, msbfs_graph **out) {
     if (!c || !out) return fail(MSBFS_ERR_INVALID_ARG, "null argument");
     if (V > 0xFFFFFFFEull) return fail(MSBFS_ERR_INVALID_GRAPH, "num_vertices exceeds u32");
     if (E > 0xFFFFFFFFull) return fail(MSBFS_ERR_INVALID_GRAPH, "num_edges exceeds u32");
@@ -216,14 +223,14 @@ extern "C" int msbfs_graph_create(msbfs_context *c, uint64_t V, uint64_t E, cons
         for (uint64_t v = 0; v < V; ++v)
             if (ro[v + 1] - ro[v] > kHeavyDegree) hout.push_back((uint32_t)v);
         // Heavy edge lists split into work items of at most `len` edges.
-        auto chunks = [](const std::vector<uint32_t> &hv, const uint32_t *off, uint32_t len) {
+        auto chunks = [vbound](const std::vector<uint32_t> &hv, const uint32_t *off, uint32_t len) {
             std::vector<uint32_t> out;
             for (uint32_t v : hv)
                 for (uint32_t b = off[v]; b < off[v + 1]; b += len) {
                     out.push_back(v);
                     out.push_back(b);
                     out.push_back(std::min(off[v + 1], b + len));
-                    out.push_back(0);
+                    out.push_back(vbound ? vbound[v] : 0u);
                 }
             return out;
         };
@@ -259,7 +266,7 @@ extern "C" int msbfs_graph_create(msbfs_context *c, uint64_t V, uint64_t E, cons
                 lin.push_back((uint32_t)v);
                 lin.push_back(cv[v]);
                 lin.push_back(cv[v + 1]);
-                lin.push_back(0);
+                lin.push_back(vbound ? vbound[v] : 0u);
             }
             g->n_light_in = (uint32_t)(lin.size() / 4);
             e = upload(&g->light_in, lin.data(), lin.size());
@@ -389,6 +396,9 @@ extern "C" int msbfs_tuning_validate(const msbfs_tuning *t) {
     if (t->stepped > 2) return fail(MSBFS_ERR_INVALID_ARG, "stepped must be 0, 1 or 2");
     if (t->deep_rows > 2) return fail(MSBFS_ERR_INVALID_ARG, "deep_rows must be 0, 1 or 2");
     if (t->deep_lattice > 2) return fail(MSBFS_ERR_INVALID_ARG, "deep_lattice must be 0, 1 or 2");
+    if (t->batch_union > 2) return fail(MSBFS_ERR_INVALID_ARG, "batch_union must be 0, 1 or 2");
+    if (t->union_words && (t->union_words > 64 || (t->union_words & (t->union_words - 1))))
+        return fail(MSBFS_ERR_INVALID_ARG, "union_words must be a power of two <= 64");
     return MSBFS_OK;
 }
 
@@ -713,16 +723,26 @@ int check_sources(const msbfs_graph *g, const uint32_t *sources, uint64_t S) {
 
 // Enqueues one traversal.  `out` (device) may be null when only the
 // reachability words are wanted.
+// un: a batch union traversal (seed pairs and output map; wide family,
+// no reachability), else null.
 int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, uint64_t S,
                 int dtype, void *out, uint32_t *reach, uint32_t reach_wpv, hipStream_t st,
-                const Plan &plan) {
+                const Plan &plan, const UnionSeeds *un = nullptr) {
     msbfs_context *c = e->ctx;
     int rc;
     const uint64_t V = g->V;
-    // sources: host -> pinned -> device
-    if ((rc = e->h_sources.ensure(S * 4)) < 0) return -rc;
-    std::memcpy(e->h_sources.p, sources, S * 4);
-    if ((rc = ensure_plain(e->sources, S * 4))) return rc;
+    if (un && (plan.family != MSBFS_FAMILY_WIDE || reach))
+        return fail(MSBFS_ERR_INVALID_ARG, "union traversal: wide family and distances only");
+    // sources (or seed vertices and bits): host -> pinned -> device
+    const uint64_t nsrc_words = un ? 2 * un->nseeds : S;
+    if ((rc = e->h_sources.ensure(nsrc_words * 4)) < 0) return -rc;
+    if (un) {
+        std::memcpy(e->h_sources.p, un->seed_v, un->nseeds * 4);
+        std::memcpy((uint32_t *)e->h_sources.p + un->nseeds, un->seed_b, un->nseeds * 4);
+    } else {
+        std::memcpy(e->h_sources.p, sources, S * 4);
+    }
+    if ((rc = ensure_plain(e->sources, nsrc_words * 4))) return rc;
     if ((rc = ensure_plain(e->flags, FlagCount * 4))) return rc;
     if ((rc = e->h_flags.ensure(FlagCount * 4)) < 0) return -rc;
     // the cooperative wide kernel needs the device to itself; stepped wide
@@ -732,7 +752,7 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
     LaunchOrder order(c, st, coop);
     if ((rc = order.begin())) return rc;
     HIP_TRY(hipEventRecord(e->ev[0], st));
-    HIP_TRY(hipMemcpyAsync(e->sources.p, e->h_sources.p, S * 4, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(e->sources.p, e->h_sources.p, nsrc_words * 4, hipMemcpyHostToDevice, st));
     HIP_TRY(launch_zero_u32(e->flags.as<uint32_t>(), FlagCount, st));
 
     e->last = msbfs_run_stats{};
@@ -827,6 +847,13 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
         p.n_hchunk_in = g->n_hchunk_in;
         p.light_in = (const uint4 *)g->light_in;
         p.n_light_in = g->n_light_in;
+        if (un) {
+            p.seed_v = e->sources.as<uint32_t>();
+            p.seed_b = e->sources.as<uint32_t>() + un->nseeds;
+            p.nseeds = (uint32_t)un->nseeds;
+            p.bmap = un->bmap;
+            p.vgraph = un->vgraph;
+        }
         e->tick_base += (uint32_t)(V + 2);
 
         // Tile width: lanes per vertex.  With an even word count of 16 or
@@ -1205,6 +1232,21 @@ int run_common(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, u
 }
 
 }  // namespace
+
+int msbfs::engine_run_union(msbfs_engine *e, const msbfs_graph *g, const UnionSeeds &u, uint64_t S,
+                            int dtype, void *out, hipStream_t st) {
+    if (!e || !g || S == 0 || u.nseeds == 0) return fail(MSBFS_ERR_INVALID_ARG, "empty union run");
+    std::lock_guard<std::mutex> lk(e->mu);
+    const Plan plan = plan_for(e, g, S);
+    int rc = finish_run(e);
+    if (rc) return rc;
+    rc = enqueue_run(e, g, nullptr, S, dtype, out, nullptr, 0, st, plan, &u);
+    if (rc) {
+        e->dirty = true;
+        e->pending = false;
+    }
+    return rc;
+}
 
 extern "C" int msbfs_engine_resolve(const msbfs_engine *e, const msbfs_graph *g, uint64_t S,
                                     int *method_out) {

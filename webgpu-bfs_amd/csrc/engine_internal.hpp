@@ -229,6 +229,21 @@ class LaunchOrder {
     std::unique_lock<std::mutex> lk_;
 };
 void graph_release(msbfs_graph *g);
+int graph_create_impl(msbfs_context *c, uint64_t V, uint64_t E, const uint32_t *ro,
+                      const uint32_t *ci, const uint32_t *vbound, msbfs_graph **out);
+// A batch union traversal (msbfs_batch): seeds as (vertex, source bit)
+// pairs, and the union's per-graph output map (device arrays).
+struct UnionSeeds {
+    uint64_t nseeds = 0;
+    const uint32_t *seed_v = nullptr, *seed_b = nullptr;  // host
+    const BatchGraphMap *bmap = nullptr;                   // device
+    const uint32_t *vgraph = nullptr;                      // device
+};
+// Enqueues one wide traversal of a union graph on `st` (not waited for:
+// msbfs_engine_check or the next call finishes it).  S: the largest source
+// count of the union's graphs.
+int engine_run_union(msbfs_engine *e, const msbfs_graph *g, const UnionSeeds &u, uint64_t S,
+                     int dtype, void *out, hipStream_t st);
 void engine_release(msbfs_engine *e);
 
 int ensure_zeroed(DevBuf &b, size_t bytes, bool force, hipStream_t st);

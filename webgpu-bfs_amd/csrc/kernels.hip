@@ -1304,14 +1304,18 @@ __device__ __forceinline__ WT wconst(u64 v) {
     wfill(a, v);
     return a;
 }
-// Valid and active source bits of the lane's words.
-__device__ __forceinline__ u64 words_live(const WideParams &p, const u64 *act, uint32_t w, u64) {
-    return word_valid(p.S, w) & act[w];
+// Valid and active source bits of the lane's words (sb: the vertex's source
+// count in a batch union traversal, 0 for all S sources).
+__device__ __forceinline__ u64 words_live(const WideParams &p, const u64 *act, uint32_t w, u64,
+                                          uint32_t sb = 0) {
+    return word_valid(sb ? sb : p.S, w) & act[w];
 }
-__device__ __forceinline__ u64x2 words_live(const WideParams &p, const u64 *act, uint32_t w, u64x2) {
+__device__ __forceinline__ u64x2 words_live(const WideParams &p, const u64 *act, uint32_t w, u64x2,
+                                            uint32_t sb = 0) {
+    const uint64_t lim = sb ? sb : p.S;
     u64x2 r;
-    r.x = word_valid(p.S, w) & act[w];
-    r.y = word_valid(p.S, w + 1) & act[w + 1];
+    r.x = word_valid(lim, w) & act[w];
+    r.y = word_valid(lim, w + 1) & act[w + 1];
     return r;
 }
 // Valid source bits of the lane's words.
@@ -1338,28 +1342,39 @@ __device__ __forceinline__ void act_or(u64 *act_lds, uint32_t w, u64x2 n) {
 
 // Per-lane distance stores for scattered vertices (push filter, heavy
 // vertices): one store per set bit.
+// Output element of (source s, vertex v): row s of the source-major output,
+// or, in a batch union traversal, row s of v's own graph.
+__device__ __forceinline__ uint64_t dist_index(const WideParams &p, uint64_t s, uint32_t v) {
+    if (!p.bmap) return s * p.V + v;
+    const BatchGraphMap g = p.bmap[p.vgraph[v]];
+    return g.out_off + s * g.V + (v - g.base);
+}
+
 __device__ __forceinline__ void wide_write_lane(const WideParams &p, uint32_t w, uint32_t v,
                                                 u64 n, uint32_t d) {
     const uint32_t u16 = p.out_kind;
     while (n) {
         const uint32_t b = (uint32_t)__builtin_ctzll(n);
         n &= n - 1;
-        store_dist(p.out, u16, ((uint64_t)w * 64 + b) * p.V + v, d, p.flags);
+        store_dist(p.out, u16, dist_index(p, (uint64_t)w * 64 + b, v), d, p.flags);
     }
 }
 
+// Seed i: vertex sources[i] with source bit i, or, in a batch union
+// traversal, vertex seed_v[i] with bit seed_b[i].
 __global__ void wide_seed_kernel(WideParams p) {
     const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    if (i >= p.S) return;
-    const uint32_t s = p.sources[i];
-    const uint32_t w = (uint32_t)(i / 64);
-    const u64 bit = 1ull << (i % 64);
+    if (i >= (p.seed_v ? (uint64_t)p.nseeds : p.S)) return;
+    const uint32_t s = p.seed_v ? p.seed_v[i] : p.sources[i];
+    const uint64_t sb = p.seed_v ? (uint64_t)p.seed_b[i] : i;
+    const uint32_t w = (uint32_t)(sb / 64);
+    const u64 bit = 1ull << (sb % 64);
     const uint64_t idx = (uint64_t)s * p.W + w;
     atomicOr(&p.seen[idx], bit);
     atomicOr(&p.fr[idx], bit);  // slot of distance 0
     atomicOr(&p.fbA[s >> 5], 1u << (s & 31));
     if (p.defer) atomicOr(&p.lv[s], 1u);  // in the frontier of level 0
-    if (p.out && p.defer == 0) store_dist(p.out, p.out_kind, i * p.V + s, 0, p.flags);
+    if (p.out && p.defer == 0) store_dist(p.out, p.out_kind, dist_index(p, sb, s), 0, p.flags);
     if (i < p.W) {  // active sources: all of them at level 0
         p.act[i] = word_valid(p.S, (uint32_t)i);
         p.act[p.W + i] = 0;
@@ -1373,10 +1388,12 @@ __global__ void wide_seed_kernel(WideParams p) {
 // The level-0 slot rows of the sources' vertices, cleared before the seed
 // kernel ORs their bits in (frontier slots are not cleared after use).
 __global__ void wide_seed_rows_kernel(WideParams p) {
-    const uint64_t n = p.S * p.W;
+    const uint64_t n = (p.seed_v ? (uint64_t)p.nseeds : p.S) * p.W;
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
-         i += (uint64_t)gridDim.x * blockDim.x)
-        p.fr[(uint64_t)p.sources[i / p.W] * p.W + i % p.W] = 0;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t v = p.seed_v ? p.seed_v[i / p.W] : p.sources[i / p.W];
+        p.fr[(uint64_t)v * p.W + i % p.W] = 0;
+    }
 }
 
 // Source-major reachability rows (reach_vw words per source) to the
@@ -1416,9 +1433,10 @@ hipError_t launch_reach_transpose(const uint32_t *rt, uint32_t vw_n, uint64_t S,
 }
 
 hipError_t launch_wide_seed(const WideParams &p, hipStream_t st) {
-    const uint64_t rblocks = std::min<uint64_t>((p.S * p.W + 255) / 256, 4096);
+    const uint64_t nseed = p.seed_v ? (uint64_t)p.nseeds : p.S;
+    const uint64_t rblocks = std::max<uint64_t>(1, std::min<uint64_t>((nseed * p.W + 255) / 256, 4096));
     hipLaunchKernelGGL(wide_seed_rows_kernel, dim3((unsigned)rblocks), dim3(256), 0, st, p);
-    const uint64_t blocks = (p.S + 255) / 256;
+    const uint64_t blocks = std::max<uint64_t>(1, (nseed + 255) / 256);
     hipLaunchKernelGGL(wide_seed_kernel, dim3((unsigned)blocks), dim3(256), 0, st, p);
     return hipGetLastError();
 }
@@ -1730,11 +1748,13 @@ __device__ __forceinline__ WT scan_gather(const WideParams &p, const WaveCtx &x,
 // words written to the slot, zeros included (slots are not cleared between
 // uses).  Returns whether the vertex got any new bit (the same on all its
 // lanes).
+// sb: the vertex's source count in a batch union traversal (bits at or past
+// it are never needed), 0 for all S sources.
 template <int TW, typename WT = u64>
 __device__ __forceinline__ bool pull_vertex(const WideParams &p, const WaveCtx &x, const u64 *cur,
                                             const uint32_t *fb_cur, u64 *nxt, uint32_t v,
                                             bool okv, uint32_t e0, uint32_t e1, u64 *lds_row,
-                                            u64 &my_bits) {
+                                            u64 &my_bits, uint32_t sb = 0) {
     constexpr uint32_t TV = 64 / TW;
     constexpr uint32_t K = sizeof(WT) / 8;  // source words per lane
     static_assert(K == 1 || TW >= 8, "two words per lane need tiles of 8 or more lanes");
@@ -1749,7 +1769,7 @@ __device__ __forceinline__ bool pull_vertex(const WideParams &p, const WaveCtx &
             const bool wok = okv && w < W;
             const uint64_t idx = (uint64_t)v * W + w;
             const WT s = wok ? *(const WT *)(p.seen + idx) : wconst<WT>(~0ull);
-            const WT need = wok ? ~s & words_live(p, x.act_cur, w, s) : wconst<WT>(0);
+            const WT need = wok ? ~s & words_live(p, x.act_cur, w, s, sb) : wconst<WT>(0);
             const u64 gmask = (~0ull / ((1ull << TV) - 1ull)) << x.vi;
             bool gon = (__ballot(nz(need)) & gmask) != 0 && e0 < e1;
             WT acc = wconst<WT>(0);
@@ -1775,7 +1795,7 @@ __device__ __forceinline__ bool pull_vertex(const WideParams &p, const WaveCtx &
         } else if (okv && w < W) {
             const uint64_t idx = (uint64_t)v * W + w;
             const u64 s = p.seen[idx];
-            const u64 need = ~s & word_valid(p.S, w) & x.act_cur[w];
+            const u64 need = ~s & word_valid(sb ? sb : p.S, w) & x.act_cur[w];
             if (need) {
                 // Parents in steps of 8 with no branches inside a
                 // step: indices past the end repeat the last parent,
@@ -1904,17 +1924,18 @@ __device__ __forceinline__ void wide_pull(const WideParams &p, const WaveCtx &x,
             const bool okv = i < nl;
             uint4 it = make_uint4(0u, 0u, 0u, 0u);
             if (okv) it = p.light_in[i];
-            const uint32_t v = it.x, e0 = it.y, e1 = it.z;
+            const uint32_t v = it.x, e0 = it.y, e1 = it.z, sb = it.w;
             bool vany;
             if constexpr (TW >= 8) {
                 if (p.pair)  // two adjacent words per lane: TV vertices over 2*TW words
                     vany = pull_vertex<TW, u64x2>(p, x, cur, fb_cur, nxt, v, okv, e0, e1,
-                                                  nullptr, my_bits);
+                                                  nullptr, my_bits, sb);
                 else
                     vany = pull_vertex<TW>(p, x, cur, fb_cur, nxt, v, okv, e0, e1, nullptr,
-                                           my_bits);
+                                           my_bits, sb);
             } else {
-                vany = pull_vertex<TW>(p, x, cur, fb_cur, nxt, v, okv, e0, e1, nullptr, my_bits);
+                vany = pull_vertex<TW>(p, x, cur, fb_cur, nxt, v, okv, e0, e1, nullptr, my_bits,
+                                       sb);
             }
             const bool app = x.wi == 0 && okv && vany;
             qa.push(app, v, &c->next_count, qnext);
@@ -1980,8 +2001,8 @@ __device__ __forceinline__ void wide_pull(const WideParams &p, const WaveCtx &x,
                     const uint32_t b = (uint32_t)__builtin_ctzll(un);
                     un &= un - 1;
                     if ((n >> b) & 1ull)
-                        store_dist<true>(p.out, u16, ((uint64_t)w * 64 + b) * V + v0 + x.lane, nd,
-                                         p.flags);
+                        store_dist<true>(p.out, u16, dist_index(p, (uint64_t)w * 64 + b, v0 + x.lane),
+                                         nd, p.flags);
                 }
             }
         }
@@ -2006,7 +2027,7 @@ __device__ __forceinline__ void wide_pull(const WideParams &p, const WaveCtx &x,
                 u64x2 need = wconst<u64x2>(0);
                 if (wok)
                     need = ~*(const u64x2 *)(p.seen + idx) & ~*(const u64x2 *)(p.nx + idx) &
-                           words_live(p, x.act_cur, w, need);
+                           words_live(p, x.act_cur, w, need, ch.w);
                 if (!__any(nz(need))) continue;  // wave-uniform
                 const u64x2 acc = heavy_gather<TW, u64x2>(p, x, cur, fb_cur, e0, e1, w, wok, need);
                 const u64x2 g = group_or<TV>(acc) & need;
@@ -2026,7 +2047,7 @@ __device__ __forceinline__ void wide_pull(const WideParams &p, const WaveCtx &x,
         const uint32_t w = (uint32_t)(t % WG) * TW + x.wi;
         const uint64_t idx = (uint64_t)h * W + w;
         u64 need = 0;
-        if (w < W) need = ~p.seen[idx] & word_valid(p.S, w) & ~p.nx[idx] & x.act_cur[w];
+        if (w < W) need = ~p.seen[idx] & word_valid(ch.w ? ch.w : p.S, w) & ~p.nx[idx] & x.act_cur[w];
         if (!__any(need != 0)) continue;  // wave-uniform
         // Branch-free steps as in the light pull: indices past the chunk
         // repeat its last parent, parents outside the frontier (and lanes
@@ -2206,7 +2227,7 @@ __device__ __forceinline__ void wide_pull_resolve(const WideParams &p, const Wav
                 const uint32_t b = (uint32_t)__builtin_ctzll(un);
                 un &= un - 1;
                 if ((n >> b) & 1ull)
-                    store_dist(p.out, u16, ((uint64_t)w * 64 + b) * p.V + h, nd, p.flags);
+                    store_dist(p.out, u16, dist_index(p, (uint64_t)w * 64 + b, h), nd, p.flags);
             }
         }
         qa.push(app, h, &c->next_count, qnext);
@@ -2235,12 +2256,28 @@ __device__ __forceinline__ uint32_t spread8x4(uint32_t x) {
 // and the pair is aligned, single stores otherwise).  nib[w][k][v]: nibble
 // i of byte k = distance of source bit 8k+i (15 = unreached); put[w][v]: bits
 // to store (a pair reached at a later, directly stored level is left alone).
-template <typename T, typename T2>
+template <typename T, typename T2, bool BM>
 __device__ __forceinline__ void final_tile_rows(T *out, uint64_t V, uint32_t v0, uint32_t w0,
                                                 uint32_t W, uint64_t S, const uint32_t *nib,
-                                                const u64 *put, uint32_t wave, uint32_t lane) {
+                                                const u64 *put, uint32_t wave, uint32_t lane,
+                                                const BatchGraphMap *bmap, const uint32_t *vgraph) {
     const uint32_t va = 2 * lane;
     const bool oka = v0 + va < V, okb = v0 + va + 1 < V;
+    // BM (batch union traversal): each vertex's rows are its own graph's,
+    // and only its graph's source bits exist
+    uint64_t ra = 0, rb = 0, sta = 0, stb = 0, lima = S, limb = S;
+    bool same = true;
+    if constexpr (BM) {
+        const uint32_t ga = oka ? vgraph[v0 + va] : 0u, gb = okb ? vgraph[v0 + va + 1] : ga;
+        const BatchGraphMap A = bmap[ga], B = bmap[gb];
+        ra = A.out_off + (v0 + va - A.base);
+        sta = A.V;
+        lima = A.S;
+        rb = B.out_off + (v0 + va + 1 - B.base);
+        stb = B.V;
+        limb = B.S;
+        same = ga == gb;
+    }
     for (uint32_t wl = 0; wl < kFinalWords; ++wl) {
         const uint32_t w = w0 + wl;
         if (w >= W) break;  // uniform
@@ -2250,20 +2287,27 @@ __device__ __forceinline__ void final_tile_rows(T *out, uint64_t V, uint32_t v0,
 #pragma unroll
         for (uint32_t k = 0; k < 8; ++k) {
             const uint32_t b = 8 * k + wave;
-            if ((uint64_t)w * 64 + b >= S) break;  // uniform, b grows with k
+            const uint64_t sidx = (uint64_t)w * 64 + b;
+            if (sidx >= S) break;  // uniform, b grows with k
             const uint32_t *nw = nib + (wl * 8 + k) * kFinalVerts + va;
             const uint32_t da = __builtin_amdgcn_ubfe(nw[0], 4 * wave, 4);
             const uint32_t db = __builtin_amdgcn_ubfe(nw[1], 4 * wave, 4);
-            const bool pa = (pwa >> b) & 1ull, pb = (pwb >> b) & 1ull;
+            const bool pa = ((pwa >> b) & 1ull) && sidx < lima, pb = ((pwb >> b) & 1ull) && sidx < limb;
             const T ta = da == 15u ? (T)~(T)0 : (T)da, tb = db == 15u ? (T)~(T)0 : (T)db;
-            const uint64_t row = ((uint64_t)w * 64 + b) * V + v0;
-            T *dst = out + row + va;
-            if (pa && pb && ((uintptr_t)dst & (sizeof(T2) - 1)) == 0) {
+            T *dst, *dstb;
+            if constexpr (BM) {
+                dst = out + ra + sidx * sta;
+                dstb = out + rb + sidx * stb;
+            } else {
+                dst = out + sidx * V + v0 + va;
+                dstb = dst + 1;
+            }
+            if (pa && pb && same && ((uintptr_t)dst & (sizeof(T2) - 1)) == 0) {
                 T2 two = (T2)ta | ((T2)tb << (8 * sizeof(T)));
                 __builtin_nontemporal_store(two, (T2 *)dst);
             } else {
                 if (pa) __builtin_nontemporal_store(ta, dst);
-                if (pb) __builtin_nontemporal_store(tb, dst + 1);
+                if (pb) __builtin_nontemporal_store(tb, dstb);
             }
         }
     }
@@ -2744,8 +2788,14 @@ __global__ void __launch_bounds__(kWideThreads) wide_final_kernel(WideParams p) 
         load(item + 1, lv_next);
         lv_next = lv_after;
         __syncthreads();
-        if (p.out)
-            final_tile_rows<T, T2>((T *)p.out, V, v0, w0, W, p.S, L_nib, L_put, wave, lane_id());
+        if (p.out) {
+            if (p.bmap)
+                final_tile_rows<T, T2, true>((T *)p.out, V, v0, w0, W, p.S, L_nib, L_put, wave,
+                                             lane_id(), p.bmap, p.vgraph);
+            else
+                final_tile_rows<T, T2, false>((T *)p.out, V, v0, w0, W, p.S, L_nib, L_put, wave,
+                                              lane_id(), nullptr, nullptr);
+        }
         __syncthreads();
     }
 }

@@ -91,6 +91,15 @@ struct WideCounters {
     unsigned long long deg_sum;      // out-degree sum of the next frontier
 };
 
+// Union traversal of a batch: many graphs as one disjoint union, graph g's
+// vertices the union's [base, base + V), its sources the source bits 0 ..
+// S - 1 (bits are per graph: a vertex only ever gets bits of its own graph),
+// its rows out[out_off + s * V + (v - base)].
+struct BatchGraphMap {
+    uint32_t base, V, S, pad;
+    uint64_t out_off;
+};
+
 struct WideParams {
     uint32_t V, W;          // vertices, 64-bit source words
     uint64_t S, E;
@@ -133,6 +142,15 @@ struct WideParams {
     uint32_t trace_cap;     // levels the trace buffer holds
     unsigned long long *trace;  // per level: start clock, dir<<32|frontier, mid clock, new bits
     struct WideCtl *ctl;    // stepped launches: level control (null for the cooperative kernel)
+    // Batch union traversal (null otherwise): seed i is vertex seed_v[i]
+    // with source bit seed_b[i] (else source i is sources[i] with bit i);
+    // vertex v belongs to bmap[vgraph[v]], whose rows take its distances.
+    // light_in / hchunk_in items then carry in .w the vertex's source count
+    // (its graph's S): bits at or past it are never needed (0: no bound).
+    const uint32_t *seed_v, *seed_b;
+    uint32_t nseeds;
+    const BatchGraphMap *bmap;
+    const uint32_t *vgraph;
 };
 
 // Level control of a stepped wide traversal (one launch per phase of each

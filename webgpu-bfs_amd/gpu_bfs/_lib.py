@@ -106,7 +106,7 @@ class Tuning(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint32) for n in (
         "alpha", "beta", "tile_words", "words_per_lane", "deep_bits", "deep_threads",
         "wide_blocks_per_cu", "direction_rule", "trace_levels", "stepped", "deep_rows",
-        "deep_lattice")]
+        "deep_lattice", "batch_union", "union_words")]
 
 
 class SweepEntry(ctypes.Structure):
@@ -198,6 +198,7 @@ SIGNATURES = {
     "msbfs_batch_check": (I32, [P]),
     "msbfs_batch_stats": (I32, [P, ctypes.POINTER(RunStats)]),
     "msbfs_batch_set_tuning": (I32, [P, ctypes.POINTER(Tuning)]),
+    "msbfs_batch_get_tuning": (I32, [P, ctypes.POINTER(Tuning)]),
     "msbfs_batch_autotune": (I32, [P, P, U64, ctypes.POINTER(Tuning), ctypes.POINTER(ctypes.c_float)]),
     "msbfs_batch_families": (I32, [P, P]),
     "msbfs_batch_sweep": (I32, [P, ctypes.POINTER(SweepEntry), U32, ctypes.POINTER(U32)]),

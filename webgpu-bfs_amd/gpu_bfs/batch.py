@@ -74,6 +74,11 @@ class Batch:
             setattr(t, k, int(v))
         L.check(L.lib().msbfs_batch_set_tuning(self._h, ctypes.byref(t)))
 
+    def get_tuning(self) -> dict:
+        t = L.Tuning()
+        L.check(L.lib().msbfs_batch_get_tuning(self._h, ctypes.byref(t)))
+        return {k: getattr(t, k) for k, _ in t._fields_}
+
     def autotune(self, out_ptr: int, out_len: int):
         t = L.Tuning()
         ms = ctypes.c_float()
@@ -98,9 +103,10 @@ class Batch:
         return res
 
     def families(self) -> list:
-        """Family (gpu_bfs.Family value) that traverses each graph: DEEP in
-        the persistent launch, WIDE for graphs whose visited bits exceed one
-        workgroup's LDS."""
+        """Family (gpu_bfs.Family value) that traverses each graph: WIDE for
+        the graphs in union traversals and for graphs whose visited bits
+        exceed one workgroup's LDS (in a union, or, with batch_union 2, on a
+        wide run of their own), DEEP for those of the persistent launch."""
         n = len(self.dgraphs)
         arr = (ctypes.c_int32 * max(n, 1))()
         L.check(L.lib().msbfs_batch_families(self._h, arr))
