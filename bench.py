@@ -427,8 +427,15 @@ def bench_batch(args, steps, warmup):
     b = gb.Batch(eng, dgs, sources)
     out = torch.empty(b.total, dtype=torch.int32, device="cuda")
     stream = torch.cuda.current_stream().cuda_stream
-    tuned, tuned_ms = b.autotune(out.data_ptr(), b.total)
-    log(f"batch512 autotune: {tuned} {tuned_ms:.3f} ms")
+    if args.tune:
+        # the given knobs instead of the autotuner (e.g. for a profile of the
+        # timed replays alone)
+        b.set_tuning(**{k: int(v) for k, v in (t.split("=") for t in args.tune)})
+        tuned, tuned_ms = b.get_tuning(), None
+        log(f"batch512 tuning given: {tuned}")
+    else:
+        tuned, tuned_ms = b.autotune(out.data_ptr(), b.total)
+        log(f"batch512 autotune: {tuned} {tuned_ms:.3f} ms")
     b.capture(out.data_ptr(), b.total, stream=stream)
     for _ in range(warmup):
         b.replay(stream=stream)
@@ -478,9 +485,10 @@ def bench_batch(args, steps, warmup):
            "plain_gteps": round(te / (plain_ms / 1e3) / 1e9, 3),
            "speedup_vs_plain": round(plain_ms / ms_per_step, 2),
            "identical_to_plain": same,
-           "autotune": {"chosen": {k: v for k, v in tuned.items() if v},
-                        "ms": round(tuned_ms, 4),
-                        "sweep": sweep_summary(b.sweep())},
+           "autotune": ({"chosen": {k: v for k, v in tuned.items() if v},
+                         "ms": round(tuned_ms, 4),
+                         "sweep": sweep_summary(b.sweep())} if tuned_ms is not None else
+                        {"given": {k: v for k, v in tuned.items() if v}}),
            "wide_graphs": int(sum(1 for f in b.families() if f == int(gb.Family.WIDE))),
            "config": {"workload": "batch512", "graphs": len(graphs),
                       "num_vertices": int(sum(g.num_vertices for g in graphs)),

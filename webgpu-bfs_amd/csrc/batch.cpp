@@ -38,10 +38,9 @@ struct msbfs_batch {
     PinnedBuf h_flags, h_stage;
     std::vector<uint32_t> h_sources;
     // ev[0], ev[1]: a run's start and end on the caller's stream; ev[2]:
-    // the fork to the union stream (unions run beside the persistent launch:
-    // their output rows are disjoint), ev[3]: the join back
+    // the fork to the unions' streams (unions run beside the persistent
+    // launch and one another: their output rows are disjoint); ev[3] unused
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    hipStream_t ustream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     void *graph_out = nullptr;
@@ -62,6 +61,8 @@ struct msbfs_batch {
         DevBuf bmap, vgraph;                 // per graph, per union vertex
         std::vector<uint32_t> seed_v, seed_b;
         uint64_t smax = 0;                   // most sources of one graph
+        hipStream_t stream = nullptr;        // its own stream (unions overlap one another)
+        hipEvent_t done = nullptr;           // its join event
     };
     std::vector<UnionClass> uclass;
     std::vector<char> in_union;   // per graph: traversed in a union (this plan)
@@ -190,15 +191,14 @@ int union_build(msbfs_batch *b) {
             rc = hip_fail(er, "batch union: map upload");
             break;
         }
-        rc = msbfs_engine_create(c, MSBFS_METHOD_PUSH_PULL_ADAPTIVE, &u.eng);
+        if ((rc = msbfs_engine_create(c, MSBFS_METHOD_PUSH_PULL_ADAPTIVE, &u.eng))) break;
+        er = hipStreamCreateWithFlags(&u.stream, hipStreamNonBlocking);
+        if (er == hipSuccess) er = hipEventCreateWithFlags(&u.done, hipEventDisableTiming);
+        if (er != hipSuccess) rc = hip_fail(er, "batch union stream");
     }
     b->uclass = std::move(cls);
     if (rc) {
-        for (auto &u : b->uclass) {
-            if (u.eng) msbfs_engine_destroy(u.eng);
-            if (u.ug) graph_release(u.ug);
-        }
-        b->uclass.clear();
+        union_release(b);
         return rc;
     }
     union_seeds(b);
@@ -209,10 +209,17 @@ int union_build(msbfs_batch *b) {
 
 void union_release(msbfs_batch *b) {
     for (auto &u : b->uclass) {
-        if (u.eng) msbfs_engine_destroy(u.eng);
+        if (u.eng) msbfs_engine_destroy(u.eng);  // (finishes its run)
         if (u.ug) graph_release(u.ug);
+        if (u.stream) {
+            (void)hipStreamSynchronize(u.stream);
+            (void)hipStreamDestroy(u.stream);
+        }
+        if (u.done) (void)hipEventDestroy(u.done);
         u.eng = nullptr;
         u.ug = nullptr;
+        u.stream = nullptr;
+        u.done = nullptr;
     }
     b->uclass.clear();
     b->union_built = false;
@@ -405,9 +412,11 @@ int batch_enqueue_wide(msbfs_batch *b, void *out, hipStream_t st) {
     return MSBFS_OK;
 }
 
-// The union traversals, one engine run each on stream `st` (each takes the
-// context's launch ordering itself; batch_finish checks them).
+// The union traversals, one engine run each on the union's own stream,
+// forked from `st` at the run's start (ev[2]) and joined back into it (each
+// takes the context's launch ordering itself; batch_finish checks them).
 int batch_enqueue_union(msbfs_batch *b, void *out, hipStream_t st) {
+    int rc = MSBFS_OK;
     for (auto &u : b->uclass) {
         UnionSeeds us;
         us.nseeds = u.seed_v.size();
@@ -415,7 +424,12 @@ int batch_enqueue_union(msbfs_batch *b, void *out, hipStream_t st) {
         us.seed_b = u.seed_b.data();
         us.bmap = u.bmap.as<BatchGraphMap>();
         us.vgraph = u.vgraph.as<uint32_t>();
-        int rc = engine_run_union(u.eng, u.ug, us, u.smax, b->dtype, out, st);
+        hipError_t er = hipStreamWaitEvent(u.stream, b->ev[2], 0);
+        if (er != hipSuccess) return hip_fail(er, "union fork");
+        rc = engine_run_union(u.eng, u.ug, us, u.smax, b->dtype, out, u.stream);
+        er = hipEventRecord(u.done, u.stream);
+        if (er == hipSuccess) er = hipStreamWaitEvent(st, u.done, 0);
+        if (er != hipSuccess && rc == MSBFS_OK) rc = hip_fail(er, "union join");
         if (rc) return rc;
     }
     return MSBFS_OK;
@@ -463,15 +477,7 @@ int batch_finish(msbfs_batch *b) {
 int batch_enqueue_rest(msbfs_batch *b, void *out, hipStream_t st, int sync) {
     b->pending = true;  // (batch_finish collects whatever was started)
     int rc = batch_enqueue_wide(b, out, st);
-    if (rc == MSBFS_OK && b->union_mode) {
-        // the unions on their own stream from the run's start (the fork
-        // event), joined back before the end event
-        hipError_t er = hipStreamWaitEvent(b->ustream, b->ev[2], 0);
-        rc = er == hipSuccess ? batch_enqueue_union(b, out, b->ustream) : hip_fail(er, "fork");
-        er = hipEventRecord(b->ev[3], b->ustream);
-        if (er == hipSuccess) er = hipStreamWaitEvent(st, b->ev[3], 0);
-        if (er != hipSuccess && rc == MSBFS_OK) rc = hip_fail(er, "join");
-    }
+    if (rc == MSBFS_OK && b->union_mode) rc = batch_enqueue_union(b, out, st);
     if (rc) {
         (void)hipEventRecord(b->ev[1], st);
         (void)batch_finish(b);
@@ -573,7 +579,6 @@ extern "C" int msbfs_batch_create(msbfs_engine *e, uint32_t n, msbfs_graph *cons
     msbfs_context *c = e->ctx;
     hipError_t er = hipSetDevice(c->device);
     if (er == hipSuccess) er = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
-    if (er == hipSuccess) er = hipStreamCreateWithFlags(&b->ustream, hipStreamNonBlocking);
     for (int i = 0; i < 4 && er == hipSuccess; ++i) er = hipEventCreate(&b->ev[i]);
     if (er != hipSuccess) {
         msbfs_batch_destroy(b);
@@ -617,14 +622,12 @@ extern "C" int msbfs_batch_destroy(msbfs_batch *b) {
     if (!b) return MSBFS_OK;
     (void)hipSetDevice(b->eng->ctx->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
-    if (b->ustream) (void)hipStreamSynchronize(b->ustream);
     for (msbfs_engine *we : b->wide_eng) msbfs_engine_destroy(we);
     union_release(b);
     drop_graph(b);
     for (auto &ev : b->ev)
         if (ev) (void)hipEventDestroy(ev);
     if (b->stream) (void)hipStreamDestroy(b->stream);
-    if (b->ustream) (void)hipStreamDestroy(b->ustream);
     for (const msbfs_graph *g : b->graphs) graph_release(const_cast<msbfs_graph *>(g));
     msbfs_engine *e = b->eng;
     delete b;
