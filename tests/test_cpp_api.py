@@ -29,3 +29,21 @@ def test_cpp_api_host_side():
 @pytest.mark.gpu
 def test_cpp_api_on_gpu():
     run(timeout=100)
+
+
+SAN_BIN = os.path.join(HERE, "cpp", "test_sanitize")
+
+
+def test_host_code_under_sanitizers():
+    """tests/cpp/test_sanitize.cpp: the library's host sources (graph builders,
+    generators, Matrix Market parser) and the oracle built with ASan + UBSan
+    (+ LeakSanitizer) by gcc and driven over empty / ragged / truncated /
+    mutated inputs; any sanitizer report fails the run."""
+    if not os.path.exists(SAN_BIN):
+        raise AssertionError(f"{SAN_BIN} is not built: run `make -C tests/cpp`")
+    env = dict(os.environ, ASAN_OPTIONS="halt_on_error=1:detect_leaks=1",
+               UBSAN_OPTIONS="halt_on_error=1:print_stacktrace=1")
+    r = subprocess.run([SAN_BIN], capture_output=True, text=True, timeout=120, env=env)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "OK:" in r.stdout and ", 0 failed" in r.stdout

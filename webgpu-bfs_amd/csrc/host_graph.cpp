@@ -80,8 +80,9 @@ void csr_from_pairs(uint64_t V, const std::vector<uint64_t> &pairs, HostCsr &out
     for (uint64_t u = 0; u < V; ++u) out.ro[u + 1] = out.ro[u] + keep[u];
     out.ci.resize(out.ro[V]);
     parallel_for(V, threads, [&](uint64_t b, uint64_t e) {
-        for (uint64_t u = b; u < e; ++u)
-            std::memcpy(out.ci.data() + out.ro[u], tmp.data() + cnt[u], keep[u] * sizeof(uint32_t));
+        for (uint64_t u = b; u < e; ++u)  // (an empty graph's vectors hold no storage)
+            if (keep[u])
+                std::memcpy(out.ci.data() + out.ro[u], tmp.data() + cnt[u], keep[u] * sizeof(uint32_t));
     });
 }
 
