@@ -647,28 +647,49 @@ __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
 // level's chain is LDS operations only: entry read, adjacency fields of the
 // vertex and of its lower neighbours, claims, appends.  (Road-like graph on
 // a 339-column grid, 114,599 vertices: 2 bits per vertex, 28.6 KB.)
-// adj: the fields after `pad` zero fields (pad >= every offset).
+// The fields follow `pad` zero fields (pad >= every offset).  Vertex ids are
+// biased (see the kernel) and `c` maps a biased id to its field index in LDS
+// terms: field x lives in the word at LDS byte address (x / FPW) * 4, so no
+// base address is added per read.
+// Bit `off % 32` of w sign-extended: 0 or all ones (one v_bfe_i32 with the
+// offset in a register).
+__device__ __forceinline__ uint32_t bit_mask_at(uint32_t w, uint32_t off) {
+    return (uint32_t)__builtin_amdgcn_sbfe((int)w, off, 1u);
+}
+
 template <int LAT>
-__device__ __forceinline__ void lattice_neighbours(
-    const __attribute__((address_space(3))) uint32_t *adj, const uint32_t (&d)[4], uint32_t pad,
-    uint32_t u, uint32_t (&nb)[2 * LAT]) {
+__device__ __forceinline__ void lattice_neighbours(const uint32_t (&d)[4], uint32_t c, uint32_t u,
+                                                   uint32_t (&nb)[2 * LAT]) {
     constexpr uint32_t FPW = 32 / LAT;  // vertex fields per word
-    const uint32_t x = u + pad;
-    const uint32_t fu = adj[x / FPW] >> ((x % FPW) * LAT);
-    uint32_t fm[LAT];
+    typedef const __attribute__((address_space(3))) uint32_t *cl32_t;
+    // bit k of field x: bit (x % FPW) * LAT + k of its word (the v_bfe_i32
+    // offset takes the low 5 bits of x * LAT + k)
+    auto word = [](uint32_t x) -> uint32_t { return *(cl32_t)(size_t)((x / FPW) * 4u); };
+    // every word is read before any is used (the reads overlap)
+    const uint32_t x = u + c;
+    const uint32_t wu = word(x);
+    uint32_t wm[LAT];
 #pragma unroll
-    for (int k = 0; k < LAT; ++k) {
-        // the lower neighbour's field (a padding field below vertex 0, or
-        // the vertex's own for an unused offset 0: its bit k is never set)
-        const uint32_t xm = x - d[k];
-        fm[k] = adj[xm / FPW] >> ((xm % FPW) * LAT);
-    }
+    for (int k = 0; k < LAT; ++k)
+        // the lower neighbour's field: a padding field below vertex 0, or
+        // the vertex's own for an unused offset 0 (its bit k is never set)
+        wm[k] = word(x - d[k]);
+    // (keeps the scheduler from placing a use, and its wait, between the reads)
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int k = 0; k < LAT; ++k) {
         // a missing edge leaves the vertex itself, whose claim finds nothing
-        nb[k] = ((fu >> k) & 1u) ? u + d[k] : u;
-        nb[LAT + k] = ((fm[k] >> k) & 1u) ? u - d[k] : u;
+        nb[k] = u + (bit_mask_at(wu, x * LAT + k) & d[k]);
+        nb[LAT + k] = u - (bit_mask_at(wm[k], (x - d[k]) * LAT + k) & d[k]);
     }
+}
+
+// Per lane: mask bit of this lane set ? if_set : if_clear (one v_cndmask on a
+// wave mask already in scalar registers, e.g. a ballot).
+__device__ __forceinline__ uint32_t lane_select(u64 mask, uint32_t if_clear, uint32_t if_set) {
+    uint32_t r;
+    asm("v_cndmask_b32_e64 %0, %1, %2, %3" : "=v"(r) : "v"(if_clear), "v"(if_set), "s"(mask));
+    return r;
 }
 
 template <int B, int ELL, int DT, int LAT = 0>
@@ -690,7 +711,8 @@ __global__ void __launch_bounds__(LAT ? 64 * kLatticeMaxWaves : 64) deep_wave_ke
     // so claim addresses need no base added.  With one, the adjacency
     // words come first and each wave's region follows.
     extern __shared__ uint32_t lds[];
-    const uint32_t wave = LAT ? threadIdx.x / 64u : 0u;
+    // (wave-uniform: scalar registers, so the region's addresses are too)
+    const uint32_t wave = LAT ? (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64u)) : 0u;
     uint32_t *const wl = LAT ? lds + p.lat_words + wave * p.lat_region : lds;
     const uint32_t gwave = LAT ? blockIdx.x * p.lat_waves + wave : blockIdx.x;
     Entry *const s_sink = (Entry *)(wl + p.lds_sink_off);  // appends a lane does not make
@@ -709,8 +731,14 @@ __global__ void __launch_bounds__(LAT ? 64 * kLatticeMaxWaves : 64) deep_wave_ke
     const uint32_t lbw = LOG ? p.log_bw : 0u, lmask = (1u << lbw) - 1u;
     const l32_t lbin = (l32_t)(wl + p.lds_bin_off);
     const g32_t sink = LOG ? (g32_t)p.log_sink + gwave * 64 + lane : nullptr;
-    const __attribute__((address_space(3))) uint32_t *const ladj =
-        (const __attribute__((address_space(3))) uint32_t *)lds;
+    // LDS byte address of a generic pointer into LDS (its low 32 bits)
+    auto lds_addr = [](const void *ptr) -> uint32_t { return (uint32_t)(uint64_t)ptr; };
+    // Lattice kernel: queue entries hold biased vertex ids u + vbias, where
+    // vbias = 8 x the LDS byte address of the visited bits (a multiple of
+    // 32), so vertex u's visited word is at LDS byte address
+    // ((u + vbias) >> 5) * 4 and its bit is (u + vbias) & 31: a claim adds
+    // no base address.
+    const uint32_t vbias = LAT ? 8u * lds_addr(vis) : 0u;
     // Queue entries: (vertex, source bits) packed; the lattice kernel runs
     // one source per task, so its entries are bare vertex ids.
     auto pk = [](uint32_t v, uint32_t m) -> Entry {
@@ -745,12 +773,15 @@ __global__ void __launch_bounds__(LAT ? 64 * kLatticeMaxWaves : 64) deep_wave_ke
         const uint32_t *const ell_s = (const uint32_t *)(
             ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(ell_a >> 32)) << 32) |
             (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)ell_a));
-        uint32_t ld[4] = {0, 0, 0, 0};  // lattice offsets and padding (scalar registers)
-        uint32_t lpad = 0;
+        uint32_t ld[4] = {0, 0, 0, 0};  // lattice offsets (scalar registers)
+        // biased id -> LDS field index: the vertex's field follows `pad`
+        // padding fields in the adjacency words at the start of LDS
+        uint32_t lfield = 0;
         if constexpr (LAT > 0) {
 #pragma unroll
             for (int k = 0; k < 4; ++k) ld[k] = (uint32_t)__builtin_amdgcn_readfirstlane((int)g.lat_d[k]);
-            lpad = (uint32_t)__builtin_amdgcn_readfirstlane((int)g.lat_pad);
+            const uint32_t lpad = (uint32_t)__builtin_amdgcn_readfirstlane((int)g.lat_pad);
+            lfield = lpad + (32u / LAT) * (lds_addr(lds) / 4u) - vbias;
         }
         const uint32_t nwords = (V + VPW - 1) / VPW;
         for (uint32_t i = lane; i < nwords; i += 64) vis[i] = 0;
@@ -763,9 +794,11 @@ __global__ void __launch_bounds__(LAT ? 64 * kLatticeMaxWaves : 64) deep_wave_ke
             const uint32_t s = p.sources[task.src_off + lane];
             const uint32_t bit = 1u << lane;
             atomicOr(&vis[s / VPW], bit << ((s % VPW) * B));
-            qa.put(lane, pk(s, bit));
+            qa.put(lane, pk(s + vbias, bit));
         }
         void *const out_task = OUT ? (void *)((char *)p.out + task.out_off * ESZ) : nullptr;
+        // the lattice kernel's row, indexed by biased ids
+        void *const out_lat = OUT ? (void *)((char *)out_task - (uint64_t)vbias * ESZ) : nullptr;
         uint32_t n = task.nsrc;  // entries of this level (wave-uniform)
         uint32_t level = 0;
         // logged output: the previous pass's log entry, stored once the next
@@ -781,21 +814,25 @@ __global__ void __launch_bounds__(LAT ? 64 * kLatticeMaxWaves : 64) deep_wave_ke
                 const uint32_t i = base + lane;
                 const bool act = i < n;
                 auto pass = [&](auto spill) {
-                    uint32_t u = 0, f = 0;
+                    // (vertex 0 is id vbias in the lattice kernel)
+                    uint32_t u = vbias, f = 0;
                     if constexpr (decltype(spill)::value) {
                         if (act) upk(qa.get(i), u, f);
                     } else {
                         // all lanes read (i < lcap); a lane without an entry
                         // takes vertex 0 and no source bits, so its row load
                         // is valid and it appends nothing
-                        upk(((lq_t)qa.lds)[i], u, f);
-                        u = act ? u : 0u;
+                        if constexpr (LAT > 0)
+                            upk(*(lq_t)(size_t)(lds_addr(qa.lds) + (uint32_t)sizeof(Entry) * i), u, f);
+                        else
+                            upk(((lq_t)qa.lds)[i], u, f);
+                        u = act ? u : vbias;
                         f = act ? f : 0u;
                     }
                     constexpr bool ALL = !decltype(spill)::value;
                     uint32_t nb[ELL];
                     if constexpr (LAT > 0) {
-                        lattice_neighbours<LAT>(ladj, ld, lpad, u, nb);
+                        lattice_neighbours<LAT>(ld, lfield, u, nb);
                     } else if (ALL || act) {
                         const __attribute__((address_space(1))) u32x4 *row =
                             (const __attribute__((address_space(1))) u32x4 *)(ell_s + (uint64_t)u * ELL);
@@ -836,23 +873,37 @@ __global__ void __launch_bounds__(LAT ? 64 * kLatticeMaxWaves : 64) deep_wave_ke
                     uint32_t newly[ELL];
                     if (act) {  // (lanes without an entry would all hit vertex 0's words)
                         uint32_t old[ELL];
-                        if constexpr (LAT > 0) f = 1u;  // one source per task
+                        if constexpr (LAT > 0) {
+                            f = 1u;  // one source per task
+                            // biased ids: the visited word's LDS byte address
+                            // and the bit come straight from the id
 #pragma unroll
-                        for (int k = 0; k < ELL; ++k)
-                            old[k] = __hip_atomic_fetch_or(lvis + nb[k] / VPW, f << ((nb[k] % VPW) * B),
-                                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                            for (int k = 0; k < ELL; ++k)
+                                old[k] = __hip_atomic_fetch_or((l32_t)(size_t)((nb[k] >> 5) * 4u), 1u << (nb[k] & 31u),
+                                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        } else {
+#pragma unroll
+                            for (int k = 0; k < ELL; ++k)
+                                old[k] = __hip_atomic_fetch_or(lvis + nb[k] / VPW, f << ((nb[k] % VPW) * B),
+                                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        }
                         if constexpr (LAT > 0 && OUT && !LOG) {
                             // the lattice kernel has no global load to keep
                             // ahead of, so its store goes in this masked block
                             if constexpr (KIND == 1)
-                                ((__attribute__((address_space(1))) uint16_t *)out_task)[u] = (uint16_t)dval;
+                                ((__attribute__((address_space(1))) uint16_t *)out_lat)[u] = (uint16_t)dval;
                             else if constexpr (KIND == 2)
-                                ((__attribute__((address_space(1))) uint8_t *)out_task)[u] = (uint8_t)dval;
+                                ((__attribute__((address_space(1))) uint8_t *)out_lat)[u] = (uint8_t)dval;
                             else
-                                ((__attribute__((address_space(1))) uint32_t *)out_task)[u] = dval;
+                                ((__attribute__((address_space(1))) uint32_t *)out_lat)[u] = dval;
                         }
+                        if constexpr (LAT > 0) {
 #pragma unroll
-                        for (int k = 0; k < ELL; ++k) newly[k] = f & ~(old[k] >> ((nb[k] % VPW) * B));
+                            for (int k = 0; k < ELL; ++k) newly[k] = ~old[k] & (1u << (nb[k] & 31u));
+                        } else {
+#pragma unroll
+                            for (int k = 0; k < ELL; ++k) newly[k] = f & ~(old[k] >> ((nb[k] % VPW) * B));
+                        }
                     } else {
 #pragma unroll
                         for (int k = 0; k < ELL; ++k) newly[k] = 0;
@@ -870,19 +921,28 @@ __global__ void __launch_bounds__(LAT ? 64 * kLatticeMaxWaves : 64) deep_wave_ke
                         tot += (uint32_t)__popcll(bk[k]);
                     }
                     if (nn + tot <= qb.lcap) {
-                        lq_t const q = (lq_t)qb.lds;
-                        lq_t const sk = (lq_t)s_sink + lane;
+                        if constexpr (LAT > 0) {
+                            // straight-line: an LDS byte address for every lane
+                            // (the slot's run, wave-uniform, plus the lane's rank
+                            // in it), then a select on the slot's ballot: the
+                            // lane's sink word when it has no new entry (a
+                            // masked branch per slot costs more instructions)
+                            const uint32_t qaddr = lds_addr(qb.lds);
+                            const uint32_t saddr = lds_addr(s_sink) + (uint32_t)sizeof(Entry) * lane;
 #pragma unroll
-                        for (int k = 0; k < ELL; ++k) {
-                            if constexpr (LAT > 0) {
-                                // straight-line: position for every lane, then a select
-                                // (a masked branch per slot costs more instructions)
-                                const uint32_t pos = mbcnt64(bk[k], base_k[k]);
-                                const uint32_t ix = newly[k] ? pos : (uint32_t)(sk - q);
-                                q[ix] = pk(nb[k], newly[k]);
-                            } else {
-                                *(newly[k] ? q + mbcnt64(bk[k], base_k[k]) : sk) = pk(nb[k], newly[k]);
+                            for (int k = 0; k < ELL; ++k) {
+                                // (the run's address is summed in scalar registers)
+                                const uint32_t run = (uint32_t)__builtin_amdgcn_readfirstlane(
+                                    (int)(qaddr + (uint32_t)sizeof(Entry) * base_k[k]));
+                                const uint32_t a = run + (uint32_t)sizeof(Entry) * mbcnt64(bk[k], 0u);
+                                *(lq_t)(size_t)lane_select(bk[k], saddr, a) = pk(nb[k], newly[k]);
                             }
+                        } else {
+                            lq_t const q = (lq_t)qb.lds;
+                            lq_t const sk = (lq_t)s_sink + lane;
+#pragma unroll
+                            for (int k = 0; k < ELL; ++k)
+                                *(newly[k] ? q + mbcnt64(bk[k], base_k[k]) : sk) = pk(nb[k], newly[k]);
                         }
                     } else {
 #pragma unroll
@@ -898,11 +958,12 @@ __global__ void __launch_bounds__(LAT ? 64 * kLatticeMaxWaves : 64) deep_wave_ke
                         // append (vertex in bin, level) to the bin's log: the
                         // position is reserved after the claims and appends
                         if (act) {
-                            const uint32_t bin = u >> lbw;
+                            const uint32_t ur = u - vbias;  // (lattice ids are biased)
+                            const uint32_t bin = ur >> lbw;
                             pend_pos = __hip_atomic_fetch_add(lbin + bin, 1u, __ATOMIC_RELAXED,
                                                               __HIP_MEMORY_SCOPE_WORKGROUP);
                             pend_at = bin << lbw;
-                            pend_val = (u & lmask) | (level << lbw);
+                            pend_val = (ur & lmask) | (level << lbw);
                         }
                     }
                     // the entry's other source bits (sources whose searches
