@@ -48,6 +48,11 @@ void orc_pack_u8(size_t n, const uint32_t *d, uint32_t *out);
 void orc_unpack_u8(size_t n, const uint32_t *words, uint32_t *d);
 }
 
+// webgpu-bfs_amd/csrc/common.hpp (library-internal, linked from host_graph.cpp)
+namespace msbfs {
+uint32_t host_levels_from(uint64_t V, const uint32_t *ro, const uint32_t *ci, uint32_t s);
+}
+
 static const uint32_t INF = 0xFFFFFFFFu;
 static int failures = 0;
 static int checks = 0;
@@ -120,6 +125,14 @@ static std::vector<uint32_t> graph_checks(const Csr &g, uint64_t S, uint64_t see
     CHECK(orc_ms_bfs_bit_parallel(g.V, g.ro.data(), g.ci.data(), src.data(), S, bit.data()) == 0);
     CHECK(fifo == par && fifo == bit);
     for (uint64_t s = 0; s < S; ++s) CHECK(fifo[s * g.V + src[s]] == 0);
+    if (S) {
+        // levels of one search: its largest finite distance + 1
+        uint32_t far = 0;
+        for (uint64_t v = 0; v < g.V; ++v)
+            if (fifo[v] != INF) far = std::max(far, fifo[v]);
+        CHECK(msbfs::host_levels_from(g.V, g.ro.data(), g.ci.data(), src[0]) == far + 1);
+    }
+    CHECK(msbfs::host_levels_from(g.V, g.ro.data(), g.ci.data(), (uint32_t)g.V) == 0);
 
     // traversed edges and reachability words against direct counts
     uint64_t te = 0;
@@ -152,6 +165,8 @@ static void grid_checks() {
         if (w > 1 && h > 1) CHECK(k == 2 && d[0] == 1 && d[1] == w);
         if ((w == 1) != (h == 1)) CHECK(k == 1 && d[0] == 1);
         if (w == 1 && h == 1) CHECK(k == 0);
+        // from a corner: |dx| + |dy| reaches w - 1 + h - 1
+        CHECK(msbfs::host_levels_from(g.V, g.ro.data(), g.ci.data(), 0) == w + h - 1);
         std::vector<uint32_t> src;
         std::vector<uint32_t> fifo = graph_checks(g, 70, w * 131 + h, &src);
         std::vector<uint32_t> man(fifo.size());

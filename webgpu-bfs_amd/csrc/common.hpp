@@ -78,6 +78,9 @@ bool csr_is_symmetric(uint64_t V, uint64_t E, const uint32_t *ro, const uint32_t
 // four distinct values, writes them ascending to d[0..k) (the rest 0) and
 // returns k (0 for a graph with no such edge); returns -1 otherwise.
 int csr_lattice_offsets(uint64_t V, const uint32_t *ro, const uint32_t *ci, uint32_t d[4]);
+// Levels of a breadth-first search from s over the CSR (the eccentricity of
+// s in its component, plus one); 0 when V is 0.
+uint32_t host_levels_from(uint64_t V, const uint32_t *ro, const uint32_t *ci, uint32_t s);
 int host_threads();
 // Hands a built CSR to the C API (msbfs_host_graph_get / _free).
 msbfs_host_graph *wrap_host_graph(HostCsr &&c);

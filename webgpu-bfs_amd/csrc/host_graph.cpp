@@ -115,6 +115,23 @@ int csr_lattice_offsets(uint64_t V, const uint32_t *ro, const uint32_t *ci, uint
     return K;
 }
 
+uint32_t host_levels_from(uint64_t V, const uint32_t *ro, const uint32_t *ci, uint32_t s) {
+    if (V == 0 || s >= V) return 0;
+    std::vector<uint32_t> dist(V, 0xFFFFFFFFu), q;
+    q.reserve(V);
+    dist[s] = 0;
+    q.push_back(s);
+    for (size_t h = 0; h < q.size(); ++h) {
+        const uint32_t u = q[h];
+        for (uint32_t j = ro[u]; j < ro[u + 1]; ++j)
+            if (dist[ci[j]] == 0xFFFFFFFFu) {
+                dist[ci[j]] = dist[u] + 1;
+                q.push_back(ci[j]);
+            }
+    }
+    return dist[q.back()] + 1;  // (the queue ends with a farthest vertex)
+}
+
 bool csr_is_symmetric(uint64_t V, uint64_t E, const uint32_t *ro, const uint32_t *ci) {
     std::vector<uint32_t> co(V + 1), ri(E);
     csr_transpose(V, ro, ci, co.data(), ri.data());
