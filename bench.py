@@ -167,6 +167,34 @@ def calibrate_copy():
                       "launches": 3}))
 
 
+_COPY_RATE = None
+
+
+def device_copy_rate(nbytes=1 << 30, reps=5):
+    """Read + write rate of a plain device-to-device copy (torch's copy
+    kernel, 1 GiB, timed with HIP events on the current stream) on this GPU:
+    a measured practical HBM ceiling to read beside the published peak.
+    Measured once per process."""
+    global _COPY_RATE
+    if _COPY_RATE is None:
+        import torch
+
+        x = torch.ones(nbytes // 4, dtype=torch.int32, device="cuda")
+        y = torch.empty_like(x)
+        y.copy_(x)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            y.copy_(x)
+        e1.record()
+        e1.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        del x, y
+        _COPY_RATE = {"gbs": round(2 * nbytes / (ms / 1e3) / 1e9, 1), "bytes_moved": 2 * nbytes,
+                      "ms": round(ms, 4), "kernel": "torch device copy, 1 GiB read + 1 GiB written"}
+    return _COPY_RATE
+
+
 def traffic_of(config):
     """HBM bytes per launch of this workload's dominant kernels from the
     committed rocprofv3 counter summary, with where they came from."""
@@ -285,6 +313,9 @@ def measure(args, config, steps, warmup, with_cpu, with_host, with_trace=False):
                      "compulsory_bytes": compulsory, "level_model_bytes": model_bytes,
                      "kernel": kernel},
     }
+    cr = device_copy_rate()
+    res["roofline"]["device_copy"] = cr
+    res["roofline"]["frac_of_copy"] = round(achieved / cr["gbs"], 5)
     if deep and stats["levels"]:
         # the deep family is bound by its dependent levels, not by bandwidth
         res["roofline"]["latency"] = {
@@ -317,7 +348,8 @@ def measure(args, config, steps, warmup, with_cpu, with_host, with_trace=False):
         if pull_us > 0:
             res["pull_phase"] = {"model_bytes": int(pull_b), "us": round(pull_us, 1),
                                  "achieved_gbs": round(pull_b / (pull_us * 1e-6) / 1e9, 1),
-                                 "frac": round(pull_b / (pull_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)}
+                                 "frac": round(pull_b / (pull_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
+                                 "frac_of_copy": round(pull_b / (pull_us * 1e-6) / 1e9 / cr["gbs"], 4)}
 
     if with_host:
         # The reference's own call shape: run_ms_bfs_into a reusable host
