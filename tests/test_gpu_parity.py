@@ -740,3 +740,87 @@ def test_stepped_wide_matches_oracle(ctx):
             assert np.array_equal(e.run_ms_bfs(dg, src, dist_dtype=np.uint16), to_u16_expect(d32))
             bm = e.run_ms_reachability(dg, src)
             assert np.array_equal(bm.words, orc.reachability_from_dist(g.num_vertices, S, d32))
+
+
+def _brute_force(n, adj, src):
+    """Hop distances by repeated frontier expansion over the adjacency
+    matrix (independent of the oracle): row i for source src[i]."""
+    out = np.full((len(src), n), INF32, dtype=np.uint32)
+    for i, s in enumerate(src):
+        seen = np.zeros(n, dtype=bool)
+        front = np.zeros(n, dtype=bool)
+        front[s] = seen[s] = True
+        d = 0
+        while front.any():
+            out[i, front] = d
+            nxt = adj[front].any(axis=0) & ~seen
+            seen |= nxt
+            front = nxt
+            d += 1
+    return out.reshape(-1)
+
+
+def _tiny_graphs():
+    """Every undirected simple graph on 1-5 vertices and every directed one
+    on 1-3 (loops excluded), as (name, n, adjacency matrix)."""
+    out = []
+    for n in range(1, 6):
+        pairs = [(i, j) for i in range(n) for j in range(i + 1, n)]
+        for mask in range(1 << len(pairs)):
+            adj = np.zeros((n, n), dtype=bool)
+            for b, (i, j) in enumerate(pairs):
+                if (mask >> b) & 1:
+                    adj[i, j] = adj[j, i] = True
+            out.append((f"u{n}:{mask}", n, adj))
+    for n in range(1, 4):
+        arcs = [(i, j) for i in range(n) for j in range(n) if i != j]
+        for mask in range(1 << len(arcs)):
+            adj = np.zeros((n, n), dtype=bool)
+            for b, (i, j) in enumerate(arcs):
+                if (mask >> b) & 1:
+                    adj[i, j] = True
+            out.append((f"d{n}:{mask}", n, adj))
+    return out
+
+
+def test_every_tiny_graph(ctx):
+    """Brute force over all small graphs: every undirected graph on up to 5
+    vertices (1,099 graphs) and every directed graph on up to 3 (69), each
+    with every vertex as a source plus a duplicate of vertex 0, against
+    distances from repeated frontier expansion of the adjacency matrix.
+    Graphs of up to 4 vertices run every method enum; all of them run the
+    deep family, the adaptive wide family, the pull-only path, u16, u8 and
+    the reachability words, and together as one batch in its three forms
+    (union traversals, one deep launch, split).  Bar: bit-exact."""
+    import torch
+
+    graphs, sources, expect = [], [], []
+    for name, n, adj in _tiny_graphs():
+        us, vs = np.nonzero(adj)
+        g = gb.CsrGraph.from_edge_list(n, list(zip(us.tolist(), vs.tolist())))
+        src = np.array(list(range(n)) + [0], dtype=np.uint32)
+        graphs.append((name, n, g))
+        sources.append(src)
+        expect.append(_brute_force(n, adj, src))
+    engines = {m: gb.GpuBfsEngine(ctx, m) for m in M}
+    core = [M.WorkgroupPersistentBfs, M.PushPullAdaptive, M.BitParallel64Pull]
+    say("tiny graphs:", len(graphs))
+    for (name, n, g), src, d32 in zip(graphs, sources, expect):
+        for m in (list(M) if n <= 4 else core):
+            got = engines[m].run_ms_bfs(g, src)
+            assert np.array_equal(got, d32), (name, m.name, got, d32)
+        e = engines[M.Auto]
+        assert np.array_equal(e.run_ms_bfs(g, src, dist_dtype=np.uint16), to_u16_expect(d32)), name
+        assert np.array_equal(e.run_ms_bfs(g, src, dist_dtype=np.uint8), to_u8_expect(d32)), name
+        bm = e.run_ms_reachability(g, src)
+        assert np.array_equal(bm.words, orc.reachability_from_dist(n, len(src), d32)), name
+    eng = gb.GpuBfsEngine(ctx, M.Auto)
+    for mode in (dict(batch_union=1), dict(batch_union=2), dict(batch_union=0, union_words=1)):
+        say("tiny graphs batch", mode)
+        b = gb.Batch(eng, [g for _, _, g in graphs], sources)
+        b.set_tuning(**mode)
+        out = torch.full((b.total,), -7, dtype=torch.int32, device="cuda")
+        b.run(out.data_ptr(), b.total)
+        host = out.cpu().numpy().view(np.uint32)
+        for (name, n, _), got, d32 in zip(graphs, b.split(host), expect):
+            assert np.array_equal(got.reshape(-1), d32), (mode, name)
