@@ -824,3 +824,51 @@ def test_every_tiny_graph(ctx):
         host = out.cpu().numpy().view(np.uint32)
         for (name, n, _), got, d32 in zip(graphs, b.split(host), expect):
             assert np.array_equal(got.reshape(-1), d32), (mode, name)
+
+
+def test_random_lattices(ctx):
+    """Seeded sweep of random lattices through the lattice kernel: 1-4
+    random offsets (up to 700), every vertex pair at an offset joined with a
+    random probability (missing edges, several components), |V| from 2 to
+    60,000, 1-700 sources with duplicates, direct and logged rows, u16 and
+    reachability.  Bar: bit-exact against the oracle."""
+    rng = np.random.default_rng(2027)
+    lat_runs = 0
+    for case in range(36):
+        V = int(rng.integers(2, 60_000))
+        k = int(rng.integers(1, 5))
+        offs = sorted(set(int(x) for x in rng.integers(1, min(700, V - 1) + 1, k)))
+        keep = float(rng.choice([0.5, 0.8, 0.97, 1.0]))
+        us, vs = [], []
+        for d in offs:
+            u = np.arange(V - d, dtype=np.int64)
+            u = u[rng.random(len(u)) < keep]
+            us.append(u)
+            vs.append(u + d)
+        us, vs = np.concatenate(us), np.concatenate(vs)
+        if len(us) == 0:
+            continue
+        g = gb.CsrGraph.from_edge_list(V, list(zip(us.tolist(), vs.tolist()))).to_undirected()
+        found = g.lattice_offsets()
+        assert set(found) <= set(offs) and found, (V, offs, found)
+        S = int(rng.integers(1, 701))
+        src = rng.integers(0, V, S).astype(np.uint32)
+        src[rng.random(S) < 0.05] = src[0]  # duplicates
+        expect = oracle(g, src)
+        dg = gb.DeviceGraph(ctx, g)
+        e = gb.GpuBfsEngine(ctx, M.WorkgroupPersistentBfs)
+        rows = case % 2
+        e.set_tuning(deep_bits=1, deep_lattice=1, deep_rows=rows)
+        say("random lattice", case, "V=%d offsets=%s keep=%.2f S=%d rows=%d" % (V, found, keep, S, rows))
+        got = e.run_ms_bfs(dg, src)
+        st = e.last_stats()
+        # (where the adjacency or a task's region does not fit LDS, the
+        # one-wave or workgroup kernel runs instead, still checked)
+        lat_runs += st["lattice_bits"] == (2 if len(found) <= 2 else 4)
+        assert np.array_equal(got, expect), (case, V, found, S, rows,
+                                             int(np.count_nonzero(got != expect)))
+        if case % 3 == 0:
+            assert np.array_equal(e.run_ms_bfs(dg, src, dist_dtype=np.uint16), to_u16_expect(expect)), case
+            bm = e.run_ms_reachability(dg, src)
+            assert np.array_equal(bm.words, orc.reachability_from_dist(V, S, expect)), case
+    assert lat_runs >= 30, lat_runs
