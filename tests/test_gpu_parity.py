@@ -367,9 +367,12 @@ def test_u8_output_and_boundaries(engines):
 
 
 def test_compact_host_transfer(engines, ctx):
-    """Host outputs cross PCIe as u8 when every distance is below 255 (here
-    in two 32 MiB chunks), as u16 otherwise, and as u32 after a rerun when a
-    distance reaches 65,535; every path widens to the caller's type exactly."""
+    """Host outputs cross PCIe in the narrowest width that holds every
+    distance below its all-ones sentinel: 4 bits when there are at most 15
+    levels (R-MAT), 8 up to 255, 12 up to 4,095 (road-like), else u16, and
+    u32 after a rerun when a distance reaches 65,535; every path widens to
+    the caller's type exactly, at each width boundary and with ragged last
+    groups."""
     import torch
     g = gb.generate_rmat(16, 16)
     src = gb.sample_sources(g, 600, seed=6, non_isolated=True)
@@ -383,10 +386,25 @@ def test_compact_host_transfer(engines, ctx):
     e.run_ms_bfs_device(g, src, out.data_ptr(), out.numel(), dist_dtype=np.uint8)
     torch.cuda.synchronize()
     assert np.array_equal(out.cpu().numpy(), to_u8_expect(expect))
-    road = gb.generate_road_like(20_000, 160, 900, seed=4)  # eccentricities > 255: u16 wire
+    road = gb.generate_road_like(20_000, 160, 900, seed=4)  # eccentricities > 255: 12-bit wire
     rs = gb.sample_sources(road, 300, seed=8)
     say("compact transfer road S=300")
-    assert np.array_equal(e.run_ms_bfs(road, rs), oracle(road, rs))
+    rexp = oracle(road, rs)
+    assert np.array_equal(e.run_ms_bfs(road, rs), rexp)
+    assert np.array_equal(e.run_ms_bfs(road, rs, dist_dtype=np.uint16), to_u16_expect(rexp))
+    # wire width boundaries: a path of n vertices from vertex 0 has n levels
+    # (4 bits up to 15, 8 up to 255, 12 up to 4,095, then u16); odd sizes
+    # leave a ragged last group of 8
+    for n in (2, 15, 16, 255, 256, 4095, 4096, 5001):
+        p = gb.CsrGraph.from_edge_list(n, [(i, i + 1) for i in range(n - 1)]).to_undirected()
+        ps = np.array([0, n - 1, n // 2, 0, 1 % n], dtype=np.uint32)[: 3 if n % 2 else 5]
+        pexp = oracle(p, ps)
+        for m in (M.WorkgroupPersistentBfs, M.PushPullAdaptive):
+            say("compact transfer path", n, m.name)
+            assert np.array_equal(engines[m].run_ms_bfs(p, ps), pexp), (n, m.name)
+            assert np.array_equal(engines[m].run_ms_bfs(p, ps, dist_dtype=np.uint16),
+                                  to_u16_expect(pexp)), (n, m.name)
+            assert engines[m].last_stats()["levels"] >= n, (n, engines[m].last_stats())
 
 
 def test_engine_autotune_keeps_results(ctx):

@@ -1193,14 +1193,63 @@ void widen_any(const void *src, int from, void *dst, int to, uint64_t n, int thr
     }
 }
 
+// Distances packed `bits` (4 or 12) each in groups of 8 (launch_pack_u16_bits)
+// -> T, all ones -> T's sentinel, on up to `threads` host threads.
+template <typename T>
+void unpack_bits(const uint8_t *src, int bits, T *dst, uint64_t n, int threads) {
+    constexpr T ts = (T)~(T)0;
+    const uint32_t sent = (1u << bits) - 1u;
+    auto body = [&](uint64_t g0, uint64_t g1) {
+        for (uint64_t g = g0; g < g1; ++g) {
+            uint32_t v[8];
+            if (bits == 4) {
+                uint32_t r;
+                std::memcpy(&r, src + 4 * g, 4);
+                for (int k = 0; k < 8; ++k) v[k] = (r >> (4 * k)) & 15u;
+            } else {
+                uint32_t w[3];
+                std::memcpy(w, src + 12 * g, 12);
+                v[0] = w[0] & 0xFFFu;
+                v[1] = (w[0] >> 12) & 0xFFFu;
+                v[2] = (w[0] >> 24) | ((w[1] & 0xFu) << 8);
+                v[3] = (w[1] >> 4) & 0xFFFu;
+                v[4] = (w[1] >> 16) & 0xFFFu;
+                v[5] = (w[1] >> 28) | ((w[2] & 0xFFu) << 4);
+                v[6] = (w[2] >> 8) & 0xFFFu;
+                v[7] = w[2] >> 20;
+            }
+            const uint64_t b = 8 * g, m = std::min<uint64_t>(8, n - b);
+            for (uint64_t k = 0; k < m; ++k) dst[b + k] = v[k] == sent ? ts : (T)v[k];
+        }
+    };
+    const uint64_t ng = (n + 7) / 8;
+    if (threads <= 1 || n < (1u << 20)) {
+        body(0, ng);
+        return;
+    }
+    std::vector<std::thread> th;
+    const uint64_t per = (ng + threads - 1) / threads;
+    for (int t = 0; t < threads; ++t) {
+        const uint64_t b = t * per, e = std::min(ng, b + per);
+        if (b < e) th.emplace_back(body, b, e);
+    }
+    for (auto &x : th) x.join();
+}
+
 // Device -> caller's host buffer through three pinned staging chunks: the
-// copy of chunk k+1.. runs on `st` while host threads widen chunk k.
-int transfer_widen(msbfs_engine *e, const void *src, int from, void *dst, int to, uint64_t n,
+// copy of chunk k+1.. runs on `st` while host threads widen chunk k.  The
+// wire holds `bits` per distance: 32, 16 or 8 (plain u32 / u16 / u8) or 4
+// or 12 (launch_pack_u16_bits groups of 8).
+int transfer_widen(msbfs_engine *e, const void *src, int bits, void *dst, int to, uint64_t n,
                    hipStream_t st) {
     constexpr int NS = 3;
-    const size_t fb = dist_bytes(from), tb = dist_bytes(to);
-    const uint64_t chunk = (32ull << 20) / fb;  // elements per chunk: 32 MiB of wire data
-    int rc = e->h_stage.ensure(NS * chunk * fb);
+    const bool packed = bits == 4 || bits == 12;
+    const size_t tb = dist_bytes(to);
+    // elements per chunk: 32 MiB of wire data, whole groups of 8
+    const uint64_t chunk = (32ull << 20) * 8 / bits / 8 * 8;
+    const uint64_t chunk_bytes = chunk / 8 * bits;
+    auto wire_bytes = [&](uint64_t cnt) { return (cnt + 7) / 8 * bits; };
+    int rc = e->h_stage.ensure(NS * chunk_bytes);
     if (rc < 0) return -rc;
     for (auto &ev : e->stage_ev)
         if (!ev) HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
@@ -1208,8 +1257,9 @@ int transfer_widen(msbfs_engine *e, const void *src, int from, void *dst, int to
     const int threads = std::min(16, host_threads());
     auto issue = [&](uint64_t k) -> int {
         const uint64_t off = k * chunk, cnt = std::min(chunk, n - off);
-        HIP_TRY(hipMemcpyAsync((char *)e->h_stage.p + (k % NS) * chunk * fb,
-                               (const char *)src + off * fb, cnt * fb, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync((char *)e->h_stage.p + (k % NS) * chunk_bytes,
+                               (const char *)src + off / 8 * bits,
+                               packed ? wire_bytes(cnt) : cnt * bits / 8, hipMemcpyDeviceToHost, st));
         HIP_TRY(hipEventRecord(e->stage_ev[k % NS], st));
         return MSBFS_OK;
     };
@@ -1218,8 +1268,17 @@ int transfer_widen(msbfs_engine *e, const void *src, int from, void *dst, int to
     for (uint64_t k = 0; k < nch; ++k) {
         HIP_TRY(hipEventSynchronize(e->stage_ev[k % NS]));
         const uint64_t off = k * chunk, cnt = std::min(chunk, n - off);
-        widen_any((const char *)e->h_stage.p + (k % NS) * chunk * fb, from,
-                  (char *)dst + off * tb, to, cnt, threads);
+        const uint8_t *w = (const uint8_t *)e->h_stage.p + (k % NS) * chunk_bytes;
+        char *d = (char *)dst + off * tb;
+        if (!packed)
+            widen_any(w, bits == 32 ? MSBFS_DIST_U32 : bits == 16 ? MSBFS_DIST_U16 : MSBFS_DIST_U8, d,
+                      to, cnt, threads);
+        else if (to == MSBFS_DIST_U32)
+            unpack_bits(w, bits, (uint32_t *)d, cnt, threads);
+        else if (to == MSBFS_DIST_U16)
+            unpack_bits(w, bits, (uint16_t *)d, cnt, threads);
+        else
+            unpack_bits(w, bits, (uint8_t *)d, cnt, threads);
         if (k + NS < nch && (rc = issue(k + NS))) return rc;
     }
     return MSBFS_OK;
@@ -1360,16 +1419,26 @@ extern "C" int msbfs_run_ms_bfs(msbfs_engine *e, const msbfs_graph *g, const uin
         if ((rc = ensure_plain(e->out_tmp, n * 4))) return rc;
         rc = run_common(e, g, sources, S, MSBFS_DIST_U32, e->out_tmp.p, nullptr, 0, st, true);
         if (rc) return rc;
-        return transfer_widen(e, e->out_tmp.p, MSBFS_DIST_U32, out, dtype, n, st);
+        return transfer_widen(e, e->out_tmp.p, 32, out, dtype, n, st);
     }
     if (rc) return rc;
-    if (dev == MSBFS_DIST_U16 && e->last.levels <= 255) {
-        // every distance is below 255: u8 on the wire
-        if ((rc = ensure_plain(e->out_pack, n))) return rc;
-        HIP_TRY(launch_pack_u16_u8(e->out_tmp.as<uint16_t>(), e->out_pack.as<uint8_t>(), n, st));
-        return transfer_widen(e, e->out_pack.p, MSBFS_DIST_U8, out, dtype, n, st);
+    if (dev == MSBFS_DIST_U16) {
+        // the narrowest wire that holds every distance below its all-ones
+        // sentinel (levels = largest distance + 1): 4, 8 or 12 bits, else u16
+        const uint32_t L = e->last.levels;
+        const int bits = L <= 15 ? 4 : L <= 255 ? 8 : L <= 4095 ? 12 : 16;
+        if (bits == 8) {
+            if ((rc = ensure_plain(e->out_pack, n))) return rc;
+            HIP_TRY(launch_pack_u16_u8(e->out_tmp.as<uint16_t>(), e->out_pack.as<uint8_t>(), n, st));
+            return transfer_widen(e, e->out_pack.p, 8, out, dtype, n, st);
+        }
+        if (bits != 16) {
+            if ((rc = ensure_plain(e->out_pack, (n + 7) / 8 * bits))) return rc;
+            HIP_TRY(launch_pack_u16_bits(e->out_tmp.as<uint16_t>(), e->out_pack.as<uint32_t>(), n, bits, st));
+            return transfer_widen(e, e->out_pack.p, bits, out, dtype, n, st);
+        }
     }
-    return transfer_widen(e, e->out_tmp.p, dev, out, dtype, n, st);
+    return transfer_widen(e, e->out_tmp.p, (int)dist_bytes(dev) * 8, out, dtype, n, st);
 }
 
 extern "C" int msbfs_run_ms_reachability(msbfs_engine *e, const msbfs_graph *g,

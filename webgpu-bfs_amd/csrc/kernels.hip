@@ -156,6 +156,58 @@ __global__ void pack_u16_u8_kernel(const uint16_t *in, uint8_t *out, uint64_t n)
     }
 }
 
+// u16 distances (all < 2^W - 1, or 0xFFFF) packed W bits each, W = 4 or 12,
+// in groups of 8 (W bytes per group, element k at bits [W k, W k + W) of the
+// group's little-endian bit string), the sentinel as all ones; a ragged last
+// group is padded with sentinels.
+template <int W>
+__global__ void pack_u16_bits_kernel(const uint16_t *in, uint32_t *out, uint64_t n) {
+    static_assert(W == 4 || W == 12, "4 or 12 bits per distance");
+    constexpr uint32_t SENT = (1u << W) - 1u;
+    const uint64_t ng = (n + 7) / 8, full = n / 8;
+    for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < ng;
+         g += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t v[8];
+        if (g < full) {
+            const uint4 x = ((const uint4 *)in)[g];
+            const uint32_t w[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                v[2 * k] = w[k] & 0xFFFFu;
+                v[2 * k + 1] = w[k] >> 16;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) v[k] = g * 8 + k < n ? (uint32_t)in[g * 8 + k] : 0xFFFFu;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = v[k] == 0xFFFFu ? SENT : v[k];
+        if constexpr (W == 4) {
+            uint32_t r = 0;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) r |= v[k] << (4 * k);
+            out[g] = r;
+        } else {
+            out[3 * g] = v[0] | v[1] << 12 | v[2] << 24;
+            out[3 * g + 1] = v[2] >> 8 | v[3] << 4 | v[4] << 16 | v[5] << 28;
+            out[3 * g + 2] = v[5] >> 4 | v[6] << 8 | v[7] << 20;
+        }
+    }
+}
+
+hipError_t launch_pack_u16_bits(const uint16_t *in, uint32_t *out, uint64_t n, int bits, hipStream_t st) {
+    const uint64_t ng = (n + 7) / 8;
+    if (ng == 0) return hipSuccess;
+    const uint64_t blocks = std::min<uint64_t>((ng + 255) / 256, 8192);
+    if (bits == 4)
+        hipLaunchKernelGGL(pack_u16_bits_kernel<4>, dim3((unsigned)blocks), dim3(256), 0, st, in, out, n);
+    else if (bits == 12)
+        hipLaunchKernelGGL(pack_u16_bits_kernel<12>, dim3((unsigned)blocks), dim3(256), 0, st, in, out, n);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
 hipError_t launch_pack_u16_u8(const uint16_t *in, uint8_t *out, uint64_t n, hipStream_t st) {
     if (n == 0) return hipSuccess;
     uint64_t blocks = (n / 8 + 255) / 256;

@@ -230,6 +230,9 @@ hipError_t launch_wide_final(const WideParams &p, int blocks, hipStream_t st);
 hipError_t launch_zero_u32(uint32_t *ptr, uint64_t n, hipStream_t st);
 // u16 distances (all < 255 or 0xFFFF) -> u8 with 0xFF for unreached.
 hipError_t launch_pack_u16_u8(const uint16_t *in, uint8_t *out, uint64_t n, hipStream_t st);
+// u16 distances (all < 2^bits - 1, or 0xFFFF) -> `bits` (4 or 12) per
+// distance in groups of 8 (bits bytes per group), the sentinel all ones.
+hipError_t launch_pack_u16_bits(const uint16_t *in, uint32_t *out, uint64_t n, int bits, hipStream_t st);
 
 constexpr int kWideThreads = 512;
 
