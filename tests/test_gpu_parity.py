@@ -890,3 +890,60 @@ def test_random_lattices(ctx):
             bm = e.run_ms_reachability(dg, src)
             assert np.array_equal(bm.words, orc.reachability_from_dist(V, S, expect)), case
     assert lat_runs >= 30, lat_runs
+
+
+def test_concurrent_callers(ctx):
+    """The reference's engine is Send + Sync and concurrent calls serialise
+    on its pools (engine.rs:56-63).  Here: six host threads on one context
+    and two resident graphs, four with engines of their own (deep, adaptive
+    wide, pull-only, Auto; host and device outputs) and two sharing one
+    engine, each running a mix of calls; every result is the oracle's.
+    ctypes releases the GIL during the calls, so they overlap on the host."""
+    import threading
+
+    import torch
+
+    road = gb.generate_road_like(20_000, 160, 900, seed=4)
+    rmat = gb.generate_rmat(13, 16, seed=11)
+    dgs = {"road": gb.DeviceGraph(ctx, road), "rmat": gb.DeviceGraph(ctx, rmat)}
+    graphs = {"road": road, "rmat": rmat}
+    cases = []
+    for i in range(6):
+        for name in ("road", "rmat"):
+            src = gb.sample_sources(graphs[name], 40 + 37 * i, seed=100 + i, non_isolated=True)
+            cases.append((name, src, oracle(graphs[name], src)))
+    shared = gb.GpuBfsEngine(ctx, M.Auto)
+    errors = []
+
+    def worker(k, eng, device_out):
+        try:
+            stream = torch.cuda.Stream()
+            for j in range(3):
+                for name, src, exp in cases[k::2] if k % 2 else cases[::3]:
+                    dg = dgs[name]
+                    if device_out:
+                        out = torch.empty(len(src) * dg.num_vertices, dtype=torch.int32, device="cuda")
+                        eng.run_ms_bfs_device(dg, src, out.data_ptr(), out.numel(),
+                                              stream=stream.cuda_stream)
+                        stream.synchronize()
+                        got = out.cpu().numpy().view(np.uint32)
+                    else:
+                        got = eng.run_ms_bfs(dg, src)
+                    if not np.array_equal(got, exp):
+                        errors.append((k, name, len(src), j))
+        except Exception as ex:  # reported below, in the main thread
+            errors.append((k, repr(ex)))
+
+    plan = [(0, gb.GpuBfsEngine(ctx, M.WorkgroupPersistentBfs), False),
+            (1, gb.GpuBfsEngine(ctx, M.PushPullAdaptive), True),
+            (2, gb.GpuBfsEngine(ctx, M.BitParallel64Pull), False),
+            (3, gb.GpuBfsEngine(ctx, M.Auto), True),
+            (4, shared, False), (5, shared, True)]
+    say("concurrent callers:", len(plan), "threads")
+    ths = [threading.Thread(target=worker, args=p) for p in plan]
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join(timeout=120)
+    assert not any(t.is_alive() for t in ths), "a caller did not finish"
+    assert not errors, errors
