@@ -165,6 +165,11 @@ typedef struct msbfs_tuning {
                                 1K-64K vertices, ms per traversal of the graphs of 1, 2 and 4 words:
                                 unions 5.9, 1.8 and 2.0 against about 2.5, 2.1 and 3.6 in the deep
                                 launch) */
+    uint32_t deep_blocks_per_cu; /* deep family persistent launches (one graph's and a batch's): at
+                                most this many workgroups per CU (<= 32); 0 auto: as many as fit,
+                                except a batch whose union traversals run beside its deep launch,
+                                which takes 1 and leaves the unions the rest of each CU (512 G(n,p)
+                                graphs: 5.58 ms per replay against 6.51 with three) */
 } msbfs_tuning;
 
 /* One BFS level of the last wide-family run (recorded when trace_levels > 0). */

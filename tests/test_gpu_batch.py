@@ -51,7 +51,7 @@ def check(batch, out, expect, u16=False):
 # every msbfs_tuning field a candidate of the autotuner may set
 TUNING_FIELDS = ("alpha", "beta", "tile_words", "words_per_lane", "deep_bits", "deep_threads",
                  "wide_blocks_per_cu", "direction_rule", "stepped", "deep_rows", "deep_lattice",
-                 "batch_union", "union_words")
+                 "batch_union", "union_words", "deep_blocks_per_cu")
 MODES = {"union": dict(batch_union=1), "deep": dict(batch_union=2),
          "split": dict(batch_union=0, union_words=2)}
 

@@ -271,13 +271,13 @@ def test_tuning_validation_rejects_bad_fields():
 
     assert rc() == _lib.OK
     assert rc(deep_threads=64, deep_bits=1, tile_words=16, words_per_lane=2) == _lib.OK
-    assert rc(batch_union=2, union_words=4, deep_lattice=1) == _lib.OK
+    assert rc(batch_union=2, union_words=4, deep_lattice=1, deep_blocks_per_cu=1) == _lib.OK
     for bad in (dict(deep_threads=32), dict(deep_threads=1), dict(deep_threads=96),
                 dict(deep_threads=2048), dict(deep_bits=3), dict(deep_bits=64),
                 dict(tile_words=3), dict(tile_words=128), dict(words_per_lane=3),
                 dict(direction_rule=2), dict(wide_blocks_per_cu=17), dict(deep_rows=3),
                 dict(deep_lattice=3), dict(batch_union=3), dict(union_words=3),
-                dict(union_words=128)):
+                dict(union_words=128), dict(deep_blocks_per_cu=33)):
         assert rc(**bad) == _lib.ERR_INVALID_ARG, bad
         assert _lib.last_error()
     assert lib.msbfs_tuning_validate(None) == _lib.ERR_INVALID_ARG

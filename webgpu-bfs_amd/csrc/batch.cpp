@@ -334,6 +334,13 @@ int batch_plan(msbfs_batch *b) {
     int occ = 0;
     HIP_TRY(deep_max_blocks_per_cu(B, p.ell, p.threads, p.lds, &occ));
     if (occ < 1) return fail(MSBFS_ERR_HIP, "batch: deep kernel cannot be resident");
+    // With union traversals beside it, the persistent launch takes one
+    // workgroup per CU by default and leaves the unions the rest (512
+    // G(n,p) graphs, ms per replay by workgroups per CU: 1 5.58, 2 6.52,
+    // 3 6.51): its tasks are latency-bound, the unions' levels are not.
+    const uint32_t occ_cap = b->tune.deep_blocks_per_cu ? b->tune.deep_blocks_per_cu
+                                                        : (b->uclass.empty() ? 0u : 1u);
+    if (occ_cap) occ = std::min(occ, (int)occ_cap);
     b->blocks = (int)std::min<uint64_t>(std::max<uint64_t>(b->ntasks, 1), (uint64_t)c->cus * occ);
     const uint64_t cap = std::max<uint64_t>((uint64_t)B * b->deep_max_v, (uint64_t)B);
     const uint64_t spill = cap > p.qcap ? cap - p.qcap : 1;
@@ -887,6 +894,14 @@ extern "C" int msbfs_batch_autotune(msbfs_batch *b, void *out, uint64_t out_len,
         knob(&msbfs_tuning::alpha, {4u, 8u, 28u});
         knob(&msbfs_tuning::beta, {12u, 48u});
         knob(&msbfs_tuning::stepped, {1u, 2u});
+        if (winner.batch_union == 0) {
+            // beside the unions the deep launch's best shape differs from
+            // its best alone (512 G(n,p) graphs, ms per replay: 640 threads
+            // x 1 workgroup per CU 5.45, 512 x 1 5.55, 768 x 1 5.63, 1024 x
+            // 1 6.13, 256 x 2 5.70, 512 x 3 6.52)
+            knob(&msbfs_tuning::deep_threads, {256u, 512u, 640u, 768u, 1024u});
+            knob(&msbfs_tuning::deep_blocks_per_cu, {1u, 2u, 4u});
+        }
     }
     b->tune = winner;
     if ((rc = batch_plan(b))) return rc;

@@ -408,6 +408,8 @@ extern "C" int msbfs_tuning_validate(const msbfs_tuning *t) {
     if (t->batch_union > 2) return fail(MSBFS_ERR_INVALID_ARG, "batch_union must be 0, 1 or 2");
     if (t->union_words && (t->union_words > 64 || (t->union_words & (t->union_words - 1))))
         return fail(MSBFS_ERR_INVALID_ARG, "union_words must be a power of two <= 64");
+    if (t->deep_blocks_per_cu > 32)
+        return fail(MSBFS_ERR_INVALID_ARG, "deep_blocks_per_cu must be <= 32");
     return MSBFS_OK;
 }
 
@@ -962,6 +964,7 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
         else
             HIP_TRY(deep_max_blocks_per_cu(B, plan.ell, threads, plan.lds, &occ));
         if (occ < 1) return fail(MSBFS_ERR_HIP, "deep kernel cannot be resident (lds %zu)", plan.lds);
+        if (e->tune.deep_blocks_per_cu) occ = std::min(occ, (int)e->tune.deep_blocks_per_cu);
         const uint64_t resident = (uint64_t)c->cus * occ;
         const int blocks = (int)std::min<uint64_t>((ntasks + waves - 1) / waves, resident);
         const uint64_t cap = std::max<uint64_t>((uint64_t)B * V, (uint64_t)B);

@@ -106,7 +106,7 @@ class Tuning(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint32) for n in (
         "alpha", "beta", "tile_words", "words_per_lane", "deep_bits", "deep_threads",
         "wide_blocks_per_cu", "direction_rule", "trace_levels", "stepped", "deep_rows",
-        "deep_lattice", "batch_union", "union_words")]
+        "deep_lattice", "batch_union", "union_words", "deep_blocks_per_cu")]
 
 
 class SweepEntry(ctypes.Structure):
