@@ -127,6 +127,18 @@ def test_road_like_deep_tile_variants(ctx, road):
     src = wl.sources(road, 64)
     assert np.array_equal(e.run_ms_bfs(road, src), oracle(road, src))
     assert e.last_stats()["family"] == int(gb.Family.WIDE)
+    # capped persistent launches (deep_blocks_per_cu): fewer workgroups take
+    # the same tasks from the queue, lattice kernel and workgroup kernel
+    src = wl.sources(road, 700)
+    expect = oracle(road, src)
+    cus = ctx.info()["compute_units"]
+    for lattice in (0, 2):
+        for cap in (1, 2):
+            e = gb.GpuBfsEngine(ctx, M.WorkgroupPersistentBfs)
+            e.set_tuning(deep_bits=1, deep_lattice=lattice, deep_blocks_per_cu=cap)
+            say("road S=700 deep_blocks_per_cu", cap, "lattice", lattice)
+            assert np.array_equal(e.run_ms_bfs(road, src), expect), (cap, lattice)
+            assert e.last_stats()["grid_blocks"] <= cus * cap, (cap, e.last_stats())
 
 
 def test_deep_one_wave_routing(ctx, road):
