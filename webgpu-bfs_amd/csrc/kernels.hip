@@ -973,28 +973,20 @@ __global__ void __launch_bounds__(LAT ? 64 * kLatticeMaxWaves : 64) deep_wave_ke
                         tot += (uint32_t)__popcll(bk[k]);
                     }
                     if (nn + tot <= qb.lcap) {
-                        if constexpr (LAT > 0) {
-                            // straight-line: an LDS byte address for every lane
-                            // (the slot's run, wave-uniform, plus the lane's rank
-                            // in it), then a select on the slot's ballot: the
-                            // lane's sink word when it has no new entry (a
-                            // masked branch per slot costs more instructions)
-                            const uint32_t qaddr = lds_addr(qb.lds);
-                            const uint32_t saddr = lds_addr(s_sink) + (uint32_t)sizeof(Entry) * lane;
+                        // straight-line: an LDS byte address for every lane
+                        // (the slot's run, wave-uniform, plus the lane's rank in
+                        // it), then a select on the slot's ballot: the lane's
+                        // sink word when it has no new entry (a masked branch
+                        // per slot costs more instructions)
+                        const uint32_t qaddr = lds_addr(qb.lds);
+                        const uint32_t saddr = lds_addr(s_sink) + (uint32_t)sizeof(Entry) * lane;
 #pragma unroll
-                            for (int k = 0; k < ELL; ++k) {
-                                // (the run's address is summed in scalar registers)
-                                const uint32_t run = (uint32_t)__builtin_amdgcn_readfirstlane(
-                                    (int)(qaddr + (uint32_t)sizeof(Entry) * base_k[k]));
-                                const uint32_t a = run + (uint32_t)sizeof(Entry) * mbcnt64(bk[k], 0u);
-                                *(lq_t)(size_t)lane_select(bk[k], saddr, a) = pk(nb[k], newly[k]);
-                            }
-                        } else {
-                            lq_t const q = (lq_t)qb.lds;
-                            lq_t const sk = (lq_t)s_sink + lane;
-#pragma unroll
-                            for (int k = 0; k < ELL; ++k)
-                                *(newly[k] ? q + mbcnt64(bk[k], base_k[k]) : sk) = pk(nb[k], newly[k]);
+                        for (int k = 0; k < ELL; ++k) {
+                            // (the run's address is summed in scalar registers)
+                            const uint32_t run = (uint32_t)__builtin_amdgcn_readfirstlane(
+                                (int)(qaddr + (uint32_t)sizeof(Entry) * base_k[k]));
+                            const uint32_t a = run + (uint32_t)sizeof(Entry) * mbcnt64(bk[k], 0u);
+                            *(lq_t)(size_t)lane_select(bk[k], saddr, a) = pk(nb[k], newly[k]);
                         }
                     } else {
 #pragma unroll
