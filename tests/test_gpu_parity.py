@@ -959,3 +959,29 @@ def test_concurrent_callers(ctx):
         t.join(timeout=120)
     assert not any(t.is_alive() for t in ths), "a caller did not finish"
     assert not errors, errors
+
+
+def test_sources_change_between_calls(ctx, road):
+    """Sources change between calls on one engine: the same array changed
+    in place, a different array of the same length, the first one again, a
+    different size, and a device-output call in between each give their own
+    sources' rows."""
+    import torch
+
+    e = gb.GpuBfsEngine(ctx, M.Auto)
+    a = wl.sources(road, 64)
+    b = (a + 7) % road.num_vertices
+    exp_a, exp_b = oracle(road, a), oracle(road, b)
+    buf = a.copy()
+    say("sources cache: same array, then changed in place")
+    assert np.array_equal(e.run_ms_bfs(road, buf), exp_a)
+    assert np.array_equal(e.run_ms_bfs(road, buf), exp_a)
+    buf[:] = b
+    assert np.array_equal(e.run_ms_bfs(road, buf), exp_b)
+    assert np.array_equal(e.run_ms_bfs(road, a), exp_a)
+    c = wl.sources(road, 65)
+    assert np.array_equal(e.run_ms_bfs(road, c), oracle(road, c))
+    out = torch.empty(64 * road.num_vertices, dtype=torch.int32, device="cuda")
+    e.run_ms_bfs_device(road, b, out.data_ptr(), out.numel())
+    assert np.array_equal(out.cpu().numpy().view(np.uint32), exp_b)
+    assert np.array_equal(e.run_ms_bfs(road, a), exp_a)

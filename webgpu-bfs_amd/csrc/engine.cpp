@@ -754,7 +754,8 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
         std::memcpy(e->h_sources.p, sources, S * 4);
     }
     if ((rc = ensure_plain(e->sources, nsrc_words * 4))) return rc;
-    if ((rc = ensure_plain(e->flags, FlagCount * 4))) return rc;
+    // the flags, then the deep launch's task counters (cleared together)
+    if ((rc = ensure_plain(e->flags, (FlagCount + 4) * 4))) return rc;
     if ((rc = e->h_flags.ensure(FlagCount * 4)) < 0) return -rc;
     // the cooperative wide kernel needs the device to itself; stepped wide
     // launches and deep launches share it
@@ -764,7 +765,7 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
     if ((rc = order.begin())) return rc;
     HIP_TRY(hipEventRecord(e->ev[0], st));
     HIP_TRY(hipMemcpyAsync(e->sources.p, e->h_sources.p, nsrc_words * 4, hipMemcpyHostToDevice, st));
-    HIP_TRY(launch_zero_u32(e->flags.as<uint32_t>(), FlagCount, st));
+    HIP_TRY(launch_zero_u32(e->flags.as<uint32_t>(), FlagCount + 4, st));
 
     e->last = msbfs_run_stats{};
     e->last.method = plan.method;
@@ -973,7 +974,6 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
         if ((rc = ensure_plain(e->queues, (size_t)blocks * waves * 2 * spill * esz))) return rc;
         if ((rc = ensure_plain(e->tasks, ntasks * sizeof(DeepTask)))) return rc;
         if ((rc = ensure_plain(e->graphs, sizeof(DeepGraph)))) return rc;
-        if ((rc = ensure_plain(e->work, 16))) return rc;
         if ((rc = e->h_tasks.ensure(ntasks * sizeof(DeepTask) + sizeof(DeepGraph))) < 0) return -rc;
         // The task table and the graph record depend only on these values;
         // when they match the last upload, the device copies are current.
@@ -1006,7 +1006,6 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
         HIP_TRY(hipMemcpyAsync(e->graphs.p, hg, sizeof(DeepGraph), hipMemcpyHostToDevice, st));
         std::memcpy(e->tasks_key, key, sizeof(key));
         }
-        HIP_TRY(launch_zero_u32(e->work.as<uint32_t>(), 4, st));
 
         DeepParams p{};
         p.tasks = e->tasks.as<DeepTask>();
@@ -1037,7 +1036,7 @@ int enqueue_run(msbfs_engine *e, const msbfs_graph *g, const uint32_t *sources, 
             p.log_defer = e->log_defer.as<uint32_t>();
             p.log_defer_after = (uint32_t)(ntasks - nlist);
         }
-        p.work = e->work.as<uint32_t>();
+        p.work = e->flags.as<uint32_t>() + FlagCount;  // (cleared with the flags)
         p.flags = e->flags.as<uint32_t>();
         // reachability: each task writes its sources' rows of a source-major
         // bitmap, transposed into the vertex-major words afterwards

@@ -160,7 +160,7 @@ struct msbfs_engine {
     uint32_t tick_base = 1;
     bool dirty = true;
     // deep workspace
-    msbfs::DevBuf queues, tasks, graphs, work, log_sink, log_defer;
+    msbfs::DevBuf queues, tasks, graphs, log_sink, log_defer;
     // shared
     msbfs::DevBuf flags, sources, out_tmp, out_pack, reach_tmp, reach_t, trace;
     // host-output path: pinned staging chunks and their copy-done events
