@@ -699,16 +699,18 @@ __global__ void __launch_bounds__(1024) deep_bfs_kernel(DeepParams p) {
 // level's chain is LDS operations only: entry read, adjacency fields of the
 // vertex and of its lower neighbours, claims, appends.  (Road-like graph on
 // a 339-column grid, 114,599 vertices: 2 bits per vertex, 28.6 KB.)
-// The fields follow `pad` zero fields (pad >= every offset).  Vertex ids are
-// biased (see the kernel) and `c` maps a biased id to its field index in LDS
-// terms: field x lives in the word at LDS byte address (x / FPW) * 4, so no
-// base address is added per read.
+
 // Bit `off % 32` of w sign-extended: 0 or all ones (one v_bfe_i32 with the
 // offset in a register).
 __device__ __forceinline__ uint32_t bit_mask_at(uint32_t w, uint32_t off) {
     return (uint32_t)__builtin_amdgcn_sbfe((int)w, off, 1u);
 }
 
+// The lattice neighbours of vertex u.  The fields follow `pad` zero fields
+// (pad >= every offset).  Vertex ids are biased (see the kernel) and `c`
+// maps a biased id to its field index in LDS terms: field x lives in the
+// word at LDS byte address (x / FPW) * 4, so no base address is added per
+// read.
 template <int LAT>
 __device__ __forceinline__ void lattice_neighbours(const uint32_t (&d)[4], uint32_t c, uint32_t u,
                                                    uint32_t (&nb)[2 * LAT]) {
