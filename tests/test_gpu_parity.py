@@ -902,6 +902,15 @@ def test_random_lattices(ctx):
             bm = e.run_ms_reachability(dg, src)
             assert np.array_equal(bm.words, orc.reachability_from_dist(V, S, expect)), case
     assert lat_runs >= 30, lat_runs
+    # near the LDS limit: a 600 x 600 grid (360,000 vertices, 90 KB of
+    # adjacency and 45 KB of visited bits in one workgroup)
+    g = gb.generate_grid2d(600, 600)
+    src = rng.integers(0, g.num_vertices, 40).astype(np.uint32)
+    e = gb.GpuBfsEngine(ctx, M.WorkgroupPersistentBfs)
+    e.set_tuning(deep_bits=1, deep_lattice=1)
+    say("random lattices: grid 600 x 600 S=40")
+    assert np.array_equal(e.run_ms_bfs(g, src), oracle(g, src))
+    assert e.last_stats()["lattice_bits"] == 2, e.last_stats()
 
 
 def test_concurrent_callers(ctx):
