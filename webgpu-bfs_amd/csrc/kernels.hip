@@ -860,167 +860,179 @@ __global__ void __launch_bounds__(LAT ? 64 * kLatticeMaxWaves : 64) deep_wave_ke
         // and a store issued late in a pass would hold up the next pass's
         // first wait on the memory counter)
         uint32_t pend_at = 0xFFFFFFFFu, pend_pos = 0, pend_val = 0;
+        // levels of at most nfast entries: one pass whose entries are all in
+        // the LDS part of the queue, and whose appends (at most ELL each) fit it
+        const uint32_t nfast = p.lds_qcap >= 64u ? min(64u, p.lds_qcap / ELL) : 0u;
         while (n) {
             if (OUT && level > DMAX && lane == 0) atomicOr(&p.flags[FlagU16Overflow], 1u);
             const uint32_t dval = min(level, DMAX);
             uint32_t nn = 0;  // entries appended for the next level (wave-uniform)
-            for (uint32_t base = 0; base < n; base += 64) {
+            // One pass over entries base .. base + 63 of this level.  SPILL:
+            // they may lie in the queue's global part; FIT: every append is
+            // known to fit the queue's LDS part.
+            auto pass = [&](uint32_t base, auto spill, auto fit) {
                 const uint32_t i = base + lane;
                 const bool act = i < n;
-                auto pass = [&](auto spill) {
-                    // (vertex 0 is id vbias in the lattice kernel)
-                    uint32_t u = vbias, f = 0;
-                    if constexpr (decltype(spill)::value) {
-                        if (act) upk(qa.get(i), u, f);
-                    } else {
-                        // all lanes read (i < lcap); a lane without an entry
-                        // takes vertex 0 and no source bits, so its row load
-                        // is valid and it appends nothing
-                        if constexpr (LAT > 0)
-                            upk(*(lq_t)(size_t)(lds_addr(qa.lds) + (uint32_t)sizeof(Entry) * i), u, f);
-                        else
-                            upk(((lq_t)qa.lds)[i], u, f);
-                        u = act ? u : vbias;
-                        f = act ? f : 0u;
+                // (vertex 0 is id vbias in the lattice kernel)
+                uint32_t u = vbias, f = 0;
+                if constexpr (decltype(spill)::value) {
+                    if (act) upk(qa.get(i), u, f);
+                } else {
+                    // all lanes read (i < lcap); a lane without an entry
+                    // takes vertex 0 and no source bits, so its row load
+                    // is valid and it appends nothing
+                    if constexpr (LAT > 0)
+                        upk(*(lq_t)(size_t)(lds_addr(qa.lds) + (uint32_t)sizeof(Entry) * i), u, f);
+                    else
+                        upk(((lq_t)qa.lds)[i], u, f);
+                    u = act ? u : vbias;
+                    f = act ? f : 0u;
+                }
+                constexpr bool ALL = !decltype(spill)::value;
+                uint32_t nb[ELL];
+                if constexpr (LAT > 0) {
+                    lattice_neighbours<LAT>(ld, lfield, u, nb);
+                } else if (ALL || act) {
+                    const __attribute__((address_space(1))) u32x4 *row =
+                        (const __attribute__((address_space(1))) u32x4 *)(ell_s + (uint64_t)u * ELL);
+#pragma unroll
+                    for (int k = 0; k < ELL / 4; ++k) {
+                        const u32x4 r = row[k];
+                        nb[4 * k] = r.x;
+                        nb[4 * k + 1] = r.y;
+                        nb[4 * k + 2] = r.z;
+                        nb[4 * k + 3] = r.w;
                     }
-                    constexpr bool ALL = !decltype(spill)::value;
-                    uint32_t nb[ELL];
+                }
+                if constexpr (LOG) {
+                    // the previous pass's log entry: one unconditional
+                    // store per lane (lanes without an entry write their
+                    // sink word), so the wait for this pass's row load
+                    // may leave it in flight; its bin position (an LDS
+                    // atomic issued late in that pass) is added only
+                    // here, so no wait for it ends that pass
+                    *(pend_at != 0xFFFFFFFFu ? (g32_t)out_task + pend_at + pend_pos : sink) = pend_val;
+                    pend_at = 0xFFFFFFFFu;
+                } else if constexpr (OUT && LAT == 0) {
+                    // the entry's lowest source bit, one straight-line
+                    // store per lane (a lane without an entry repeats
+                    // lane 0's)
+                    const uint32_t b0 = B == 1 ? 0u : __builtin_ctz(f | 0x80000000u);
+                    const uint32_t u_l0 = (uint32_t)__builtin_amdgcn_readlane((int)u, 0);
+                    const uint32_t b_l0 = (uint32_t)__builtin_amdgcn_readlane((int)b0, 0);
+                    const uint32_t off = (act ? b0 : b_l0) * V + (act ? u : u_l0);
+                    if constexpr (KIND == 1)
+                        ((__attribute__((address_space(1))) uint16_t *)out_task)[off] = (uint16_t)dval;
+                    else if constexpr (KIND == 2)
+                        ((__attribute__((address_space(1))) uint8_t *)out_task)[off] = (uint8_t)dval;
+                    else
+                        ((__attribute__((address_space(1))) uint32_t *)out_task)[off] = dval;
+                }
+                // claims: all slots' LDS atomics, then one wait
+                uint32_t newly[ELL];
+                if (act) {  // (lanes without an entry would all hit vertex 0's words)
+                    uint32_t old[ELL];
                     if constexpr (LAT > 0) {
-                        lattice_neighbours<LAT>(ld, lfield, u, nb);
-                    } else if (ALL || act) {
-                        const __attribute__((address_space(1))) u32x4 *row =
-                            (const __attribute__((address_space(1))) u32x4 *)(ell_s + (uint64_t)u * ELL);
+                        f = 1u;  // one source per task
+                        // biased ids: the visited word's LDS byte address
+                        // and the bit come straight from the id
 #pragma unroll
-                        for (int k = 0; k < ELL / 4; ++k) {
-                            const u32x4 r = row[k];
-                            nb[4 * k] = r.x;
-                            nb[4 * k + 1] = r.y;
-                            nb[4 * k + 2] = r.z;
-                            nb[4 * k + 3] = r.w;
-                        }
-                    }
-                    if constexpr (LOG) {
-                        // the previous pass's log entry: one unconditional
-                        // store per lane (lanes without an entry write their
-                        // sink word), so the wait for this pass's row load
-                        // may leave it in flight; its bin position (an LDS
-                        // atomic issued late in that pass) is added only
-                        // here, so no wait for it ends that pass
-                        *(pend_at != 0xFFFFFFFFu ? (g32_t)out_task + pend_at + pend_pos : sink) = pend_val;
-                        pend_at = 0xFFFFFFFFu;
-                    } else if constexpr (OUT && LAT == 0) {
-                        // the entry's lowest source bit, one straight-line
-                        // store per lane (a lane without an entry repeats
-                        // lane 0's)
-                        const uint32_t b0 = B == 1 ? 0u : __builtin_ctz(f | 0x80000000u);
-                        const uint32_t u_l0 = (uint32_t)__builtin_amdgcn_readlane((int)u, 0);
-                        const uint32_t b_l0 = (uint32_t)__builtin_amdgcn_readlane((int)b0, 0);
-                        const uint32_t off = (act ? b0 : b_l0) * V + (act ? u : u_l0);
-                        if constexpr (KIND == 1)
-                            ((__attribute__((address_space(1))) uint16_t *)out_task)[off] = (uint16_t)dval;
-                        else if constexpr (KIND == 2)
-                            ((__attribute__((address_space(1))) uint8_t *)out_task)[off] = (uint8_t)dval;
-                        else
-                            ((__attribute__((address_space(1))) uint32_t *)out_task)[off] = dval;
-                    }
-                    // claims: all slots' LDS atomics, then one wait
-                    uint32_t newly[ELL];
-                    if (act) {  // (lanes without an entry would all hit vertex 0's words)
-                        uint32_t old[ELL];
-                        if constexpr (LAT > 0) {
-                            f = 1u;  // one source per task
-                            // biased ids: the visited word's LDS byte address
-                            // and the bit come straight from the id
-#pragma unroll
-                            for (int k = 0; k < ELL; ++k)
-                                old[k] = __hip_atomic_fetch_or((l32_t)(size_t)((nb[k] >> 5) * 4u), 1u << (nb[k] & 31u),
-                                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                        } else {
-#pragma unroll
-                            for (int k = 0; k < ELL; ++k)
-                                old[k] = __hip_atomic_fetch_or(lvis + nb[k] / VPW, f << ((nb[k] % VPW) * B),
-                                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                        }
-                        if constexpr (LAT > 0 && OUT && !LOG) {
-                            // the lattice kernel has no global load to keep
-                            // ahead of, so its store goes in this masked block
-                            if constexpr (KIND == 1)
-                                ((__attribute__((address_space(1))) uint16_t *)out_lat)[u] = (uint16_t)dval;
-                            else if constexpr (KIND == 2)
-                                ((__attribute__((address_space(1))) uint8_t *)out_lat)[u] = (uint8_t)dval;
-                            else
-                                ((__attribute__((address_space(1))) uint32_t *)out_lat)[u] = dval;
-                        }
-                        if constexpr (LAT > 0) {
-#pragma unroll
-                            for (int k = 0; k < ELL; ++k) newly[k] = ~old[k] & (1u << (nb[k] & 31u));
-                        } else {
-#pragma unroll
-                            for (int k = 0; k < ELL; ++k) newly[k] = f & ~(old[k] >> ((nb[k] % VPW) * B));
-                        }
+                        for (int k = 0; k < ELL; ++k)
+                            old[k] = __hip_atomic_fetch_or((l32_t)(size_t)((nb[k] >> 5) * 4u), 1u << (nb[k] & 31u),
+                                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                     } else {
 #pragma unroll
-                        for (int k = 0; k < ELL; ++k) newly[k] = 0;
+                        for (int k = 0; k < ELL; ++k)
+                            old[k] = __hip_atomic_fetch_or(lvis + nb[k] / VPW, f << ((nb[k] % VPW) * B),
+                                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                     }
-                    // slots, slot-major: the new entries of slot k follow those
-                    // of slots < k, in lane order within the slot (one ballot
-                    // and one v_mbcnt pair per slot; queue order is free)
-                    u64 bk[ELL];
-                    uint32_t base_k[ELL];
-                    uint32_t tot = 0;
+                    if constexpr (LAT > 0 && OUT && !LOG) {
+                        // the lattice kernel has no global load to keep
+                        // ahead of, so its store goes in this masked block
+                        if constexpr (KIND == 1)
+                            ((__attribute__((address_space(1))) uint16_t *)out_lat)[u] = (uint16_t)dval;
+                        else if constexpr (KIND == 2)
+                            ((__attribute__((address_space(1))) uint8_t *)out_lat)[u] = (uint8_t)dval;
+                        else
+                            ((__attribute__((address_space(1))) uint32_t *)out_lat)[u] = dval;
+                    }
+                    if constexpr (LAT > 0) {
+#pragma unroll
+                        for (int k = 0; k < ELL; ++k) newly[k] = ~old[k] & (1u << (nb[k] & 31u));
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < ELL; ++k) newly[k] = f & ~(old[k] >> ((nb[k] % VPW) * B));
+                    }
+                } else {
+#pragma unroll
+                    for (int k = 0; k < ELL; ++k) newly[k] = 0;
+                }
+                // slots, slot-major: the new entries of slot k follow those
+                // of slots < k, in lane order within the slot (one ballot
+                // and one v_mbcnt pair per slot; queue order is free)
+                u64 bk[ELL];
+                uint32_t base_k[ELL];
+                uint32_t tot = 0;
+#pragma unroll
+                for (int k = 0; k < ELL; ++k) {
+                    bk[k] = __ballot(newly[k] != 0);
+                    base_k[k] = nn + tot;
+                    tot += (uint32_t)__popcll(bk[k]);
+                }
+                if (decltype(fit)::value || nn + tot <= qb.lcap) {
+                    // straight-line: an LDS byte address for every lane
+                    // (the slot's run, wave-uniform, plus the lane's rank in
+                    // it), then a select on the slot's ballot: the lane's
+                    // sink word when it has no new entry (a masked branch
+                    // per slot costs more instructions)
+                    const uint32_t qaddr = lds_addr(qb.lds);
+                    const uint32_t saddr = lds_addr(s_sink) + (uint32_t)sizeof(Entry) * lane;
 #pragma unroll
                     for (int k = 0; k < ELL; ++k) {
-                        bk[k] = __ballot(newly[k] != 0);
-                        base_k[k] = nn + tot;
-                        tot += (uint32_t)__popcll(bk[k]);
+                        // (the run's address is summed in scalar registers)
+                        const uint32_t run = (uint32_t)__builtin_amdgcn_readfirstlane(
+                            (int)(qaddr + (uint32_t)sizeof(Entry) * base_k[k]));
+                        const uint32_t a = run + (uint32_t)sizeof(Entry) * mbcnt64(bk[k], 0u);
+                        *(lq_t)(size_t)lane_select(bk[k], saddr, a) = pk(nb[k], newly[k]);
                     }
-                    if (nn + tot <= qb.lcap) {
-                        // straight-line: an LDS byte address for every lane
-                        // (the slot's run, wave-uniform, plus the lane's rank in
-                        // it), then a select on the slot's ballot: the lane's
-                        // sink word when it has no new entry (a masked branch
-                        // per slot costs more instructions)
-                        const uint32_t qaddr = lds_addr(qb.lds);
-                        const uint32_t saddr = lds_addr(s_sink) + (uint32_t)sizeof(Entry) * lane;
+                } else {
 #pragma unroll
-                        for (int k = 0; k < ELL; ++k) {
-                            // (the run's address is summed in scalar registers)
-                            const uint32_t run = (uint32_t)__builtin_amdgcn_readfirstlane(
-                                (int)(qaddr + (uint32_t)sizeof(Entry) * base_k[k]));
-                            const uint32_t a = run + (uint32_t)sizeof(Entry) * mbcnt64(bk[k], 0u);
-                            *(lq_t)(size_t)lane_select(bk[k], saddr, a) = pk(nb[k], newly[k]);
-                        }
-                    } else {
-#pragma unroll
-                        for (int k = 0; k < ELL; ++k) {
-                            if (newly[k]) {
-                                if (!qb.put(mbcnt64(bk[k], base_k[k]), pk(nb[k], newly[k])))
-                                    atomicOr(&p.flags[FlagQueueOverflow], 1u);
-                            }
+                    for (int k = 0; k < ELL; ++k) {
+                        if (newly[k]) {
+                            if (!qb.put(mbcnt64(bk[k], base_k[k]), pk(nb[k], newly[k])))
+                                atomicOr(&p.flags[FlagQueueOverflow], 1u);
                         }
                     }
-                    nn += tot;
-                    if constexpr (LOG) {
-                        // append (vertex in bin, level) to the bin's log: the
-                        // position is reserved after the claims and appends
-                        if (act) {
-                            const uint32_t ur = u - vbias;  // (lattice ids are biased)
-                            const uint32_t bin = ur >> lbw;
-                            pend_pos = __hip_atomic_fetch_add(lbin + bin, 1u, __ATOMIC_RELAXED,
-                                                              __HIP_MEMORY_SCOPE_WORKGROUP);
-                            pend_at = bin << lbw;
-                            pend_val = (ur & lmask) | (level << lbw);
-                        }
+                }
+                nn += tot;
+                if constexpr (LOG) {
+                    // append (vertex in bin, level) to the bin's log: the
+                    // position is reserved after the claims and appends
+                    if (act) {
+                        const uint32_t ur = u - vbias;  // (lattice ids are biased)
+                        const uint32_t bin = ur >> lbw;
+                        pend_pos = __hip_atomic_fetch_add(lbin + bin, 1u, __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_WORKGROUP);
+                        pend_at = bin << lbw;
+                        pend_val = (ur & lmask) | (level << lbw);
                     }
-                    // the entry's other source bits (sources whose searches
-                    // reach the vertex at the same level)
-                    if constexpr (OUT && B > 1)
-                        if (act) deep_write_bits<B, KIND>(out_task, V, u, f & (f - 1), dval);
-                };
-                if (base + 64 <= qa.lcap)
-                    pass(std::false_type{});
-                else
-                    pass(std::true_type{});
+                }
+                // the entry's other source bits (sources whose searches
+                // reach the vertex at the same level)
+                if constexpr (OUT && B > 1)
+                    if (act) deep_write_bits<B, KIND>(out_task, V, u, f & (f - 1), dval);
+            };
+            if (__builtin_expect(n <= nfast, 1)) {
+                // the usual level: one pass, no spill, appends that fit (no
+                // taken branch on its path but the level loop's own)
+                pass(0u, std::false_type{}, std::true_type{});
+            } else {
+                for (uint32_t base = 0; base < n; base += 64) {
+                    if (base + 64 <= qa.lcap)
+                        pass(base, std::false_type{}, std::false_type{});
+                    else
+                        pass(base, std::true_type{}, std::false_type{});
+                }
             }
             DeepQueue<Entry> tq = qa;
             qa = qb;
