@@ -865,7 +865,7 @@ __global__ void __launch_bounds__(LAT ? 64 * kLatticeMaxWaves : 64) deep_wave_ke
         const uint32_t nfast = p.lds_qcap >= 64u ? min(64u, p.lds_qcap / ELL) : 0u;
         while (n) {
             if (OUT && level > DMAX && lane == 0) atomicOr(&p.flags[FlagU16Overflow], 1u);
-            const uint32_t dval = min(level, DMAX);
+            uint32_t dval = min(level, DMAX);
             uint32_t nn = 0;  // entries appended for the next level (wave-uniform)
             // One pass over entries base .. base + 63 of this level.  SPILL:
             // they may lie in the queue's global part; FIT: every append is
@@ -1022,23 +1022,35 @@ __global__ void __launch_bounds__(LAT ? 64 * kLatticeMaxWaves : 64) deep_wave_ke
                 if constexpr (OUT && B > 1)
                     if (act) deep_write_bits<B, KIND>(out_task, V, u, f & (f - 1), dval);
             };
+            // the level's end: the queues swap
+            auto next_level = [&]() {
+                DeepQueue<Entry> tq = qa;
+                qa = qb;
+                qb = tq;
+                n = nn;
+                ++level;
+            };
             if (__builtin_expect(n <= nfast, 1)) {
-                // the usual level: one pass, no spill, appends that fit (no
-                // taken branch on its path but the level loop's own)
-                pass(0u, std::false_type{}, std::true_type{});
-            } else {
-                for (uint32_t base = 0; base < n; base += 64) {
-                    if (base + 64 <= qa.lcap)
-                        pass(base, std::false_type{}, std::false_type{});
-                    else
-                        pass(base, std::true_type{}, std::false_type{});
+                // the usual levels: one pass each (no spill, appends that
+                // fit), in a loop of their own whose only taken branch is its
+                // back edge; it ends at an empty level or a larger one
+                for (;;) {
+                    pass(0u, std::false_type{}, std::true_type{});
+                    next_level();
+                    if (n - 1u >= nfast) break;  // (n == 0 wraps)
+                    if (OUT && level > DMAX && lane == 0) atomicOr(&p.flags[FlagU16Overflow], 1u);
+                    dval = min(level, DMAX);
+                    nn = 0;
                 }
+                continue;
             }
-            DeepQueue<Entry> tq = qa;
-            qa = qb;
-            qb = tq;
-            n = nn;
-            ++level;
+            for (uint32_t base = 0; base < n; base += 64) {
+                if (base + 64 <= qa.lcap)
+                    pass(base, std::false_type{}, std::false_type{});
+                else
+                    pass(base, std::true_type{}, std::false_type{});
+            }
+            next_level();
         }
         if constexpr (LOG)
             if (pend_at != 0xFFFFFFFFu) ((g32_t)out_task)[pend_at + pend_pos] = pend_val;
