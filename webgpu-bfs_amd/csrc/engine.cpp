@@ -219,7 +219,7 @@ int msbfs::graph_create_impl(msbfs_context *c, uint64_t V, uint64_t E, const uin
                 // records; for a lattice small enough for LDS, one host
                 // BFS from vertex 0 gives it now (its eccentricity + 1, at
                 // least half the diameter), so the first traversal is
-                // routed like the later ones (road-like S=2048: 1.8 ms
+                // routed like the later ones (road-like S=2048: 1.6 ms
                 // instead of 3.4 ms in the workgroup kernel).
                 if ((uint64_t)g->lat_words * 4 < (uint64_t)c->lds_max)
                     g->deep_levels.store(host_levels_from(V, ro, ci, 0), std::memory_order_relaxed);
